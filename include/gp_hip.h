@@ -1,0 +1,173 @@
+/*
+ * gp_hip.h -- C ABI of libgp_hip, the MI355X solver-loop library for MicroSim's 2-D grand-potential (GP)
+ * finite-difference model.
+ *
+ * The library replaces the serial driver's solver-loop routines (reference paths relative to the MicroSim tree,
+ * GPs = Grand_potential_Finite_difference_2D_serial):
+ *
+ *   gp_step    <- solverloop_phasefield(start,end); apply_boundary_conditions(0);
+ *                 solverloop_concentration(start,end); apply_boundary_conditions(0);
+ *                 (GPs/microsim_gp.c:102-106, GPs/solverloop/solverloop.h:21-71,
+ *                  GPs/solverloop/boundary_serial.h:176-203)
+ *   gp_smooth  <- smooth(start,end); apply_boundary_conditions(0);      (GPs/microsim_gp.c:97-100,
+ *                 GPs/solverloop/solverloop.h:72-90)
+ *   gp_upload / gp_download
+ *              <- the AoS array `struct fields *gridinfo` the host owns (GPs/functions/global_vars.h:327-332),
+ *                 indexed x*layer_size + y with 3 halo cells per side (GPs/solverloop/serialinfo_xy.h:7-46)
+ *   gp_params  <- every global the loop reads (GPs/functions/global_vars.h, set by
+ *                 GPs/functions/reading_input_parameters.h and init_propertymatrices in
+ *                 GPs/functions/functionF_01.h:169-230)
+ *   gp_stats   <- struct max_min global_max_min (GPs/functions/global_vars.h:130-139), filled in
+ *                 GPs/solverloop/calculate_divergence_concentration.h:128-151
+ *
+ * Conventions
+ *   - Plain C, no exceptions cross the boundary. Every int-returning call returns GP_OK (0) or a negative
+ *     GP_E* code; gp_last_error() gives the message for the last failing call on that handle.
+ *   - Host arrays use the reference's halo-inclusive layout: field-major SoA, each component an array of
+ *     rows_x*rows_y doubles (rows = mesh + 6), index x*rows_y + y (y fastest). Interior cells are x,y in
+ *     [3, mesh+2]. The caller owns host buffers; the library owns its device buffers.
+ *   - One HIP stream per handle. A handle is not thread-safe; distinct handles are independent.
+ *   - Boundary conditions: halo cells are copies of interior cells given by the index maps in gp_params
+ *     (the reference's proxy tables, GPs/functions/utility_functions.h:471-650). Only Neumann (1) and
+ *     periodic (3) sides exist in these maps; the reference's Dirichlet/free sides copy nothing
+ *     (GPs/solverloop/boundary_serial.h:15) and are rejected by the host with an error.
+ */
+#ifndef GP_HIP_H_
+#define GP_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GP_MAXP 6            /* maximum NUMPHASES */
+#define GP_MAXNC 4           /* maximum NUMCOMPONENTS-1 */
+#define GP_HALO 3            /* halo width of the reference layout */
+
+/* status codes */
+#define GP_OK 0
+#define GP_E_ARG (-1)        /* invalid argument / unsupported parameter combination */
+#define GP_E_HIP (-2)        /* HIP runtime error */
+#define GP_E_OOM (-3)        /* device allocation failed */
+#define GP_E_NAN (-4)        /* non-finite value detected by gp_step's checks */
+#define GP_E_IO (-5)         /* file I/O (host library) */
+#define GP_E_NODEV (-6)      /* no usable GPU */
+
+/* precision modes */
+#define GP_FP64 0            /* FP64 storage and arithmetic, same operation order as the reference */
+#define GP_FP32 1            /* FP32 phi/mu storage and stencil arithmetic, compensated increments */
+
+/* closed-form solve of d(mu) = (dc/dmu)^-1 dc (GPs/solverloop/calculate_divergence_concentration.h:93-125) */
+#define GP_SOLVE_LU 0
+#define GP_SOLVE_BINARY 1
+#define GP_SOLVE_TERNARY 2
+#define GP_SOLVE_DILUTE 3
+
+/* Flat snapshot of the globals read by the solver loop. Index conventions:
+ *   pair (a,b)        -> a*GP_MAXP + b
+ *   triple (a,b,c)    -> (a*GP_MAXP + b)*GP_MAXP + c
+ *   [phase][i][j]     -> (phase*GP_MAXNC + i)*GP_MAXNC + j
+ *   [phase][i]        -> phase*GP_MAXNC + i
+ *   rotation (a,b)    -> ((a*GP_MAXP + b)*2 + i)*2 + j   (2-D block of the 3x3 matrix)        */
+typedef struct gp_params {
+  int32_t nphases;             /* NUMPHASES; the last phase is the liquid/matrix */
+  int32_t ncomp;               /* NUMCOMPONENTS; NC = ncomp-1 mu fields */
+  int64_t mesh_x, mesh_y;      /* MESH_X, MESH_Y (interior cells) */
+  double dx, dy, dt;           /* DELTA_X, DELTA_Y, DELTA_t */
+  double epsilon;              /* epsilon */
+  double tau;                  /* tau: returned by FunctionTau when sum phi_a phi_b == 0 */
+  double T, Teq;               /* T (isothermal), Equilibrium_temperature */
+  int32_t fn_w;                /* Function_W: 1 obstacle, 2 well */
+  int32_t fn_aniso;            /* Function_anisotropy: 0 isotropic (A_00), 1 anisotropic (A_01) */
+  int32_t fold;                /* Anisotropy_type (4) */
+  int32_t solve;               /* GP_SOLVE_* from BINARY/TERNARY/DILUTE flags */
+  int32_t jat_skip_row;        /* allocated y row whose y-face anti-trapping flux is skipped:
+                                  MESH_Y-2 in the serial driver (calculate_fluxes_concentration.h:65); -1 none */
+  int32_t reserved0;
+  int32_t halo_src_x[2][6];    /* [field 0 phi / 1 mu][k]: source x of allocated x = 0,1,2, mx+3, mx+4, mx+5 */
+  int32_t halo_src_y[2][6];    /* same for y */
+  double tau_ab[GP_MAXP * GP_MAXP];
+  double gamma[GP_MAXP * GP_MAXP];
+  double gamma_abc[GP_MAXP * GP_MAXP * GP_MAXP];
+  double dab[GP_MAXP * GP_MAXP];
+  double rot[GP_MAXP * GP_MAXP * 4];       /* Rotation_matrix[a][b] (2-D block) */
+  double rot_inv[GP_MAXP * GP_MAXP * 4];   /* Inv_Rotation_matrix[a][b] (2-D block) */
+  double diffusivity[GP_MAXP * GP_MAXNC * GP_MAXNC];
+  double A[GP_MAXP * GP_MAXNC * GP_MAXNC]; /* A[a][i][j] as given (f = sum_{i<=j} A_ij c_i c_j + ...) */
+  double cmu[GP_MAXP * GP_MAXNC * GP_MAXNC]; /* (d mu/d c)^-1 per phase */
+  double B[GP_MAXP * GP_MAXNC];            /* B(T) */
+  double C[GP_MAXP];                       /* C(T) */
+  double Beq[GP_MAXP * GP_MAXNC];          /* B(Teq) */
+  double dBbdT[GP_MAXP * GP_MAXNC];        /* dB/dT */
+} gp_params;
+
+/* Diagnostics. max/min are cumulative from the last gp_upload with the reference's initial values
+ * (phi_max 1, phi_min 0, mu_max = mu_min = 1, GPs/solverloop/serialinfo_xy.h:52-59); sums are over interior
+ * cells of the last step. solute/energy are evaluated on the state after the last step. */
+typedef struct gp_stats {
+  double phi_max[GP_MAXP], phi_min[GP_MAXP];
+  double mu_max[GP_MAXNC], mu_min[GP_MAXNC];
+  double rel_change_phi[GP_MAXP];          /* sum (delta phi_a)^2 */
+  double rel_change_mu[GP_MAXNC];          /* sum (delta mu_k)^2 */
+  double solute[GP_MAXNC];                 /* sum_cells c_k, c_k = sum_a h_a(phi) c_a^k(mu), times dx*dy */
+  double energy;                           /* sum_cells [psi + eps*a(grad phi) + w(phi)/eps] dx*dy */
+  int64_t step;                            /* steps taken since gp_upload */
+  int64_t nonfinite;                       /* interior values that are not finite */
+} gp_stats;
+
+/* AoS cell of the reference host (struct fields, GPs/functions/global_vars.h:327-332) for the adapters. */
+typedef struct gp_cell {
+  double *phia;
+  double *compi;
+  double *deltaphi;
+  double temperature;
+} gp_cell;
+
+typedef struct gp_handle gp_handle;
+
+/* Create a solver for the given parameters on the current HIP device. precision: GP_FP64 or GP_FP32.
+ * Replaces the set-up the loop relies on: serialinfo_xy (GPs/solverloop/serialinfo_xy.h:1-60) for the grid,
+ * initialize_functions_solverloop (GPs/solverloop/initialize_functions_solverloop.h:4-57) for the model choice. */
+int gp_create(const gp_params *params, int precision, gp_handle **out);
+
+/* Copy the state in (halo-inclusive SoA, see above); halos are rebuilt from the interior with the maps,
+ * as apply_boundary_conditions(0) does before the loop (GPs/microsim_gp.c:84). Resets the statistics. */
+int gp_upload(gp_handle *h, const double *phi, const double *mu);
+
+/* Same from the reference's AoS array (ncells = rows_x*rows_y). */
+int gp_upload_fields(gp_handle *h, const gp_cell *cells, int64_t ncells);
+
+/* Copy the state out. dphi may be NULL; it receives the last step's projected delta-phi on the cells the
+ * interior update reads (x,y in [2, mesh+3]) and 0 elsewhere. */
+int gp_download(gp_handle *h, double *phi, double *mu, double *dphi);
+
+/* Same into the reference's AoS array (phia, compi, deltaphi of every cell). */
+int gp_download_fields(gp_handle *h, gp_cell *cells, int64_t ncells);
+
+/* nsteps pre-conditioning steps (the NSMOOTH-1 loop): no driving force, no anti-trapping current. */
+int gp_smooth(gp_handle *h, int64_t nsteps);
+
+/* nsteps explicit time steps (phi sweep, BC, mu sweep, BC). stats may be NULL; when given it receives the
+ * diagnostics of the last step (synchronises the stream). */
+int gp_step(gp_handle *h, int64_t nsteps, gp_stats *stats);
+
+/* Diagnostics of the current state without stepping (max/min, solute, energy; rel_change of the last step). */
+int gp_get_stats(gp_handle *h, gp_stats *stats);
+
+/* Block until all work queued on the handle's stream is done. */
+int gp_synchronize(gp_handle *h);
+
+/* Message of the last failing call on h (or of the last failing gp_create when h is NULL). */
+const char *gp_last_error(const gp_handle *h);
+
+void gp_destroy(gp_handle *h);
+
+/* Library version string, e.g. "gp_hip 0.1 gfx950". */
+const char *gp_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GP_HIP_H_ */

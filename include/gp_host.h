@@ -1,0 +1,108 @@
+/*
+ * gp_host.h -- host side of the drop-in: reads the serial driver's Input.in / Filling.in, builds the initial
+ * state and the boundary index maps, evaluates the FP64 free-energy coefficients, and reads/writes the
+ * driver's VTK files. CPU only (C99 + libm); produces the gp_params consumed by libgp_hip (gp_hip.h).
+ *
+ * Reference routines replaced (GPs = Grand_potential_Finite_difference_2D_serial):
+ *   gp_read_input       <- reading_input_parameters  (GPs/functions/reading_input_parameters.h:1-517)
+ *                          + read_boundary_conditions (GPs/functions/read_boundary_conditions.h:4-57)
+ *   gp_property_matrices<- initialize_variables + init_propertymatrices
+ *                          (GPs/functions/initialize_variables.h:4-20, GPs/functions/functionF_01.h:169-230)
+ *   gp_fill_domain      <- fill_domain + fill_phase_* + fill_composition_cube
+ *                          (GPs/functions/fill_domain.h:4-151, GPs/functions/filling.h:6-270)
+ *   gp_write_vtk        <- writetofile_serial2D[_binary] (GPs/solverloop/file_writer.h:23-255)
+ *   gp_read_vtk         <- readfromfile_serial2D[_binary] (GPs/solverloop/file_writer.h:47-70, 256-416)
+ *   gp_write_echo       <- the <infile>.out / <infile>.bd echo files
+ *                          (GPs/functions/reading_input_parameters.h:310-516, read_boundary_conditions.h:195-207)
+ */
+#ifndef GP_HOST_H_
+#define GP_HOST_H_
+
+#include "gp_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GP_NAMELEN 51
+
+/* Everything Input.in sets. p holds what the solver loop reads; the rest drives the host. */
+typedef struct gp_input {
+  gp_params p;
+  int32_t dimension;
+  int64_t mesh_z;
+  double dz;
+  int64_t ntimesteps, savet, nsmooth, starttime, restart, track_progress;
+  int32_t ascii;                 /* WRITEFORMAT = ASCII */
+  int32_t writecomposition;
+  int32_t isothermal;
+  int32_t noise_phasefield;
+  double amp_noise_phase;
+  int32_t shift;
+  int64_t shiftj;
+  int32_t fn_f;                  /* Function_F */
+  int32_t binary, ternary, dilute;
+  double R, V, Tfill, temperature_scale, tilt_angle, rtheta;
+  char phases[GP_MAXP][GP_NAMELEN];
+  char components[GP_MAXNC + 1][GP_NAMELEN];
+  int32_t max_length;            /* widest phase/component name, for the progress lines */
+  /* thermodynamic tables, [a][b][k] */
+  double ceq[GP_MAXP][GP_MAXP][GP_MAXNC];
+  double cfill[GP_MAXP][GP_MAXP][GP_MAXNC];
+  double slopes[GP_MAXP][GP_MAXP][GP_MAXNC];
+  /* full 3x3 rotation matrices as the reference keeps them (after its in-place LU) */
+  double rot3[GP_MAXP][GP_MAXP][3][3];
+  double rot3_inv[GP_MAXP][GP_MAXP][3][3];
+  /* boundary tables: [side 0 X-,1 X+,2 Y+,3 Y-,4 Z+,5 Z-][field 0 phi,1 mu,2 T] */
+  int32_t bc_type[6][3];
+  int32_t bc_points[6][3][3];
+  int32_t bc_proxy[6][3][3];
+  int32_t bc_proxy_set[6][3];
+  /* bookkeeping of which keys were seen (validation) */
+  uint32_t seen;
+  uint8_t seen_A[GP_MAXP], seen_D[GP_MAXP], seen_rot[GP_MAXP][GP_MAXP];
+  double eigen_dummy;
+} gp_input;
+
+/* Parse Input.in (both passes: parameters, then BOUNDARY / BOUNDARY_VALUE lines), compute the property
+ * matrices and rotation matrices, and fill in->p. Unknown keys are reported on stdout like the reference
+ * ("Unrecongized parameter : ..."), unless quiet. Returns GP_OK or GP_E_ARG/GP_E_IO with err filled. */
+int gp_read_input(const char *input_path, gp_input *in, int quiet, char *err, size_t errlen);
+
+/* Check the parameters are in the supported set (2-D, isothermal, F=1, Neumann/periodic, no noise, no SHIFT). */
+int gp_validate_input(const gp_input *in, char *err, size_t errlen);
+
+/* init_propertymatrices(T) on the host: cmu, B(T), C(T), Beq, dBbdT into in->p. */
+void gp_property_matrices(gp_input *in);
+
+/* Halo-inclusive sizes. */
+int64_t gp_rows_x(const gp_params *p);
+int64_t gp_rows_y(const gp_params *p);
+
+/* fill_domain(argv) into halo-inclusive SoA arrays phi[N][rows], mu[NC][rows] (rows = rows_x*rows_y).
+ * dphi may be NULL. */
+int gp_fill_domain(const gp_input *in, const char *filling_path, double *phi, double *mu, double *dphi,
+                   int quiet, char *err, size_t errlen);
+
+/* Copy halos from the interior with the boundary maps (apply_boundary_conditions(0)). */
+void gp_apply_bc_host(const gp_params *p, double *phi, double *mu);
+
+/* DATA/<out>_<t>.vtk in the serial layout; binary unless in->ascii. */
+int gp_write_vtk(const gp_input *in, const char *path, const double *phi, const double *mu);
+int gp_read_vtk(const gp_input *in, const char *path, double *phi, double *mu);
+
+/* <stem>.out and <stem>.bd echo files. */
+int gp_write_echo(const gp_input *in, const char *input_path);
+
+/* Composition field sum_b c_b(mu) h_b(phi) of one cell (the Composition_<k> output). */
+double gp_composition(const gp_params *p, const double *phi_cell, const double *mu_cell, int k);
+
+/* sizeof checks for foreign-function mirrors of these structs. */
+size_t gp_sizeof_input(void);
+size_t gp_sizeof_params(void);
+size_t gp_sizeof_stats(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GP_HOST_H_ */

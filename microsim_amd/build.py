@@ -1,0 +1,72 @@
+"""In-tree build of the native pieces (no cmake): explicit gcc / hipcc command lines.
+
+    python -m microsim_amd.build            # host library, HIP library, host driver
+    python -m microsim_amd.build --host     # CPU parts only
+
+Outputs go to microsim_amd/lib/: libgp_host.so, libgp_hip.so (gfx950 code object), microsim_gp.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+LIB = os.path.join(HERE, "lib")
+INC = os.path.join(REPO, "include")
+ARCH = os.environ.get("GP_OFFLOAD_ARCH", "gfx950")
+
+HOST_CFLAGS = ["-O2", "-std=gnu11", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-Wno-unused-function"]
+# FP64 parity needs IEEE division/sqrt and no contraction; packed FP32 instructions are switched off.
+HIP_FLAGS = ["-O3", "-fPIC", "-shared", "-std=c++17", "--offload-arch=%s" % ARCH, "-ffp-contract=off",
+             "-fno-fast-math", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wno-unused-result", "-munsafe-fp-atomics"]
+
+
+def _run(cmd, cwd=None):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, cwd=cwd)
+
+
+def hipcc() -> str:
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def build_host():
+    os.makedirs(LIB, exist_ok=True)
+    _run(["gcc"] + HOST_CFLAGS + ["-shared", "-I" + INC, "-o", os.path.join(LIB, "libgp_host.so"),
+                                  os.path.join(HERE, "host", "gp_host.c"), "-lm"])
+
+
+def build_hip():
+    os.makedirs(LIB, exist_ok=True)
+    srcs = [os.path.join(HERE, "csrc", f) for f in sorted(os.listdir(os.path.join(HERE, "csrc")))
+            if f.endswith(".hip")]
+    out = os.path.join(LIB, "libgp_hip.so")
+    tmp = out + ".tmp"
+    _run([hipcc()] + HIP_FLAGS + ["-I" + INC, "-I" + os.path.join(HERE, "csrc"), "-o", tmp] + srcs)
+    os.replace(tmp, out)
+
+
+def build_driver():
+    src = os.path.join(HERE, "host", "microsim_gp.c")
+    if not os.path.exists(src):
+        return
+    _run(["gcc"] + HOST_CFLAGS + ["-I" + INC, "-o", os.path.join(LIB, "microsim_gp"), src,
+                                  "-L" + LIB, "-lgp_host", "-lgp_hip", "-Wl,-rpath,$ORIGIN", "-lm"])
+
+
+def build_all(host_only: bool = False):
+    build_host()
+    if not host_only:
+        build_hip()
+        build_driver()
+
+
+if __name__ == "__main__":
+    build_all(host_only="--host" in sys.argv)

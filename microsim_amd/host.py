@@ -1,0 +1,93 @@
+"""Python face of the host library: Input.in / Filling.in -> (gp_input, initial state) as numpy arrays.
+
+Mirrors what the serial driver does before its time loop (``Grand_potential_Finite_difference_2D_serial/
+microsim_gp.c:42-84``): read the parameters, build the grid, fill the domain (or read the restart snapshot)
+and apply the boundary conditions.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _abi
+
+
+class InputError(RuntimeError):
+    pass
+
+
+@dataclass
+class Case:
+    inp: _abi.GPInput
+    phi: np.ndarray   # [N, rows_x, rows_y]
+    mu: np.ndarray    # [NC, rows_x, rows_y]
+
+    @property
+    def params(self) -> _abi.GPParams:
+        return self.inp.p
+
+    @property
+    def N(self) -> int:
+        return self.inp.p.nphases
+
+    @property
+    def NC(self) -> int:
+        return self.inp.p.ncomp - 1
+
+    @property
+    def rows(self):
+        return int(self.inp.p.mesh_x) + 6, int(self.inp.p.mesh_y) + 6
+
+    def interior(self, arr: np.ndarray) -> np.ndarray:
+        return arr[..., 3:-3, 3:-3]
+
+
+def read_input(input_path: str, quiet: bool = True, validate: bool = True) -> _abi.GPInput:
+    lib = _abi.host_lib()
+    inp = _abi.GPInput()
+    err = C.create_string_buffer(512)
+    rc = lib.gp_read_input(input_path.encode(), C.byref(inp), int(quiet), err, 512)
+    if rc != 0:
+        raise InputError(err.value.decode())
+    if validate:
+        rc = lib.gp_validate_input(C.byref(inp), err, 512)
+        if rc != 0:
+            raise InputError(err.value.decode())
+    return inp
+
+
+def empty_state(inp: _abi.GPInput):
+    rx, ry = int(inp.p.mesh_x) + 6, int(inp.p.mesh_y) + 6
+    N, NC = inp.p.nphases, inp.p.ncomp - 1
+    return np.zeros((N, rx, ry)), np.zeros((max(NC, 0), rx, ry))
+
+
+def load_case(input_path: str, filling_path: str, restart_vtk: str = None, quiet: bool = True) -> Case:
+    """Parameters plus initial state: fill_domain, or the restart snapshot when given; halos applied."""
+    lib = _abi.host_lib()
+    inp = read_input(input_path, quiet=quiet)
+    phi, mu = empty_state(inp)
+    err = C.create_string_buffer(512)
+    if restart_vtk is None:
+        rc = lib.gp_fill_domain(C.byref(inp), filling_path.encode(), _abi.dptr(phi), _abi.dptr(mu), None,
+                                int(quiet), err, 512)
+        if rc != 0:
+            raise InputError(err.value.decode())
+    else:
+        rc = lib.gp_read_vtk(C.byref(inp), restart_vtk.encode(), _abi.dptr(phi), _abi.dptr(mu))
+        if rc != 0:
+            raise InputError("cannot read %s" % restart_vtk)
+    lib.gp_apply_bc_host(C.byref(inp.p), _abi.dptr(phi), _abi.dptr(mu))
+    return Case(inp, phi, mu)
+
+
+def write_vtk(case_inp: _abi.GPInput, path: str, phi: np.ndarray, mu: np.ndarray) -> None:
+    lib = _abi.host_lib()
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    mu = np.ascontiguousarray(mu, dtype=np.float64)
+    rc = lib.gp_write_vtk(C.byref(case_inp), path.encode(), _abi.dptr(phi), _abi.dptr(mu))
+    if rc != 0:
+        raise IOError("cannot write %s" % path)

@@ -1,0 +1,152 @@
+"""Generate golden vectors from the serial reference solver compiled from its own sources.
+
+Run in a container that has the reference tree (``make -C oracle ref`` builds ``oracle/_ref/microsim_gp`` from
+``Grand_potential_Finite_difference_2D_serial/microsim_gp.c``):
+
+    python tests/golden/make_golden.py            # all cases
+    python tests/golden/make_golden.py bin_aniso_neu
+
+For every case this writes ``tests/golden/<case>/``:
+
+* ``Input.in``, ``Filling.in``     -- the inputs given to the reference (and to this build),
+* ``ref.npz``                       -- interior phi[N,MX,MY] and mu[NC,MX,MY] at the listed steps, read from the
+                                       reference's big-endian binary VTK snapshots (full FP64 precision),
+* ``out_0.vtk``                     -- the reference's own t=0 VTK file (byte-exact writer fixture),
+* ``Input.out``, ``Input.bd``       -- the reference's parameter / boundary echo files,
+* ``restart0.npz`` (restart cases)  -- the t=0 state handed to the reference through ``RESTART = 1``.
+
+Only data is stored: no reference source, script or binary.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, REPO)
+from microsim_amd import configs as C  # noqa: E402
+
+REF_BIN = os.path.join(REPO, "oracle", "_ref", "microsim_gp")
+
+
+def _bin_base(mx, my, **kw):
+    inp = C.GPInput(mesh_x=mx, mesh_y=my, nphases=2, ncomp=2, dt=0.05, gamma=[1.0], tau_ab=[0.3], dab=[0.02],
+                    components=["Al", "Cu"], phases=["alpha", "liquid"], epsilon=4.0, tau=0.3, **kw)
+    C.binary_material(inp)
+    return inp
+
+
+def _tern_base(mx, my, seed, N=3, **kw):
+    rng = np.random.default_rng(seed)
+    names = ["alpha", "beta", "gamma", "delta"][: N - 1] + ["liquid"]
+    inp = C.GPInput(mesh_x=mx, mesh_y=my, nphases=N, ncomp=3, dt=0.03, gamma=C.upper(N, 1.0),
+                    tau_ab=C.upper(N, 0.3), dab=C.upper(N, 0.02), gamma_abc=[0.0] * (N * (N - 1) * (N - 2) // 6),
+                    components=["Al", "Cu", "Ni"], phases=names, epsilon=4.0, tau=0.3, **kw)
+    C.ternary_material(inp, rng)
+    return inp, rng
+
+
+def cases():
+    out = {}
+
+    inp = _bin_base(40, 48, bc_phi=(1, 1, 1, 1))
+    out["bin_aniso_neu"] = (inp, [C.Fill("cylinder", 0, (20, 24, 8.0))], None, [0, 1, 2, 10, 100])
+
+    inp = _bin_base(48, 48, bc_phi=(1, 1, 1, 1), restart=1)
+    phi0, mu0 = C.tanh_seed_state(48, 48, 10.0, 2.0, [2.0 * 1.0 * 0.34])
+    out["bin_tanh_restart"] = (inp, [], (phi0, mu0), [0, 1, 10, 200])
+
+    inp, rng = _tern_base(48, 40, 7, bc_phi=(3, 3, 3, 3), dy=1.25, writecomposition=1)
+    fills = [C.Fill("cylinder", 0, (12, 10, 6.0)), C.Fill("cylinder", 1, (34, 28, 7.5)),
+             C.Fill("cylinder", 1, (40, 8, 5.0)), C.Fill("cylinder", 0, (2, 30, 5.0))]
+    out["tern_aniso_per"] = (inp, fills, None, [0, 1, 10, 100])
+
+    inp, rng = _tern_base(40, 40, 11, bc_phi=(3, 3, 1, 1), fn_anisotropy=0)
+    inp.gamma_abc = [2.0]
+    fills = [C.Fill("cylinder", 0, (14, 20, 7.0)), C.Fill("cylinder", 1, (24, 20, 7.0))]
+    out["tern_iso_mixed"] = (inp, fills, None, [0, 1, 10, 100])
+
+    inp = _bin_base(36, 44, bc_phi=(1, 1, 3, 3), fn_w=2)
+    inp.rotations = {(0, 1): (0.0, 0.0, 30.0)}
+    inp.dab = [0.04]
+    out["bin_well_rot"] = (inp, [C.Fill("cylinder", 0, (18, 22, 9.0))], None, [0, 1, 10, 100])
+
+    inp, rng = _tern_base(36, 36, 21, bc_phi=(1, 1, 1, 1), solve="LU", T=0.98)
+    fills = [C.Fill("cylinder", 0, (12, 12, 6.0)), C.Fill("cylinder", 1, (24, 22, 6.0))]
+    out["tern_lu_undercool"] = (inp, fills, None, [0, 1, 10, 60])
+
+    inp, rng = _tern_base(36, 36, 23, bc_phi=(3, 3, 3, 3), solve="DILUTE")
+    fills = [C.Fill("cylinder", 1, (18, 18, 8.0))]
+    out["tern_dilute"] = (inp, fills, None, [0, 1, 10, 60])
+
+    inp = _bin_base(40, 40, bc_phi=(1, 1, 1, 1), nsmooth=6)
+    out["bin_smooth"] = (inp, [C.Fill("cylinder", 0, (20, 20, 9.0))], None, [0, 1, 10, 40])
+
+    inp, rng = _tern_base(44, 44, 31, N=4, bc_phi=(3, 3, 1, 1))
+    inp.gamma_abc = [1.5, 0.5, 1.0, 2.0]
+    inp.dt = 0.02
+    fills = [C.Fill("cylinder", 0, (12, 12, 7.0)), C.Fill("cylinder", 1, (30, 14, 7.0)),
+             C.Fill("cylinder", 2, (21, 30, 8.0))]
+    out["quad_k3"] = (inp, fills, None, [0, 1, 10, 50])
+
+    inp = _bin_base(32, 36, bc_phi=(1, 1, 1, 1), fn_anisotropy=0, writeformat="ASCII")
+    out["bin_iso_cube_ascii"] = (inp, [C.Fill("cube", 0, (0, 0, 31, 8))], None, [0, 1])
+
+    inp, fills, _ = C.config_ternary(96, 4, seed=5, steps=200, savet=1, min_spacing=30.0)
+    out["tern_c1_mini"] = (inp, fills, None, [0, 1, 10, 200])
+    return out
+
+
+def run_case(name, inp, fills, restart, steps, dest):
+    inp.ntimesteps = max(steps)
+    inp.savet = 1
+    inp.track_progress = max(1, max(steps))
+    work = tempfile.mkdtemp(prefix="gp_golden_")
+    try:
+        rs = None
+        if restart is not None:
+            rs = (restart[0], restart[1], "out")
+        C.write_case(work, inp, fills, restart_state=rs)
+        r = subprocess.run([REF_BIN, "Input.in", "Filling.in", "out"], cwd=work, capture_output=True, text=True,
+                           timeout=600)
+        if r.returncode != 0:
+            raise RuntimeError("reference failed for %s: %s" % (name, r.stderr[-2000:]))
+        os.makedirs(dest, exist_ok=True)
+        for fn in ("Input.in", "Filling.in", "Input.out", "Input.bd"):
+            shutil.copy(os.path.join(work, fn), os.path.join(dest, fn))
+        shutil.copy(os.path.join(work, "DATA", "out_0.vtk"), os.path.join(dest, "out_0.vtk"))
+        N, K = inp.nphases, inp.ncomp
+        phis, mus = [], []
+        for s in steps:
+            p, m = C.read_vtk(os.path.join(work, "DATA", "out_%d.vtk" % s), N, K, inp.mesh_x, inp.mesh_y)
+            phis.append(p)
+            mus.append(m)
+        np.savez_compressed(os.path.join(dest, "ref.npz"), steps=np.array(steps, dtype=np.int64),
+                            phi=np.stack(phis), mu=np.stack(mus))
+        if restart is not None:
+            np.savez_compressed(os.path.join(dest, "restart0.npz"), phi=restart[0], mu=restart[1])
+        with open(os.path.join(dest, "stdout.txt"), "w") as f:
+            f.write(r.stdout)
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
+def main(argv):
+    if not os.path.exists(REF_BIN):
+        sys.exit("build the reference first: make -C oracle ref")
+    allc = cases()
+    names = argv[1:] or sorted(allc)
+    for n in names:
+        inp, fills, restart, steps = allc[n]
+        run_case(n, inp, fills, restart, steps, os.path.join(HERE, n))
+        print("golden:", n, steps)
+
+
+if __name__ == "__main__":
+    main(sys.argv)
