@@ -167,6 +167,19 @@ void gp_destroy(gp_handle *h);
 /* Library version string, e.g. "gp_hip 0.1 gfx950". */
 const char *gp_version(void);
 
+/* Measurement support (no counterpart in the reference; for benchmarks and profilers). While enabled, every
+ * kernel launch issued by gp_step / gp_smooth is bracketed by HIP events recorded on the handle's stream.
+ * gp_profile_read synchronises, returns the mean duration in ms and the launch count per kernel kind
+ * (arrays of GP_NKERNELS) accumulated since the previous read, and resets them. */
+#define GP_KERNEL_PHI 0      /* delta-phi kernel */
+#define GP_KERNEL_MU 1       /* anti-trapping + mu/phi update kernel */
+#define GP_KERNEL_STATS 2    /* statistics fold */
+#define GP_KERNEL_FUSED 3    /* single-pass step kernel */
+#define GP_NKERNELS 4
+int gp_profile_enable(gp_handle *h, int enable);
+int gp_profile_read(gp_handle *h, double *avg_ms, int64_t *count);
+const char *gp_kernel_name(int kind);
+
 #ifdef __cplusplus
 }
 #endif

@@ -29,6 +29,8 @@ GP_SOLVE_BINARY = 1
 GP_SOLVE_TERNARY = 2
 GP_SOLVE_DILUTE = 3
 
+GP_NKERNELS = 4
+
 P, NCM = GP_MAXP, GP_MAXNC
 
 
@@ -134,6 +136,10 @@ def _declare_hip(lib):
     lib.gp_destroy.restype = None
     lib.gp_version.argtypes = []
     lib.gp_version.restype = C.c_char_p
+    lib.gp_profile_enable.argtypes = [h, C.c_int]
+    lib.gp_profile_read.argtypes = [h, _dp, C.POINTER(C.c_int64)]
+    lib.gp_kernel_name.argtypes = [C.c_int]
+    lib.gp_kernel_name.restype = C.c_char_p
 
 
 _host = None

@@ -1,0 +1,659 @@
+/*
+ * gp_hip.hip -- libgp_hip: the C ABI of include/gp_hip.h over hand-written CDNA4 (gfx950) kernels.
+ *
+ * Device layout (one handle):
+ *   every field component is a plane of rows_x = mesh_x+6 rows; a row holds the halo-inclusive y range
+ *   0..mesh_y+5 starting at element kYOff, so the first interior cell y = 3 sits on a 64-element boundary;
+ *   the row pitch ps is a multiple of 64 elements. phi (N planes) and mu (NC planes) are ping-ponged,
+ *   dphi (N planes) holds the last step's projected delta-phi.
+ * A step is two launches:
+ *   k_phi : delta-phi on x,y in [2, mesh+3] (the interior plus the one-cell ring the update reads),
+ *   k_mu  : anti-trapping currents, mu update, phi += delta-phi on the interior, halo images written
+ *           directly (each boundary cell scatters itself to the halo cells whose proxy it is), per-block
+ *           statistics partials;
+ * followed by a one-block reduction of the partials into the cumulative statistics.
+ * Reference routines replaced: see include/gp_hip.h.
+ */
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "gp_hip.h"
+#include "gp_model.h"
+
+using namespace gpk;
+
+namespace {
+
+constexpr int kYOff = 61;        /* element offset of y = 0 inside a row: y = 3 lands on a 64-element boundary */
+constexpr int kBX = 64;          /* threads along y (one wavefront) */
+constexpr int kBY = 4;           /* rows per block */
+constexpr int kNStatMax = 3 * GP_MAXP + 3 * GP_MAXNC + 1;
+
+struct Maps {
+  int32_t sx[2][6], sy[2][6];
+  int32_t hx[6], hy[6];
+};
+
+char g_create_error[512];
+
+}  // namespace
+
+struct gp_handle {
+  gp_params p;
+  int precision;
+  int N, NC;
+  int64_t rx, ry, ps, fs;   /* rows, row pitch (elements), plane size (elements) */
+  hipStream_t stream;
+  int device;
+  void *phi[2], *mu[2], *dphi;
+  void *model;              /* Model<R> in device memory */
+  double *partials;         /* [nblocks][kNStatMax] */
+  double *stats_dev;        /* cumulative stats (kNStatMax) */
+  int cur;
+  int64_t nblocks_mu;
+  int64_t steps;
+  Maps maps;
+  bool profiling;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof;
+  char err[512];
+};
+
+/* bracket one launch with events when profiling (kind = GP_KERNEL_*) */
+struct ProfScope {
+  gp_handle *h;
+  hipEvent_t e0, e1;
+  int kind;
+  ProfScope(gp_handle *hh, int k) : h(hh), e0(nullptr), e1(nullptr), kind(k) {
+    if (h->profiling && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess)
+      hipEventRecord(e0, h->stream);
+  }
+  ~ProfScope() {
+    if (e1) {
+      hipEventRecord(e1, h->stream);
+      h->prof.push_back({kind, {e0, e1}});
+    }
+  }
+};
+
+static int fail(gp_handle *h, int code, const char *fmt, ...) {
+  char *dst = h ? h->err : g_create_error;
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(dst, 512, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIPCHK(h, call)                                                                         \
+  do {                                                                                          \
+    hipError_t e_ = (call);                                                                     \
+    if (e_ != hipSuccess) return fail((h), GP_E_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+/* ------------------------------------------------------------------------------------------------ kernels */
+
+template <int N, int NC, typename R, bool SMOOTH>
+__global__ __launch_bounds__(kBX *kBY) void k_phi(const Model<R> *__restrict__ mp, const R *__restrict__ phi,
+                                                  const R *__restrict__ mu, R *__restrict__ dphi, int64_t fs,
+                                                  int64_t ps, int64_t mx, int64_t my) {
+  const int64_t y = 2 + (int64_t)blockIdx.x * kBX + threadIdx.x;
+  const int64_t x = 2 + (int64_t)blockIdx.y * kBY + threadIdx.y;
+  if (x > mx + 3 || y > my + 3) return;
+  const Model<R> &m = *mp;
+  const int64_t c = x * ps + y + kYOff;
+  View<R> v{phi, mu, nullptr, fs, ps, c};
+  R d[N];
+  phi_delta<N, NC>(m, v, SMOOTH, d);
+  for (int a = 0; a < N; a++) dphi[a * fs + c] = d[a];
+}
+
+template <typename R>
+__device__ __forceinline__ void write_images(R *__restrict__ base, int ncomp, const R *vals, int f, const Maps &mp,
+                                             int64_t x, int64_t y, int64_t fs, int64_t ps) {
+  int64_t X[3], Y[3];
+  int nx = 1, ny = 1;
+  X[0] = x;
+  Y[0] = y;
+  for (int k = 0; k < 6; k++) {
+    if (mp.sx[f][k] == x && nx < 3) X[nx++] = mp.hx[k];
+    if (mp.sy[f][k] == y && ny < 3) Y[ny++] = mp.hy[k];
+  }
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++) {
+      if (i == 0 && j == 0) continue;
+      const int64_t c = X[i] * ps + Y[j] + kYOff;
+      for (int a = 0; a < ncomp; a++) base[a * fs + c] = vals[a];
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
+template <typename T>
+__device__ __forceinline__ T wave_min(T v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
+  return v;
+}
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+template <int N, int NC, typename R, bool SMOOTH>
+__global__ __launch_bounds__(kBX *kBY) void k_mu(const Model<R> *__restrict__ mp, const R *__restrict__ phi,
+                                                 const R *__restrict__ mu, const R *__restrict__ dphi,
+                                                 R *__restrict__ phi_o, R *__restrict__ mu_o, Maps maps,
+                                                 double *__restrict__ partials, int64_t fs, int64_t ps, int64_t mx,
+                                                 int64_t my) {
+  const int64_t y = 3 + (int64_t)blockIdx.x * kBX + threadIdx.x;
+  const int64_t x = 3 + (int64_t)blockIdx.y * kBY + threadIdx.y;
+  const bool live = (x <= mx + 2 && y <= my + 2);
+  constexpr int NS = 3 * N + 3 * NC + 1;
+  double st[NS];
+  for (int a = 0; a < N; a++) {
+    st[a] = -INFINITY;
+    st[N + a] = INFINITY;
+    st[2 * N + 2 * NC + a] = 0.0;
+  }
+  for (int k = 0; k < NC; k++) {
+    st[2 * N + k] = -INFINITY;
+    st[2 * N + NC + k] = INFINITY;
+    st[3 * N + 2 * NC + k] = 0.0;
+  }
+  st[NS - 1] = 0.0;
+  if (live) {
+    const Model<R> &m = *mp;
+    const int64_t c = x * ps + y + kYOff;
+    View<R> v{phi, mu, dphi, fs, ps, c};
+    R jx0[NC], jxb[NC], jy0[NC], jyl[NC], d[N], pn[N], mn[NC], dm[NC];
+    if (!SMOOTH) {
+      jat_x<N, NC>(m, v, 0, 0, jx0);
+      jat_x<N, NC>(m, v, -1, 0, jxb);
+      jat_y<N, NC>(m, v, 0, 0, y, jy0);
+      jat_y<N, NC>(m, v, 0, -1, y - 1, jyl);
+    } else {
+      for (int k = 0; k < NC; k++) jx0[k] = jxb[k] = jy0[k] = jyl[k] = R(0);
+    }
+    for (int a = 0; a < N; a++) d[a] = dphi[a * fs + c];
+    mu_update<N, NC>(m, v, SMOOTH, d, jx0, jxb, jy0, jyl, pn, mn, dm);
+    for (int a = 0; a < N; a++) phi_o[a * fs + c] = pn[a];
+    for (int k = 0; k < NC; k++) mu_o[k * fs + c] = mn[k];
+    if (x < 7 || x > mx - 2 || y < 7 || y > my - 2) {
+      write_images(phi_o, N, pn, 0, maps, x, y, fs, ps);
+      write_images(mu_o, NC, mn, 1, maps, x, y, fs, ps);
+    }
+    for (int a = 0; a < N; a++) {
+      double pv = (double)pn[a], dv = (double)d[a];
+      st[a] = pv;
+      st[N + a] = pv;
+      st[2 * N + 2 * NC + a] = dv * dv;
+      if (!isfinite(pv)) st[NS - 1] += 1.0;
+    }
+    for (int k = 0; k < NC; k++) {
+      double mv = (double)mn[k], dv = (double)dm[k];
+      st[2 * N + k] = mv;
+      st[2 * N + NC + k] = mv;
+      st[3 * N + 2 * NC + k] = dv * dv;
+      if (!isfinite(mv)) st[NS - 1] += 1.0;
+    }
+  }
+  /* block reduction: wave shuffles, then the kBY waves through LDS */
+  __shared__ double red[kBY][NS];
+  for (int i = 0; i < NS; i++) {
+    double r;
+    if (i < N || (i >= 2 * N && i < 2 * N + NC)) r = wave_max(st[i]);
+    else if (i < 2 * N + 2 * NC) r = wave_min(st[i]);
+    else r = wave_sum(st[i]);
+    if (threadIdx.x == 0) red[threadIdx.y][i] = r;
+  }
+  __syncthreads();
+  if (threadIdx.y == 0 && threadIdx.x < NS) {
+    const int i = threadIdx.x;
+    double r = red[0][i];
+    for (int w = 1; w < kBY; w++) {
+      if (i < N || (i >= 2 * N && i < 2 * N + NC)) r = fmax(r, red[w][i]);
+      else if (i < 2 * N + 2 * NC) r = fmin(r, red[w][i]);
+      else r += red[w][i];
+    }
+    const int64_t bid = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+    partials[bid * kNStatMax + i] = r;
+  }
+}
+
+/* one block: fold the per-block partials (fixed order) into the cumulative statistics */
+template <int N, int NC>
+__global__ __launch_bounds__(256) void k_stats(const double *__restrict__ partials, int64_t nblocks,
+                                               double *__restrict__ cum) {
+  constexpr int NS = 3 * N + 3 * NC + 1;
+  __shared__ double red[256];
+  for (int i = 0; i < NS; i++) {
+    const bool mx = (i < N || (i >= 2 * N && i < 2 * N + NC));
+    const bool mn = !mx && i < 2 * N + 2 * NC;
+    double r = mx ? -INFINITY : mn ? INFINITY : 0.0;
+    for (int64_t b = threadIdx.x; b < nblocks; b += 256) {
+      double v = partials[b * kNStatMax + i];
+      r = mx ? fmax(r, v) : mn ? fmin(r, v) : r + v;
+    }
+    red[threadIdx.x] = r;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (threadIdx.x < s) {
+        double o = red[threadIdx.x + s];
+        red[threadIdx.x] = mx ? fmax(red[threadIdx.x], o) : mn ? fmin(red[threadIdx.x], o) : red[threadIdx.x] + o;
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      double v = red[0];
+      if (mx) cum[i] = fmax(cum[i], v);
+      else if (mn) cum[i] = fmin(cum[i], v);
+      else if (i < NS - 1) cum[i] = v;
+      else cum[i] += v;
+    }
+    __syncthreads();
+  }
+}
+
+/* ------------------------------------------------------------------------------------------------ dispatch */
+
+template <int N, int NC, typename R>
+static int launch_step(gp_handle *h, bool smooth) {
+  const R *phi = (const R *)h->phi[h->cur], *mu = (const R *)h->mu[h->cur];
+  R *phi_o = (R *)h->phi[1 - h->cur], *mu_o = (R *)h->mu[1 - h->cur];
+  R *dphi = (R *)h->dphi;
+  const Model<R> *m = (const Model<R> *)h->model;
+  const int64_t mx = h->p.mesh_x, my = h->p.mesh_y;
+  dim3 blk(kBX, kBY);
+  dim3 g1((unsigned)((my + 2 + kBX - 1) / kBX), (unsigned)((mx + 2 + kBY - 1) / kBY));
+  dim3 g2((unsigned)((my + kBX - 1) / kBX), (unsigned)((mx + kBY - 1) / kBY));
+  if (smooth) {
+    {
+      ProfScope ps(h, GP_KERNEL_PHI);
+      hipLaunchKernelGGL((k_phi<N, NC, R, true>), g1, blk, 0, h->stream, m, phi, mu, dphi, h->fs, h->ps, mx, my);
+    }
+    {
+      ProfScope ps(h, GP_KERNEL_MU);
+      hipLaunchKernelGGL((k_mu<N, NC, R, true>), g2, blk, 0, h->stream, m, phi, mu, dphi, phi_o, mu_o, h->maps,
+                         h->partials, h->fs, h->ps, mx, my);
+    }
+  } else {
+    {
+      ProfScope ps(h, GP_KERNEL_PHI);
+      hipLaunchKernelGGL((k_phi<N, NC, R, false>), g1, blk, 0, h->stream, m, phi, mu, dphi, h->fs, h->ps, mx, my);
+    }
+    {
+      ProfScope ps(h, GP_KERNEL_MU);
+      hipLaunchKernelGGL((k_mu<N, NC, R, false>), g2, blk, 0, h->stream, m, phi, mu, dphi, phi_o, mu_o, h->maps,
+                         h->partials, h->fs, h->ps, mx, my);
+    }
+    {
+      ProfScope ps(h, GP_KERNEL_STATS);
+      hipLaunchKernelGGL((k_stats<N, NC>), dim3(1), dim3(256), 0, h->stream, h->partials, h->nblocks_mu,
+                         h->stats_dev);
+    }
+  }
+  HIPCHK(h, hipGetLastError());
+  h->cur = 1 - h->cur;
+  return GP_OK;
+}
+
+typedef int (*step_fn)(gp_handle *, bool);
+
+template <typename R>
+static step_fn pick(int N, int NC) {
+#define GP_CASE(a, b) \
+  if (N == a && NC == b) return launch_step<a, b, R>;
+  GP_CASE(2, 1) GP_CASE(3, 2) GP_CASE(3, 1) GP_CASE(2, 2) GP_CASE(4, 2) GP_CASE(4, 1) GP_CASE(3, 3)
+#undef GP_CASE
+  return nullptr;
+}
+
+static step_fn pick_any(const gp_handle *h) {
+  return h->precision == GP_FP64 ? pick<double>(h->N, h->NC) : pick<float>(h->N, h->NC);
+}
+
+/* ------------------------------------------------------------------------------------------------ helpers */
+
+static size_t esize(const gp_handle *h) { return h->precision == GP_FP64 ? sizeof(double) : sizeof(float); }
+
+static void stats_reset_host(gp_handle *h, double *s) {
+  const int N = h->N, NC = h->NC;
+  for (int a = 0; a < N; a++) {
+    s[a] = 1.0;      /* phi_max */
+    s[N + a] = 0.0;  /* phi_min */
+    s[2 * N + 2 * NC + a] = 0.0;
+  }
+  for (int k = 0; k < NC; k++) {
+    s[2 * N + k] = 1.0;       /* mu_max */
+    s[2 * N + NC + k] = 1.0;  /* mu_min */
+    s[3 * N + 2 * NC + k] = 0.0;
+  }
+  s[3 * N + 3 * NC] = 0.0;
+}
+
+/* halo-inclusive host planes (rows_x*rows_y, y fastest) -> device planes, with the BC maps applied */
+static int upload_planes(gp_handle *h, void *dst, const double *src, int ncomp, int field) {
+  const int64_t rx = h->rx, ry = h->ry, n = rx * ry;
+  std::vector<double> tmp(src, src + (size_t)ncomp * n);
+  const int64_t hx[6] = {0, 1, 2, rx - 3, rx - 2, rx - 1}, hy[6] = {0, 1, 2, ry - 3, ry - 2, ry - 1};
+  for (int c = 0; c < ncomp; c++) {
+    double *v = tmp.data() + (size_t)c * n;
+    for (int s = 0; s < 6; s++)
+      for (int64_t y = 0; y < ry; y++) v[hx[s] * ry + y] = v[(int64_t)h->p.halo_src_x[field][s] * ry + y];
+    for (int s = 0; s < 6; s++)
+      for (int64_t x = 0; x < rx; x++) v[x * ry + hy[s]] = v[x * ry + h->p.halo_src_y[field][s]];
+  }
+  const size_t es = esize(h);
+  std::vector<char> stage((size_t)ncomp * h->fs * es, 0);
+  for (int c = 0; c < ncomp; c++)
+    for (int64_t x = 0; x < rx; x++)
+      for (int64_t y = 0; y < ry; y++) {
+        const double v = tmp[(size_t)c * n + x * ry + y];
+        const size_t o = ((size_t)c * h->fs + x * h->ps + y + kYOff);
+        if (h->precision == GP_FP64) ((double *)stage.data())[o] = v;
+        else ((float *)stage.data())[o] = (float)v;
+      }
+  HIPCHK(h, hipMemcpyAsync(dst, stage.data(), stage.size(), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return GP_OK;
+}
+
+static int download_planes(gp_handle *h, double *dst, const void *src, int ncomp) {
+  const int64_t rx = h->rx, ry = h->ry, n = rx * ry;
+  const size_t es = esize(h);
+  std::vector<char> stage((size_t)ncomp * h->fs * es);
+  HIPCHK(h, hipMemcpyAsync(stage.data(), src, stage.size(), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (int c = 0; c < ncomp; c++)
+    for (int64_t x = 0; x < rx; x++)
+      for (int64_t y = 0; y < ry; y++) {
+        const size_t o = ((size_t)c * h->fs + x * h->ps + y + kYOff);
+        dst[(size_t)c * n + x * ry + y] =
+            h->precision == GP_FP64 ? ((double *)stage.data())[o] : (double)((float *)stage.data())[o];
+      }
+  return GP_OK;
+}
+
+static int validate(const gp_params *p, char *err) {
+  if (p->nphases < 2 || p->nphases > GP_MAXP) return snprintf(err, 512, "nphases %d outside [2,%d]", p->nphases, GP_MAXP), GP_E_ARG;
+  if (p->ncomp < 2 || p->ncomp - 1 > GP_MAXNC) return snprintf(err, 512, "ncomp %d outside [2,%d]", p->ncomp, GP_MAXNC + 1), GP_E_ARG;
+  if (p->mesh_x < 4 || p->mesh_y < 4) return snprintf(err, 512, "mesh must be at least 4x4"), GP_E_ARG;
+  if (p->mesh_x > (1 << 20) || p->mesh_y > (1 << 20)) return snprintf(err, 512, "mesh too large"), GP_E_ARG;
+  if (p->fn_w != 1 && p->fn_w != 2) return snprintf(err, 512, "fn_w must be 1 (obstacle) or 2 (well)"), GP_E_ARG;
+  if (p->fn_aniso != 0 && p->fn_aniso != 1) return snprintf(err, 512, "fn_aniso must be 0 or 1"), GP_E_ARG;
+  if (p->fn_aniso == 1 && p->fold != 4) return snprintf(err, 512, "anisotropy type %d not supported (4 only)", p->fold), GP_E_ARG;
+  if (!(p->dx > 0 && p->dy > 0 && p->dt > 0 && p->epsilon > 0)) return snprintf(err, 512, "dx, dy, dt, epsilon must be > 0"), GP_E_ARG;
+  if (p->solve < 0 || p->solve > 3) return snprintf(err, 512, "unknown solve mode %d", p->solve), GP_E_ARG;
+  if (p->solve == GP_SOLVE_BINARY && p->ncomp != 2) return snprintf(err, 512, "BINARY solve needs ncomp = 2"), GP_E_ARG;
+  if (p->solve == GP_SOLVE_TERNARY && p->ncomp != 3) return snprintf(err, 512, "TERNARY solve needs ncomp = 3"), GP_E_ARG;
+  const int64_t ex[2] = {p->mesh_x + 2, p->mesh_y + 2};
+  for (int f = 0; f < 2; f++)
+    for (int k = 0; k < 6; k++) {
+      if (p->halo_src_x[f][k] < 3 || p->halo_src_x[f][k] > ex[0] || p->halo_src_y[f][k] < 3 || p->halo_src_y[f][k] > ex[1])
+        return snprintf(err, 512, "halo map of field %d entry %d points outside the interior", f, k), GP_E_ARG;
+    }
+  return GP_OK;
+}
+
+/* ------------------------------------------------------------------------------------------------ C ABI */
+
+extern "C" {
+
+const char *gp_version(void) { return "gp_hip 0.1 gfx950"; }
+
+const char *gp_last_error(const gp_handle *h) { return h ? h->err : g_create_error; }
+
+int gp_create(const gp_params *params, int precision, gp_handle **out) {
+  if (!params || !out) return fail(nullptr, GP_E_ARG, "null argument");
+  *out = nullptr;
+  int rc = validate(params, g_create_error);
+  if (rc) return rc;
+  if (precision != GP_FP64 && precision != GP_FP32) return fail(nullptr, GP_E_ARG, "unknown precision %d", precision);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(nullptr, GP_E_NODEV, "no HIP device");
+  gp_handle *h = new gp_handle();
+  memset(h->err, 0, sizeof h->err);
+  h->p = *params;
+  h->precision = precision;
+  h->N = params->nphases;
+  h->NC = params->ncomp - 1;
+  h->rx = params->mesh_x + 6;
+  h->ry = params->mesh_y + 6;
+  h->ps = ((h->ry + kYOff + 63) / 64) * 64;
+  h->fs = h->rx * h->ps;
+  h->cur = 0;
+  h->steps = 0;
+  if (!pick_any(h)) {
+    delete h;
+    return fail(nullptr, GP_E_ARG, "no kernel instantiated for %d phases / %d components", params->nphases, params->ncomp);
+  }
+  const int64_t hx[6] = {0, 1, 2, h->rx - 3, h->rx - 2, h->rx - 1}, hy[6] = {0, 1, 2, h->ry - 3, h->ry - 2, h->ry - 1};
+  for (int f = 0; f < 2; f++)
+    for (int k = 0; k < 6; k++) {
+      h->maps.sx[f][k] = params->halo_src_x[f][k];
+      h->maps.sy[f][k] = params->halo_src_y[f][k];
+    }
+  for (int k = 0; k < 6; k++) {
+    h->maps.hx[k] = (int32_t)hx[k];
+    h->maps.hy[k] = (int32_t)hy[k];
+  }
+  hipGetDevice(&h->device);
+  h->nblocks_mu = ((params->mesh_y + kBX - 1) / kBX) * ((params->mesh_x + kBY - 1) / kBY);
+  const size_t es = precision == GP_FP64 ? 8 : 4;
+  const size_t pn = (size_t)h->N * h->fs * es, mn = (size_t)h->NC * h->fs * es;
+  hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  bool ok = e == hipSuccess;
+  ok = ok && hipMalloc(&h->phi[0], pn) == hipSuccess && hipMalloc(&h->phi[1], pn) == hipSuccess;
+  ok = ok && hipMalloc(&h->mu[0], mn) == hipSuccess && hipMalloc(&h->mu[1], mn) == hipSuccess;
+  ok = ok && hipMalloc(&h->dphi, pn) == hipSuccess;
+  ok = ok && hipMalloc((void **)&h->partials, sizeof(double) * kNStatMax * h->nblocks_mu) == hipSuccess;
+  ok = ok && hipMalloc((void **)&h->stats_dev, sizeof(double) * kNStatMax) == hipSuccess;
+  const size_t msize = precision == GP_FP64 ? sizeof(Model<double>) : sizeof(Model<float>);
+  ok = ok && hipMalloc(&h->model, msize) == hipSuccess;
+  if (!ok) {
+    gp_destroy(h);
+    return fail(nullptr, GP_E_OOM, "device allocation failed (%zu bytes per phi copy)", pn);
+  }
+  ok = hipMemsetAsync(h->phi[0], 0, pn, h->stream) == hipSuccess && hipMemsetAsync(h->phi[1], 0, pn, h->stream) == hipSuccess &&
+       hipMemsetAsync(h->mu[0], 0, mn, h->stream) == hipSuccess && hipMemsetAsync(h->mu[1], 0, mn, h->stream) == hipSuccess &&
+       hipMemsetAsync(h->dphi, 0, pn, h->stream) == hipSuccess;
+  if (precision == GP_FP64) {
+    Model<double> m;
+    make_model(params, &m);
+    ok = ok && hipMemcpyAsync(h->model, &m, sizeof m, hipMemcpyHostToDevice, h->stream) == hipSuccess;
+    ok = ok && hipStreamSynchronize(h->stream) == hipSuccess;
+  } else {
+    Model<float> m;
+    make_model(params, &m);
+    ok = ok && hipMemcpyAsync(h->model, &m, sizeof m, hipMemcpyHostToDevice, h->stream) == hipSuccess;
+    ok = ok && hipStreamSynchronize(h->stream) == hipSuccess;
+  }
+  if (!ok) {
+    gp_destroy(h);
+    return fail(nullptr, GP_E_HIP, "device initialisation failed");
+  }
+  double s[kNStatMax] = {0};
+  stats_reset_host(h, s);
+  if (hipMemcpy(h->stats_dev, s, sizeof s, hipMemcpyHostToDevice) != hipSuccess) {
+    gp_destroy(h);
+    return fail(nullptr, GP_E_HIP, "device initialisation failed");
+  }
+  *out = h;
+  return GP_OK;
+}
+
+void gp_destroy(gp_handle *h) {
+  if (!h) return;
+  if (h->stream) hipStreamSynchronize(h->stream);
+  for (auto &r : h->prof) {
+    hipEventDestroy(r.second.first);
+    hipEventDestroy(r.second.second);
+  }
+  h->prof.clear();
+  for (int i = 0; i < 2; i++) {
+    if (h->phi[i]) hipFree(h->phi[i]);
+    if (h->mu[i]) hipFree(h->mu[i]);
+  }
+  if (h->dphi) hipFree(h->dphi);
+  if (h->partials) hipFree(h->partials);
+  if (h->stats_dev) hipFree(h->stats_dev);
+  if (h->model) hipFree(h->model);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+}
+
+int gp_upload(gp_handle *h, const double *phi, const double *mu) {
+  if (!h || !phi || !mu) return h ? fail(h, GP_E_ARG, "null argument") : GP_E_ARG;
+  int rc = upload_planes(h, h->phi[h->cur], phi, h->N, 0);
+  if (rc) return rc;
+  rc = upload_planes(h, h->mu[h->cur], mu, h->NC, 1);
+  if (rc) return rc;
+  /* the other copy starts identical so its halo rows are defined */
+  HIPCHK(h, hipMemcpyAsync(h->phi[1 - h->cur], h->phi[h->cur], (size_t)h->N * h->fs * esize(h), hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->mu[1 - h->cur], h->mu[h->cur], (size_t)h->NC * h->fs * esize(h), hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(h, hipMemsetAsync(h->dphi, 0, (size_t)h->N * h->fs * esize(h), h->stream));
+  double s[kNStatMax] = {0};
+  stats_reset_host(h, s);
+  HIPCHK(h, hipMemcpyAsync(h->stats_dev, s, sizeof s, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->steps = 0;
+  return GP_OK;
+}
+
+int gp_download(gp_handle *h, double *phi, double *mu, double *dphi) {
+  if (!h) return GP_E_ARG;
+  int rc;
+  if (phi && (rc = download_planes(h, phi, h->phi[h->cur], h->N))) return rc;
+  if (mu && (rc = download_planes(h, mu, h->mu[h->cur], h->NC))) return rc;
+  if (dphi && (rc = download_planes(h, dphi, h->dphi, h->N))) return rc;
+  return GP_OK;
+}
+
+int gp_upload_fields(gp_handle *h, const gp_cell *cells, int64_t ncells) {
+  if (!h || !cells) return h ? fail(h, GP_E_ARG, "null argument") : GP_E_ARG;
+  const int64_t n = h->rx * h->ry;
+  if (ncells != n) return fail(h, GP_E_ARG, "ncells %lld != rows_x*rows_y %lld", (long long)ncells, (long long)n);
+  std::vector<double> phi((size_t)h->N * n), mu((size_t)h->NC * n);
+  for (int64_t i = 0; i < n; i++) {
+    for (int a = 0; a < h->N; a++) phi[(size_t)a * n + i] = cells[i].phia[a];
+    for (int k = 0; k < h->NC; k++) mu[(size_t)k * n + i] = cells[i].compi[k];
+  }
+  return gp_upload(h, phi.data(), mu.data());
+}
+
+int gp_download_fields(gp_handle *h, gp_cell *cells, int64_t ncells) {
+  if (!h || !cells) return h ? fail(h, GP_E_ARG, "null argument") : GP_E_ARG;
+  const int64_t n = h->rx * h->ry;
+  if (ncells != n) return fail(h, GP_E_ARG, "ncells %lld != rows_x*rows_y %lld", (long long)ncells, (long long)n);
+  std::vector<double> phi((size_t)h->N * n), mu((size_t)h->NC * n), dphi((size_t)h->N * n);
+  int rc = gp_download(h, phi.data(), mu.data(), dphi.data());
+  if (rc) return rc;
+  for (int64_t i = 0; i < n; i++) {
+    for (int a = 0; a < h->N; a++) {
+      cells[i].phia[a] = phi[(size_t)a * n + i];
+      if (cells[i].deltaphi) cells[i].deltaphi[a] = dphi[(size_t)a * n + i];
+    }
+    for (int k = 0; k < h->NC; k++) cells[i].compi[k] = mu[(size_t)k * n + i];
+  }
+  return GP_OK;
+}
+
+int gp_smooth(gp_handle *h, int64_t nsteps) {
+  if (!h || nsteps < 0) return h ? fail(h, GP_E_ARG, "bad nsteps") : GP_E_ARG;
+  step_fn f = pick_any(h);
+  for (int64_t t = 0; t < nsteps; t++) {
+    int rc = f(h, true);
+    if (rc) return rc;
+  }
+  return GP_OK;
+}
+
+int gp_get_stats(gp_handle *h, gp_stats *st) {
+  if (!h || !st) return h ? fail(h, GP_E_ARG, "null argument") : GP_E_ARG;
+  double s[kNStatMax];
+  HIPCHK(h, hipMemcpyAsync(s, h->stats_dev, sizeof s, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  memset(st, 0, sizeof *st);
+  const int N = h->N, NC = h->NC;
+  for (int a = 0; a < N; a++) {
+    st->phi_max[a] = s[a];
+    st->phi_min[a] = s[N + a];
+    st->rel_change_phi[a] = s[2 * N + 2 * NC + a];
+  }
+  for (int k = 0; k < NC; k++) {
+    st->mu_max[k] = s[2 * N + k];
+    st->mu_min[k] = s[2 * N + NC + k];
+    st->rel_change_mu[k] = s[3 * N + 2 * NC + k];
+  }
+  st->nonfinite = (int64_t)s[3 * N + 3 * NC];
+  st->step = h->steps;
+  return GP_OK;
+}
+
+int gp_step(gp_handle *h, int64_t nsteps, gp_stats *stats) {
+  if (!h || nsteps < 0) return h ? fail(h, GP_E_ARG, "bad nsteps") : GP_E_ARG;
+  step_fn f = pick_any(h);
+  for (int64_t t = 0; t < nsteps; t++) {
+    int rc = f(h, false);
+    if (rc) return rc;
+    h->steps++;
+  }
+  if (stats) {
+    int rc = gp_get_stats(h, stats);
+    if (rc) return rc;
+    if (stats->nonfinite) return fail(h, GP_E_NAN, "%lld non-finite values after step %lld", (long long)stats->nonfinite, (long long)h->steps);
+  }
+  return GP_OK;
+}
+
+int gp_profile_enable(gp_handle *h, int enable) {
+  if (!h) return GP_E_ARG;
+  h->profiling = enable != 0;
+  return GP_OK;
+}
+
+const char *gp_kernel_name(int kind) {
+  static const char *names[GP_NKERNELS] = {"k_phi", "k_mu", "k_stats", "k_step_fused"};
+  return (kind >= 0 && kind < GP_NKERNELS) ? names[kind] : "?";
+}
+
+int gp_profile_read(gp_handle *h, double *avg_ms, int64_t *count) {
+  if (!h || !avg_ms || !count) return h ? fail(h, GP_E_ARG, "null argument") : GP_E_ARG;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  double tot[GP_NKERNELS] = {0};
+  int64_t cnt[GP_NKERNELS] = {0};
+  for (auto &r : h->prof) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, r.second.first, r.second.second) == hipSuccess && r.first >= 0 && r.first < GP_NKERNELS) {
+      tot[r.first] += ms;
+      cnt[r.first]++;
+    }
+    hipEventDestroy(r.second.first);
+    hipEventDestroy(r.second.second);
+  }
+  h->prof.clear();
+  for (int k = 0; k < GP_NKERNELS; k++) {
+    avg_ms[k] = cnt[k] ? tot[k] / (double)cnt[k] : 0.0;
+    count[k] = cnt[k];
+  }
+  return GP_OK;
+}
+
+int gp_synchronize(gp_handle *h) {
+  if (!h) return GP_E_ARG;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return GP_OK;
+}
+
+}  // extern "C"

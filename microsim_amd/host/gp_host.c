@@ -679,6 +679,7 @@ int gp_fill_domain(const gp_input *in, const char *filling_path, double *phi, do
     return GP_E_IO;
   }
   char line[1000], key[128] = "", val[128] = "";
+  int nfills = 0;
   while (fgets(line, sizeof line, fr)) {
     sscanf(line, "%100s = %100[^;];", key, val);
     if (key[0] == '#') continue;
@@ -700,6 +701,7 @@ int gp_fill_domain(const gp_input *in, const char *filling_path, double *phi, do
             else fill_liquid_cell(&g, x, y, quiet);
           }
       }
+      nfills++;
     } else if (strcmp(key, "FILLCYLINDER") == 0 || strcmp(key, "FILLSPHERE") == 0) {
       int cyl = strcmp(key, "FILLCYLINDER") == 0;
       if (cyl && !quiet) printf("Filling cylinder\n");
@@ -710,10 +712,22 @@ int gp_fill_domain(const gp_input *in, const char *filling_path, double *phi, do
       long z0 = atol(tok[3]), z1 = cyl ? atol(tok[4]) : 0;
       double r = atof(tok[cyl ? 5 : 4]);
       if (ph < 0 || ph >= N) { fclose(fr); seterr(err, errlen, "%s phase out of range", key); return GP_E_ARG; }
+      /* The reference sweeps the whole grid twice per seed (solid pass, liquid pass). Outside the seed the
+       * solid pass is a no-op (solid fractions are only ever 0 or 1 there) and the liquid pass recomputes
+       * 1 - sum(solids) from unchanged values, so both passes are restricted to the seed's bounding box here;
+       * one full liquid pass after the last fill line restores every other cell. Same state, O(seed area). */
+      int64_t bx0 = 0, bx1 = g.rx - 1, by0 = 0, by1 = g.ry - 1;
+      if (r >= 0) {
+        int64_t rr = (int64_t)ceil(r) + 1;
+        bx0 = xc - rr > 0 ? xc - rr : 0;
+        bx1 = xc + rr < g.rx - 1 ? xc + rr : g.rx - 1;
+        by0 = yc - rr > 0 ? yc - rr : 0;
+        by1 = yc + rr < g.ry - 1 ? yc + rr : g.ry - 1;
+      }
       for (int pass = 0; pass < 2; pass++) {
         int b = pass == 0 ? ph : N - 1;
-        for (int64_t x = 0; x < g.rx; x++)
-          for (int64_t y = 0; y < g.ry; y++) {
+        for (int64_t x = bx0; x <= bx1; x++)
+          for (int64_t y = by0; y <= by1; y++) {
             if (b < N - 1) {
               long d2 = (x - xc) * (x - xc) + (y - yc) * (y - yc);
               int inside;
@@ -725,6 +739,7 @@ int gp_fill_domain(const gp_input *in, const char *filling_path, double *phi, do
             }
           }
       }
+      nfills++;
       if (cyl && !quiet) printf("End filling cylinder\n");
     } else if (strcmp(key, "FILLELLIPSE") == 0) {
       /* fill_phase_ellipse (GPs/functions/filling.h:54-116): indexes x*MESH_Y+y of the halo-inclusive array
@@ -753,9 +768,13 @@ int gp_fill_domain(const gp_input *in, const char *filling_path, double *phi, do
             }
           }
       }
+      nfills++;
     }
   }
   fclose(fr);
+  if (nfills) /* the last line's full liquid pass */
+    for (int64_t x = 0; x < g.rx; x++)
+      for (int64_t y = 0; y < g.ry; y++) fill_liquid_cell(&g, x, y, 1);
   if (!quiet) printf("Filling composition\n");
   /* fill_composition_cube (GPs/functions/filling.h:221-270): solid cells at ceq[b][b], liquid at
    * cfill[L][L], both through Mu(c, Teq, .) */

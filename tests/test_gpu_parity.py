@@ -1,0 +1,144 @@
+"""FP64 GPU parity: libgp_hip through its C ABI against the reference golden vectors and the FP64 oracle.
+
+Gate (north star): max relative error <= 1e-12 per field. The FP64 kernels keep the reference's operation
+order without contraction, so the expected result is exact equality; the tests assert the 1e-12 gate and
+report the number of differing values.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from gp_testlib import golden_cases, load_golden, max_rel_err, oracle_smooth, oracle_step
+from microsim_amd import _abi
+
+pytestmark = pytest.mark.gpu
+
+GATE = 1e-12
+
+
+def _solver(params, precision=_abi.GP_FP64):
+    from microsim_amd.solver import GPSolver
+    return GPSolver(params, precision)
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("name", golden_cases())
+def test_gpu_matches_reference_golden(name):
+    _say("[gpu] golden case", name)
+    case, ref = load_golden(name)
+    s = _solver(case.params)
+    try:
+        s.upload(case.phi, case.mu)
+        t = 0
+        for i, step in enumerate(ref["steps"]):
+            if step > t:
+                if t == 0 and case.inp.nsmooth > 1:
+                    s.smooth(case.inp.nsmooth - 1)
+                s.step(step - t)
+                t = step
+            phi, mu = s.download()
+            p, m = phi[:, 3:-3, 3:-3], mu[:, 3:-3, 3:-3]
+            if case.inp.ascii:
+                np.testing.assert_allclose(p, ref["phi"][i], atol=1e-6)
+                np.testing.assert_allclose(m, ref["mu"][i], rtol=1e-6)
+                continue
+            ep, em = max_rel_err(p, ref["phi"][i]), max_rel_err(m, ref["mu"][i])
+            nd = int((p != ref["phi"][i]).sum() + (m != ref["mu"][i]).sum())
+            _say("   step %d: max rel err phi %.3g mu %.3g, differing values %d" % (step, ep, em, nd))
+            assert ep <= GATE and em <= GATE
+    finally:
+        s.close()
+
+
+@pytest.mark.parametrize("name", ["tern_c1_mini", "bin_tanh_restart", "quad_k3"])
+def test_gpu_matches_oracle_dphi_and_halos(name):
+    """Full halo-inclusive state and the last step's delta-phi against the oracle."""
+    _say("[gpu] oracle halo/dphi", name)
+    case, _ = load_golden(name)
+    n = 37
+    s = _solver(case.params)
+    try:
+        s.upload(case.phi, case.mu)
+        s.step(n)
+        phi, mu, dphi = s.download(with_dphi=True)
+    finally:
+        s.close()
+    ophi, omu = case.phi.copy(), case.mu.copy()
+    odphi = np.zeros_like(ophi)
+    oracle_step(case.params, ophi, omu, n, dphi=odphi)
+    assert max_rel_err(phi, ophi) <= GATE and max_rel_err(mu, omu) <= GATE
+    sl = (slice(None), slice(2, -2), slice(2, -2))
+    assert max_rel_err(dphi[sl], odphi[sl]) <= GATE
+
+
+def test_gpu_stats_match_oracle():
+    from microsim_amd._abi import GPStats
+    import gp_testlib
+    _say("[gpu] stats")
+    case, _ = load_golden("tern_aniso_per")
+    s = _solver(case.params)
+    try:
+        s.upload(case.phi, case.mu)
+        got = s.step(20, stats=True)
+    finally:
+        s.close()
+    st = GPStats()
+    gp_testlib.oracle().oracle_stats_init(st)
+    ophi, omu = case.phi.copy(), case.mu.copy()
+    oracle_step(case.params, ophi, omu, 20, stats=st)
+    N, NC = case.N, case.NC
+    assert got["phi_max"] == list(st.phi_max[:N]) and got["phi_min"] == list(st.phi_min[:N])
+    assert got["mu_max"] == list(st.mu_max[:NC]) and got["mu_min"] == list(st.mu_min[:NC])
+    np.testing.assert_allclose(got["rel_change_phi"], list(st.rel_change_phi[:N]), rtol=1e-12)
+    np.testing.assert_allclose(got["rel_change_mu"], list(st.rel_change_mu[:NC]), rtol=1e-12)
+    assert got["nonfinite"] == 0 and got["step"] == 20
+
+
+def test_gpu_deterministic():
+    _say("[gpu] determinism")
+    case, _ = load_golden("tern_c1_mini")
+    outs = []
+    for _ in range(2):
+        s = _solver(case.params)
+        try:
+            s.upload(case.phi, case.mu)
+            s.step(25)
+            outs.append(s.download())
+        finally:
+            s.close()
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+
+
+def test_gpu_device_sqrt_div_correctly_rounded():
+    """The parity argument relies on IEEE-correct double sqrt and division on the device; the anti-trapping
+    normals (sqrt) and every gradient (division) are exercised by the golden decks, so bit-equality there
+    already implies it on the values that occur. This checks it on adversarial inputs through the library:
+    a deck whose dx, dy, dt have long mantissas."""
+    from microsim_amd import configs, host
+    import os
+    import tempfile
+    _say("[gpu] odd spacings")
+    rng = np.random.default_rng(3)
+    inp = configs.GPInput(mesh_x=33, mesh_y=47, nphases=3, ncomp=3, dx=0.9876543219, dy=1.2345678901,
+                          dt=0.0123456789, gamma=[1.1, 0.9, 1.0], tau_ab=[0.31, 0.29, 0.33], dab=[0.021, 0.017, 0.03],
+                          gamma_abc=[0.3], bc_phi=(3, 3, 1, 1), rotations={(0, 1): (0.0, 0.0, 7.0)})
+    configs.ternary_material(inp, rng)
+    fills = [configs.Fill("cylinder", 0, (10, 14, 5.0)), configs.Fill("cylinder", 1, (22, 30, 6.5))]
+    d = tempfile.mkdtemp()
+    configs.write_case(d, inp, fills)
+    case = host.load_case(os.path.join(d, "Input.in"), os.path.join(d, "Filling.in"))
+    s = _solver(case.params)
+    try:
+        s.upload(case.phi, case.mu)
+        s.step(30)
+        phi, mu = s.download()
+    finally:
+        s.close()
+    ophi, omu = case.phi.copy(), case.mu.copy()
+    oracle_step(case.params, ophi, omu, 30)
+    assert max_rel_err(phi, ophi) <= GATE and max_rel_err(mu, omu) <= GATE
