@@ -162,7 +162,7 @@ def main():
             out["cpu_baseline"] = cb
             try:
                 small, _ = make_case(1024, 64)
-                nth = os.cpu_count() or 1
+                nth = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
                 out["cpu_oracle_threads"] = {"value": oracle_threads(small, nth), "cores": nth,
                                              "kind": "port", "unit": "cell-updates/s"}
             except Exception as e:  # extra figure only

@@ -5,13 +5,12 @@
  *   every field component is a plane of rows_x = mesh_x+6 rows; a row holds the halo-inclusive y range
  *   0..mesh_y+5 starting at element kYOff, so the first interior cell y = 3 sits on a 64-element boundary;
  *   the row pitch ps is a multiple of 64 elements. phi (N planes) and mu (NC planes) are ping-ponged,
- *   dphi (N planes) holds the last step's projected delta-phi.
- * A step is two launches:
- *   k_phi : delta-phi on x,y in [2, mesh+3] (the interior plus the one-cell ring the update reads),
- *   k_mu  : anti-trapping currents, mu update, phi += delta-phi on the interior, halo images written
- *           directly (each boundary cell scatters itself to the halo cells whose proxy it is), per-block
- *           statistics partials;
- * followed by a one-block reduction of the partials into the cumulative statistics.
+ *   dphi (N planes) holds delta-phi when it is asked for.
+ * A time step is one launch of k_march (gp_march.h): a single pass that reads phi and mu once, recomputes
+ * delta-phi and the anti-trapping currents on the fly and writes the new phi and mu (and the halo images of
+ * boundary cells), followed by the statistics fold k_stats.
+ * The two-pass form (k_phi: delta-phi on x,y in [2, mesh+3]; k_mu: currents and update on the interior) runs
+ * the smoothing steps, recomputes delta-phi for gp_download, and is selectable for steps with GP_PATH=twopass.
  * Reference routines replaced: see include/gp_hip.h.
  */
 #include <hip/hip_runtime.h>
@@ -25,22 +24,18 @@
 #include <algorithm>
 #include <vector>
 
+#include "gp_common.h"
 #include "gp_hip.h"
+#include "gp_march.h"
 #include "gp_model.h"
 
 using namespace gpk;
 
 namespace {
 
-constexpr int kYOff = 61;        /* element offset of y = 0 inside a row: y = 3 lands on a 64-element boundary */
 constexpr int kBX = 64;          /* threads along y (one wavefront) */
 constexpr int kBY = 4;           /* rows per block */
 constexpr int kNStatMax = 3 * GP_MAXP + 3 * GP_MAXNC + 1;
-
-struct Maps {
-  int32_t sx[2][6], sy[2][6];
-  int32_t hx[6], hy[6];
-};
 
 char g_create_error[512];
 
@@ -60,7 +55,13 @@ struct gp_handle {
   int cur;
   int64_t nblocks_mu;
   int64_t steps;
+  int64_t nparts_max;       /* rows of the partials buffer */
+  int march_seg, march_nstrips;
+  int64_t march_nwaves;     /* 0 until the march plan is made */
+  bool use_march;           /* single-pass steps (default) or the two-pass form */
+  bool prev_valid;          /* the other buffer holds the state before the last step */
   Maps maps;
+  bool pow2;
   bool profiling;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof;
   char err[512];
@@ -100,7 +101,7 @@ static int fail(gp_handle *h, int code, const char *fmt, ...) {
 
 /* ------------------------------------------------------------------------------------------------ kernels */
 
-template <int N, int NC, typename R, bool SMOOTH>
+template <int N, int NC, typename R, bool SMOOTH, bool P2>
 __global__ __launch_bounds__(kBX *kBY) void k_phi(const Model<R> *__restrict__ mp, const R *__restrict__ phi,
                                                   const R *__restrict__ mu, R *__restrict__ dphi, int64_t fs,
                                                   int64_t ps, int64_t mx, int64_t my) {
@@ -109,48 +110,13 @@ __global__ __launch_bounds__(kBX *kBY) void k_phi(const Model<R> *__restrict__ m
   if (x > mx + 3 || y > my + 3) return;
   const Model<R> &m = *mp;
   const int64_t c = x * ps + y + kYOff;
-  View<R> v{phi, mu, nullptr, fs, ps, c};
+  View<R, P2> v{phi, mu, nullptr, fs, ps, c};
   R d[N];
   phi_delta<N, NC>(m, v, SMOOTH, d);
   for (int a = 0; a < N; a++) dphi[a * fs + c] = d[a];
 }
 
-template <typename R>
-__device__ __forceinline__ void write_images(R *__restrict__ base, int ncomp, const R *vals, int f, const Maps &mp,
-                                             int64_t x, int64_t y, int64_t fs, int64_t ps) {
-  int64_t X[3], Y[3];
-  int nx = 1, ny = 1;
-  X[0] = x;
-  Y[0] = y;
-  for (int k = 0; k < 6; k++) {
-    if (mp.sx[f][k] == x && nx < 3) X[nx++] = mp.hx[k];
-    if (mp.sy[f][k] == y && ny < 3) Y[ny++] = mp.hy[k];
-  }
-  for (int i = 0; i < nx; i++)
-    for (int j = 0; j < ny; j++) {
-      if (i == 0 && j == 0) continue;
-      const int64_t c = X[i] * ps + Y[j] + kYOff;
-      for (int a = 0; a < ncomp; a++) base[a * fs + c] = vals[a];
-    }
-}
-
-template <typename T>
-__device__ __forceinline__ T wave_max(T v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-  return v;
-}
-template <typename T>
-__device__ __forceinline__ T wave_min(T v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
-  return v;
-}
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
-}
-
-template <int N, int NC, typename R, bool SMOOTH>
+template <int N, int NC, typename R, bool SMOOTH, bool P2>
 __global__ __launch_bounds__(kBX *kBY) void k_mu(const Model<R> *__restrict__ mp, const R *__restrict__ phi,
                                                  const R *__restrict__ mu, const R *__restrict__ dphi,
                                                  R *__restrict__ phi_o, R *__restrict__ mu_o, Maps maps,
@@ -175,7 +141,7 @@ __global__ __launch_bounds__(kBX *kBY) void k_mu(const Model<R> *__restrict__ mp
   if (live) {
     const Model<R> &m = *mp;
     const int64_t c = x * ps + y + kYOff;
-    View<R> v{phi, mu, dphi, fs, ps, c};
+    View<R, P2> v{phi, mu, dphi, fs, ps, c};
     R jx0[NC], jxb[NC], jy0[NC], jyl[NC], d[N], pn[N], mn[NC], dm[NC];
     if (!SMOOTH) {
       jat_x<N, NC>(m, v, 0, 0, jx0);
@@ -227,48 +193,77 @@ __global__ __launch_bounds__(kBX *kBY) void k_mu(const Model<R> *__restrict__ mp
       else r += red[w][i];
     }
     const int64_t bid = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
-    partials[bid * kNStatMax + i] = r;
+    const int64_t nparts = (int64_t)gridDim.x * gridDim.y;
+    partials[i * nparts + bid] = r;  /* statistic-major: the fold reads each statistic contiguously */
   }
 }
 
-/* one block: fold the per-block partials (fixed order) into the cumulative statistics */
+/* Fold the per-block partials into the cumulative statistics: one block per statistic, each reading its
+ * contiguous row of partials (statistic-major layout) in a fixed order, so the result is deterministic. */
+constexpr int kStatThreads = 1024;
+
 template <int N, int NC>
-__global__ __launch_bounds__(256) void k_stats(const double *__restrict__ partials, int64_t nblocks,
-                                               double *__restrict__ cum) {
+__global__ __launch_bounds__(kStatThreads) void k_stats(const double *__restrict__ partials, int64_t nparts,
+                                                        double *__restrict__ cum) {
   constexpr int NS = 3 * N + 3 * NC + 1;
-  __shared__ double red[256];
-  for (int i = 0; i < NS; i++) {
-    const bool mx = (i < N || (i >= 2 * N && i < 2 * N + NC));
-    const bool mn = !mx && i < 2 * N + 2 * NC;
-    double r = mx ? -INFINITY : mn ? INFINITY : 0.0;
-    for (int64_t b = threadIdx.x; b < nblocks; b += 256) {
-      double v = partials[b * kNStatMax + i];
-      r = mx ? fmax(r, v) : mn ? fmin(r, v) : r + v;
-    }
-    red[threadIdx.x] = r;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-      if (threadIdx.x < s) {
-        double o = red[threadIdx.x + s];
-        red[threadIdx.x] = mx ? fmax(red[threadIdx.x], o) : mn ? fmin(red[threadIdx.x], o) : red[threadIdx.x] + o;
-      }
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-      double v = red[0];
-      if (mx) cum[i] = fmax(cum[i], v);
-      else if (mn) cum[i] = fmin(cum[i], v);
-      else if (i < NS - 1) cum[i] = v;
-      else cum[i] += v;
+  __shared__ double red[kStatThreads];
+  const int i = blockIdx.x;
+  if (i >= NS) return;
+  const bool mx = (i < N || (i >= 2 * N && i < 2 * N + NC));
+  const bool mn = !mx && i < 2 * N + 2 * NC;
+  double r = mx ? -INFINITY : mn ? INFINITY : 0.0;
+  const double *row = partials + (int64_t)i * nparts;
+  for (int64_t b = threadIdx.x; b < nparts; b += kStatThreads) {
+    double v = row[b];
+    r = mx ? fmax(r, v) : mn ? fmin(r, v) : r + v;
+  }
+  red[threadIdx.x] = r;
+  __syncthreads();
+  for (int s = kStatThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      double o = red[threadIdx.x + s];
+      red[threadIdx.x] = mx ? fmax(red[threadIdx.x], o) : mn ? fmin(red[threadIdx.x], o) : red[threadIdx.x] + o;
     }
     __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double v = red[0];
+    if (mx) cum[i] = fmax(cum[i], v);
+    else if (mn) cum[i] = fmin(cum[i], v);
+    else if (i < NS - 1) cum[i] = v;
+    else cum[i] += v;
   }
 }
 
 /* ------------------------------------------------------------------------------------------------ dispatch */
 
-template <int N, int NC, typename R>
-static int launch_step(gp_handle *h, bool smooth) {
+enum { MODE_SMOOTH = 0, MODE_TWOPASS = 1, MODE_MARCH = 2, MODE_DPHI_PREV = 3 };
+
+/* Segment length of the march: enough wavefronts for about two rounds of the resident capacity, at least
+ * 16 rows per segment (each segment recomputes two rows of delta-phi). */
+template <int N, int NC, typename R, bool P2>
+static void march_plan(gp_handle *h) {
+  const int64_t mx = h->p.mesh_x, my = h->p.mesh_y;
+  int ncu = 0, nb = 0;
+  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_march<N, NC, R, P2>, 256, 0) != hipSuccess || nb < 1) nb = 1;
+  if (ncu < 1) ncu = 1;
+  const int64_t nstrips = (my + kMarchOut - 1) / kMarchOut;
+  const int64_t target = 2 * (int64_t)nb * 4 * ncu;
+  int64_t nseg = (target + nstrips - 1) / nstrips;
+  int64_t seg = (mx + nseg - 1) / nseg;
+  if (seg < 16) seg = 16;
+  const char *e = getenv("GP_SEG_LEN");
+  if (e && atoi(e) > 0) seg = atoi(e);
+  if (seg > mx) seg = mx;
+  nseg = (mx + seg - 1) / seg;
+  h->march_seg = (int)seg;
+  h->march_nstrips = (int)nstrips;
+  h->march_nwaves = nstrips * nseg;
+}
+
+template <int N, int NC, typename R, bool P2>
+static int launch_step_p(gp_handle *h, int mode) {
   const R *phi = (const R *)h->phi[h->cur], *mu = (const R *)h->mu[h->cur];
   R *phi_o = (R *)h->phi[1 - h->cur], *mu_o = (R *)h->mu[1 - h->cur];
   R *dphi = (R *)h->dphi;
@@ -277,30 +272,56 @@ static int launch_step(gp_handle *h, bool smooth) {
   dim3 blk(kBX, kBY);
   dim3 g1((unsigned)((my + 2 + kBX - 1) / kBX), (unsigned)((mx + 2 + kBY - 1) / kBY));
   dim3 g2((unsigned)((my + kBX - 1) / kBX), (unsigned)((mx + kBY - 1) / kBY));
+  const bool smooth = mode == MODE_SMOOTH;
+  if (mode == MODE_DPHI_PREV) {
+    /* delta-phi of the last step, recomputed from the previous state (still in the other buffer) */
+    const R *pp = (const R *)h->phi[1 - h->cur], *mp = (const R *)h->mu[1 - h->cur];
+    hipLaunchKernelGGL((k_phi<N, NC, R, false, P2>), g1, blk, 0, h->stream, m, pp, mp, dphi, h->fs, h->ps, mx, my);
+    HIPCHK(h, hipGetLastError());
+    return GP_OK;
+  }
+  if (mode == MODE_MARCH) {
+    if (h->march_nwaves == 0) march_plan<N, NC, R, P2>(h);
+    const unsigned nblk = (unsigned)((h->march_nwaves + 3) / 4);
+    {
+      ProfScope ps(h, GP_KERNEL_FUSED);
+      hipLaunchKernelGGL((k_march<N, NC, R, P2>), dim3(nblk), dim3(256), 0, h->stream, m, phi, mu, phi_o, mu_o,
+                         h->maps, h->partials, h->fs, h->ps, mx, my, h->march_seg, h->march_nstrips,
+                         (int)h->march_nwaves);
+    }
+    {
+      ProfScope ps(h, GP_KERNEL_STATS);
+      hipLaunchKernelGGL((k_stats<N, NC>), dim3(3 * N + 3 * NC + 1), dim3(kStatThreads), 0, h->stream,
+                         h->partials, (int64_t)h->march_nwaves, h->stats_dev);
+    }
+    HIPCHK(h, hipGetLastError());
+    h->cur = 1 - h->cur;
+    return GP_OK;
+  }
   if (smooth) {
     {
       ProfScope ps(h, GP_KERNEL_PHI);
-      hipLaunchKernelGGL((k_phi<N, NC, R, true>), g1, blk, 0, h->stream, m, phi, mu, dphi, h->fs, h->ps, mx, my);
+      hipLaunchKernelGGL((k_phi<N, NC, R, true, P2>), g1, blk, 0, h->stream, m, phi, mu, dphi, h->fs, h->ps, mx, my);
     }
     {
       ProfScope ps(h, GP_KERNEL_MU);
-      hipLaunchKernelGGL((k_mu<N, NC, R, true>), g2, blk, 0, h->stream, m, phi, mu, dphi, phi_o, mu_o, h->maps,
+      hipLaunchKernelGGL((k_mu<N, NC, R, true, P2>), g2, blk, 0, h->stream, m, phi, mu, dphi, phi_o, mu_o, h->maps,
                          h->partials, h->fs, h->ps, mx, my);
     }
   } else {
     {
       ProfScope ps(h, GP_KERNEL_PHI);
-      hipLaunchKernelGGL((k_phi<N, NC, R, false>), g1, blk, 0, h->stream, m, phi, mu, dphi, h->fs, h->ps, mx, my);
+      hipLaunchKernelGGL((k_phi<N, NC, R, false, P2>), g1, blk, 0, h->stream, m, phi, mu, dphi, h->fs, h->ps, mx, my);
     }
     {
       ProfScope ps(h, GP_KERNEL_MU);
-      hipLaunchKernelGGL((k_mu<N, NC, R, false>), g2, blk, 0, h->stream, m, phi, mu, dphi, phi_o, mu_o, h->maps,
+      hipLaunchKernelGGL((k_mu<N, NC, R, false, P2>), g2, blk, 0, h->stream, m, phi, mu, dphi, phi_o, mu_o, h->maps,
                          h->partials, h->fs, h->ps, mx, my);
     }
     {
       ProfScope ps(h, GP_KERNEL_STATS);
-      hipLaunchKernelGGL((k_stats<N, NC>), dim3(1), dim3(256), 0, h->stream, h->partials, h->nblocks_mu,
-                         h->stats_dev);
+      hipLaunchKernelGGL((k_stats<N, NC>), dim3(3 * N + 3 * NC + 1), dim3(kStatThreads), 0, h->stream,
+                         h->partials, h->nblocks_mu, h->stats_dev);
     }
   }
   HIPCHK(h, hipGetLastError());
@@ -308,7 +329,12 @@ static int launch_step(gp_handle *h, bool smooth) {
   return GP_OK;
 }
 
-typedef int (*step_fn)(gp_handle *, bool);
+template <int N, int NC, typename R>
+static int launch_step(gp_handle *h, int mode) {
+  return h->pow2 ? launch_step_p<N, NC, R, true>(h, mode) : launch_step_p<N, NC, R, false>(h, mode);
+}
+
+typedef int (*step_fn)(gp_handle *, int);
 
 template <typename R>
 static step_fn pick(int N, int NC) {
@@ -450,6 +476,15 @@ int gp_create(const gp_params *params, int precision, gp_handle **out) {
   }
   hipGetDevice(&h->device);
   h->nblocks_mu = ((params->mesh_y + kBX - 1) / kBX) * ((params->mesh_x + kBY - 1) / kBY);
+  {
+    /* the march uses at most one wavefront per (60-cell strip, 16-row segment) */
+    const int64_t nw = ((params->mesh_y + kMarchOut - 1) / kMarchOut) * ((params->mesh_x + 15) / 16);
+    h->nparts_max = std::max(h->nblocks_mu, nw);
+  }
+  const char *path = getenv("GP_PATH");
+  h->use_march = !(path && strcmp(path, "twopass") == 0);
+  h->march_nwaves = 0;
+  h->prev_valid = false;
   const size_t es = precision == GP_FP64 ? 8 : 4;
   const size_t pn = (size_t)h->N * h->fs * es, mn = (size_t)h->NC * h->fs * es;
   hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
@@ -457,7 +492,7 @@ int gp_create(const gp_params *params, int precision, gp_handle **out) {
   ok = ok && hipMalloc(&h->phi[0], pn) == hipSuccess && hipMalloc(&h->phi[1], pn) == hipSuccess;
   ok = ok && hipMalloc(&h->mu[0], mn) == hipSuccess && hipMalloc(&h->mu[1], mn) == hipSuccess;
   ok = ok && hipMalloc(&h->dphi, pn) == hipSuccess;
-  ok = ok && hipMalloc((void **)&h->partials, sizeof(double) * kNStatMax * h->nblocks_mu) == hipSuccess;
+  ok = ok && hipMalloc((void **)&h->partials, sizeof(double) * kNStatMax * h->nparts_max) == hipSuccess;
   ok = ok && hipMalloc((void **)&h->stats_dev, sizeof(double) * kNStatMax) == hipSuccess;
   const size_t msize = precision == GP_FP64 ? sizeof(Model<double>) : sizeof(Model<float>);
   ok = ok && hipMalloc(&h->model, msize) == hipSuccess;
@@ -471,11 +506,13 @@ int gp_create(const gp_params *params, int precision, gp_handle **out) {
   if (precision == GP_FP64) {
     Model<double> m;
     make_model(params, &m);
+    h->pow2 = m.pow2 != 0;
     ok = ok && hipMemcpyAsync(h->model, &m, sizeof m, hipMemcpyHostToDevice, h->stream) == hipSuccess;
     ok = ok && hipStreamSynchronize(h->stream) == hipSuccess;
   } else {
     Model<float> m;
     make_model(params, &m);
+    h->pow2 = m.pow2 != 0;
     ok = ok && hipMemcpyAsync(h->model, &m, sizeof m, hipMemcpyHostToDevice, h->stream) == hipSuccess;
     ok = ok && hipStreamSynchronize(h->stream) == hipSuccess;
   }
@@ -528,6 +565,7 @@ int gp_upload(gp_handle *h, const double *phi, const double *mu) {
   HIPCHK(h, hipMemcpyAsync(h->stats_dev, s, sizeof s, hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->steps = 0;
+  h->prev_valid = false;
   return GP_OK;
 }
 
@@ -536,7 +574,13 @@ int gp_download(gp_handle *h, double *phi, double *mu, double *dphi) {
   int rc;
   if (phi && (rc = download_planes(h, phi, h->phi[h->cur], h->N))) return rc;
   if (mu && (rc = download_planes(h, mu, h->mu[h->cur], h->NC))) return rc;
-  if (dphi && (rc = download_planes(h, dphi, h->dphi, h->N))) return rc;
+  if (dphi) {
+    if (h->prev_valid) {
+      /* single-pass steps do not store delta-phi: recompute the last step's from the previous state */
+      if ((rc = pick_any(h)(h, MODE_DPHI_PREV))) return rc;
+    }
+    if ((rc = download_planes(h, dphi, h->dphi, h->N))) return rc;
+  }
   return GP_OK;
 }
 
@@ -573,9 +617,10 @@ int gp_smooth(gp_handle *h, int64_t nsteps) {
   if (!h || nsteps < 0) return h ? fail(h, GP_E_ARG, "bad nsteps") : GP_E_ARG;
   step_fn f = pick_any(h);
   for (int64_t t = 0; t < nsteps; t++) {
-    int rc = f(h, true);
+    int rc = f(h, MODE_SMOOTH);
     if (rc) return rc;
   }
+  if (nsteps > 0) h->prev_valid = false; /* dphi now holds the smoothing increment */
   return GP_OK;
 }
 
@@ -605,9 +650,10 @@ int gp_step(gp_handle *h, int64_t nsteps, gp_stats *stats) {
   if (!h || nsteps < 0) return h ? fail(h, GP_E_ARG, "bad nsteps") : GP_E_ARG;
   step_fn f = pick_any(h);
   for (int64_t t = 0; t < nsteps; t++) {
-    int rc = f(h, false);
+    int rc = f(h, h->use_march ? MODE_MARCH : MODE_TWOPASS);
     if (rc) return rc;
     h->steps++;
+    h->prev_valid = h->use_march;
   }
   if (stats) {
     int rc = gp_get_stats(h, stats);

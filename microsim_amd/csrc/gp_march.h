@@ -1,0 +1,672 @@
+/*
+ * gp_march.h -- single-pass step kernel for CDNA4: one wavefront per 60-cell y-strip marching along x.
+ *
+ * Lane l of a wavefront holds cell y = y0 - 2 + l of the current rows; y-neighbours come from lane shuffles,
+ * x-neighbours from the rows kept in registers between march steps. At march step x the wavefront holds the
+ * raw rows x-1, x, x+1 (phi, mu and their y+-1 neighbours) and
+ *   1. forms the face gradients and averages of rows x-1..x+1,
+ *   2. computes delta-phi of row x -- the anisotropic face terms (q_ab, a_c, da_c/dq) once per face and per
+ *      unordered phase pair: the back x-face is the previous step's front face and the left y-face is the
+ *      right y-face of the lane below; q_ba = -q_ab and a_c(q_ba) = a_c(q_ab) hold bit for bit, so the ordered
+ *      pairs reuse them with an exact sign change,
+ *   3. computes the anti-trapping currents on the right y-face of row x and the x-face between x-1 and x,
+ *   4. updates mu and phi of row x-1 and writes them (plus the halo images of boundary cells).
+ * Every value keeps the operation order of the per-cell functions in gp_model.h (which follow the serial
+ * reference), so the FP64 results are identical to those of gp_model.h and of the reference.
+ * A wavefront whose rows are all bulk (every Laplacian of phi zero, phi >= 0) skips the phase-field work of the
+ * row: delta-phi is exactly zero there. Lanes 2..61 produce output; lanes 0, 1, 62, 63 only feed neighbours.
+ */
+#pragma once
+
+#include "gp_common.h"
+#include "gp_model.h"
+
+namespace gpk {
+
+constexpr int kMarchLanes = 64;
+constexpr int kMarchOut = 60;
+
+__device__ __forceinline__ double lane_up(double v) { return __shfl_up(v, 1, 64); }   /* value of lane-1 */
+__device__ __forceinline__ double lane_dn(double v) { return __shfl_down(v, 1, 64); } /* value of lane+1 */
+__device__ __forceinline__ float lane_up(float v) { return __shfl_up(v, 1, 64); }
+__device__ __forceinline__ float lane_dn(float v) { return __shfl_down(v, 1, 64); }
+
+template <int N>
+struct Pairs {
+  static constexpr int n = N * (N - 1) / 2;
+  static constexpr GP_HD int idx(int a, int b) { return a * (2 * N - a - 1) / 2 + (b - a - 1); } /* a < b */
+};
+
+template <int N, int NC, typename R>
+struct RowRaw {
+  R ph[N], phu[N], phd[N];
+  R mu[NC], muu[NC], mud[NC];
+};
+
+template <int N, int NC, typename R>
+__device__ __forceinline__ void row_load(RowRaw<N, NC, R> &r, const R *__restrict__ phi, const R *__restrict__ mu,
+                                         int64_t fs, int64_t idx) {
+#pragma unroll
+  for (int a = 0; a < N; a++) r.ph[a] = phi[a * fs + idx];
+#pragma unroll
+  for (int k = 0; k < NC; k++) r.mu[k] = mu[k * fs + idx];
+}
+
+template <int N, int NC, typename R>
+__device__ __forceinline__ void row_halo(RowRaw<N, NC, R> &r) {
+#pragma unroll
+  for (int a = 0; a < N; a++) {
+    r.phu[a] = lane_up(r.ph[a]);
+    r.phd[a] = lane_dn(r.ph[a]);
+  }
+#pragma unroll
+  for (int k = 0; k < NC; k++) {
+    r.muu[k] = lane_up(r.mu[k]);
+    r.mud[k] = lane_dn(r.mu[k]);
+  }
+}
+
+/* per-face anisotropy data of one face, per unordered pair: U = a_c^2 r (or r, isotropic), V = (a_c |q|^2) rd */
+template <int N, typename R>
+struct FaceAn {
+  R U[Pairs<N>::n > 0 ? Pairs<N>::n : 1];
+  R V[Pairs<N>::n > 0 ? Pairs<N>::n : 1];
+};
+
+/* x-face between rows r and r+1: sx, gx of the face, cy of both rows (q_divx, function_A_01.h:135-206) */
+template <int N, typename R>
+__device__ __forceinline__ void face_x(const Model<R> &m, const R *sxv, const R *gxv, const R *cy0, const R *cy1,
+                                       FaceAn<N, R> &f) {
+#pragma unroll
+  for (int a = 0; a < N; a++)
+#pragma unroll
+    for (int b = a + 1; b < N; b++) {
+      const int p = Pairs<N>::idx(a, b);
+      const R r0 = (sxv[a] * gxv[b] - sxv[b] * gxv[a]);
+      if (!m.fn_aniso) {
+        f.U[p] = r0;
+        f.V[p] = R(0);
+        continue;
+      }
+      R r[2], q[2], dq[2], rd[2];
+      r[0] = r0;
+      r[1] = R(0.5) * (sxv[a] * (cy0[b] + cy1[b]) - sxv[b] * (cy0[a] + cy1[a]));
+      rot2(m.rot[a][b], r, q);
+      const R ac = aniso_ac(m.dab[a][b], q);
+      aniso_dadq(m.dab[a][b], q, dq);
+      rot2(m.rot_inv[a][b], dq, rd);
+      const R ac2 = ac * ac;
+      f.U[p] = ac2 * r0;
+      f.V[p] = ac * qnorm2(q) * rd[0];
+    }
+}
+
+/* y-face between lanes y and y+1 of one row: sy, gy of the face, cx at y and y+1 (q_divy) */
+template <int N, typename R>
+__device__ __forceinline__ void face_y(const Model<R> &m, const R *syv, const R *gyv, const R *cx0, const R *cx1,
+                                       FaceAn<N, R> &f) {
+#pragma unroll
+  for (int a = 0; a < N; a++)
+#pragma unroll
+    for (int b = a + 1; b < N; b++) {
+      const int p = Pairs<N>::idx(a, b);
+      const R r1 = (syv[a] * gyv[b] - syv[b] * gyv[a]);
+      if (!m.fn_aniso) {
+        f.U[p] = r1;
+        f.V[p] = R(0);
+        continue;
+      }
+      R r[2], q[2], dq[2], rd[2];
+      r[0] = R(0.5) * (syv[a] * (cx0[b] + cx1[b]) - syv[b] * (cx0[a] + cx1[a]));
+      r[1] = r1;
+      rot2(m.rot[a][b], r, q);
+      const R ac = aniso_ac(m.dab[a][b], q);
+      aniso_dadq(m.dab[a][b], q, dq);
+      rot2(m.rot_inv[a][b], dq, rd);
+      const R ac2 = ac * ac;
+      f.U[p] = ac2 * r1;
+      f.V[p] = ac * qnorm2(q) * rd[1];
+    }
+}
+
+template <int N, typename R>
+__device__ __forceinline__ void face_shuffle_up(const FaceAn<N, R> &src, FaceAn<N, R> &dst) {
+#pragma unroll
+  for (int p = 0; p < Pairs<N>::n; p++) {
+    dst.U[p] = lane_up(src.U[p]);
+    dst.V[p] = lane_up(src.V[p]);
+  }
+}
+
+/* div(da/dgrad phi_a) from the four faces (function_A_01_divdAdgradphi / function_A_00_divdAdgradphi) */
+template <int N, class V, typename R>
+__device__ __forceinline__ R div_dadgrad(const Model<R> &m, int a, const FaceAn<N, R> &ff, const FaceAn<N, R> &fb,
+                                         const FaceAn<N, R> &fr, const FaceAn<N, R> &fl, const R *sxf, const R *sxb,
+                                         const R *syr, const R *syl) {
+  R sum1 = R(0.0), sum2 = R(0.0);
+#pragma unroll
+  for (int b = 0; b < N; b++) {
+    if (b == a) continue;
+    const int p = a < b ? Pairs<N>::idx(a, b) : Pairs<N>::idx(b, a);
+    const bool s = a < b;
+    const R Uf = s ? ff.U[p] : -ff.U[p], Ub = s ? fb.U[p] : -fb.U[p];
+    const R Ur = s ? fr.U[p] : -fr.U[p], Ul = s ? fl.U[p] : -fl.U[p];
+    if (m.fn_aniso) {
+      const R Vf = s ? ff.V[p] : -ff.V[p], Vb = s ? fb.V[p] : -fb.V[p];
+      const R Vr = s ? fr.V[p] : -fr.V[p], Vl = s ? fl.V[p] : -fl.V[p];
+      R d1 = divx<V>(m, (Uf * (-sxf[b])) - (Ub * (-sxb[b])));
+      d1 += divy<V>(m, (Ur * (-syr[b])) - (Ul * (-syl[b])));
+      d1 *= m.gamma[a][b];
+      sum1 += d1;
+      R d2 = divx<V>(m, (Vf * (-sxf[b])) - (Vb * (-sxb[b])));
+      d2 += divy<V>(m, (Vr * (-syr[b])) - (Vl * (-syl[b])));
+      d2 *= m.gamma[a][b];
+      sum2 += d2;
+    } else {
+      R d = divx<V>(m, Uf * (-sxf[b]) - (Ub * (-sxb[b])));
+      d += divy<V>(m, Ur * (-syr[b]) - (Ul * (-syl[b])));
+      d *= m.gamma[a][b];
+      sum1 += d;
+    }
+  }
+  if (m.fn_aniso) return (R(2.0) * (sum1 + sum2) + R(0.0));
+  return R(2.0) * sum1;
+}
+
+/* d a / d phi_a at the cell centre with per-pair sharing (function_A_01_dAdphi) */
+template <int N, typename R>
+__device__ __forceinline__ void dadphi_all(const Model<R> &m, const R *phi, const R *cxc, const R *cyc, R *out) {
+  if (!m.fn_aniso) {
+#pragma unroll
+    for (int a = 0; a < N; a++) out[a] = dAdphi_A00<N>(m, phi, cxc, cyc, a);
+    return;
+  }
+  constexpr int NP = Pairs<N>::n;
+  R qp[NP > 0 ? NP : 1][2], dq[NP > 0 ? NP : 1][2], ac1[NP > 0 ? NP : 1], q2[NP > 0 ? NP : 1];
+#pragma unroll
+  for (int a = 0; a < N; a++)
+#pragma unroll
+    for (int b = a + 1; b < N; b++) {
+      const int p = Pairs<N>::idx(a, b);
+      R r[2];
+      r[0] = (phi[a] * cxc[b] - phi[b] * cxc[a]);
+      r[1] = (phi[a] * cyc[b] - phi[b] * cyc[a]);
+      rot2(m.rot[a][b], r, qp[p]);
+      aniso_dadq(m.dab[a][b], qp[p], dq[p]);
+      q2[p] = qnorm2(qp[p]);
+      ac1[p] = aniso_ac(m.dab[a][b], qp[p]);
+    }
+#pragma unroll
+  for (int a = 0; a < N; a++) {
+    R sum1 = R(0.0), sum2 = R(0.0);
+#pragma unroll
+    for (int b = 0; b < N; b++) {
+      if (b == a) continue;
+      const bool s = a < b;
+      const int p = s ? Pairs<N>::idx(a, b) : Pairs<N>::idx(b, a);
+      R q[2], d[2], v[2], vr[2];
+      q[0] = s ? qp[p][0] : -qp[p][0];
+      q[1] = s ? qp[p][1] : -qp[p][1];
+      d[0] = s ? dq[p][0] : -dq[p][0];
+      d[1] = s ? dq[p][1] : -dq[p][1];
+      v[0] = cxc[b];
+      v[1] = cyc[b];
+      rot2(m.rot[a][b], v, vr);
+      R s1 = R(0.0), s2 = R(0.0);
+      s1 += q[0] * vr[0];
+      s1 += q[1] * vr[1];
+      s2 += d[0] * vr[0];
+      s2 += d[1] * vr[1];
+      const R ac2 = ac1[p] * ac1[p];
+      s1 *= m.gamma[a][b] * ac2;
+      s2 *= m.gamma[a][b] * ac1[p] * q2[p];
+      sum1 += s1;
+      sum2 += s2;
+    }
+    out[a] = R(2.0) * (sum1 + sum2);
+  }
+}
+
+/* anti-trapping current on one face, from the two face cells c (centre) and o (other: front or right).
+ * gL: normal derivative of the liquid across the face; tL: tangential liquid average; ga, ta per phase. */
+template <int N, int NC, typename R>
+__device__ __forceinline__ void jat_face(const Model<R> &m, const R *phc, const R *pho, const R (*ccc)[NC],
+                                         const R (*cco)[NC], const R *dc, const R *dov, const R *gnorm,
+                                         const R *gtan, R *j) {
+  const int L = N - 1;
+#pragma unroll
+  for (int k = 0; k < NC; k++) j[k] = R(0.0);
+  bool any = false;
+#pragma unroll
+  for (int a = 0; a < N - 1; a++) any = any || (dc[a] != R(0.0)) || (dov[a] != R(0.0));
+  if (!any) return;
+  const R nl = sqrt(gtan[L] * gtan[L] + gnorm[L] * gnorm[L]);
+#pragma unroll
+  for (int a = 0; a < N - 1; a++) {
+    const R ng = sqrt(gtan[a] * gtan[a] + gnorm[a] * gnorm[a]);
+    if (!(ng != R(0.0))) continue;
+    R s_c = R(0), s_o = R(0);
+    if (m.fn_w == 1) {
+      s_c = s_obst<N>(phc, a);
+      s_o = s_obst<N>(pho, a);
+    }
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+      const R cl_o = cco[L][k], cl_c = ccc[L][k], cs_o = cco[a][k], cs_c = ccc[a][k];
+      R sp = gnorm[a] * gnorm[L];
+      sp += gtan[a] * gtan[L];
+      if (nl > R(0.0)) sp /= (nl * ng);
+      if (m.fn_w == 2)
+        j[k] += m.dratio[a][k] * fabs(sp) * m.jat_well * ((cl_o - cs_o) * dov[a] + (cl_c - cs_c) * dc[a]) * gnorm[a] / ng;
+      else
+        j[k] += m.dratio[a][k] * fabs(sp) * R(0.5) * (s_o * (cl_o - cs_o) * dov[a] + s_c * (cl_c - cs_c) * dc[a]) * gnorm[a] / ng;
+    }
+  }
+}
+
+template <int N, int NC, typename R>
+__device__ __forceinline__ void cmu_all(const Model<R> &m, const R *mu, R (*cc)[NC]) {
+#pragma unroll
+  for (int a = 0; a < N; a++)
+#pragma unroll
+    for (int k = 0; k < NC; k++) cc[a][k] = c_mu<NC>(m, mu, a, k);
+}
+
+template <int N, int NC, typename R>
+__device__ __forceinline__ void mobility_cell(const Model<R> &m, const R *phi, R (*M)[NC]) {
+#pragma unroll
+  for (int k = 0; k < NC; k++)
+#pragma unroll
+    for (int l = 0; l < NC; l++) {
+      R s = R(0.0);
+#pragma unroll
+      for (int b = 0; b < N; b++)
+#pragma unroll
+        for (int q = 0; q < NC; q++) s += m.DC[b][k][l][q] * phi[b];
+      M[k][l] = s;
+    }
+}
+
+/* delta-phi of the centre from precomputed centre / face data (calculate_divergence_phasefield_2D body) */
+template <int N, int NC, class V, typename R>
+__device__ __forceinline__ void dphi_cell(const Model<R> &m, const R *phi, const R *divphi, const R *psi,
+                                          const R *cxc, const R *cyc, const FaceAn<N, R> &ff,
+                                          const FaceAn<N, R> &fb, const FaceAn<N, R> &fr, const FaceAn<N, R> &fl,
+                                          const R *sxf, const R *sxb, const R *syr, const R *syl, R *out) {
+  bool any = false;
+#pragma unroll
+  for (int a = 0; a < N; a++) any = any || (fabs(divphi[a]) > R(0.0));
+#pragma unroll
+  for (int a = 0; a < N; a++) out[a] = R(0.0);
+  if (any) {
+    R dA[N];
+    dadphi_all<N>(m, phi, cxc, cyc, dA);
+    R lambda[N], sum_lambda = R(0.0);
+    int active = 0;
+#pragma unroll
+    for (int a = 0; a < N; a++) {
+      lambda[a] = R(0);
+      if (fabs(divphi[a]) > R(0.0)) {
+        const R dD = div_dadgrad<N, V>(m, a, ff, fb, fr, fl, sxf, sxb, syr, syl);
+        R dp = R(0.0);
+#pragma unroll
+        for (int b = 0; b < N; b++) dp += psi[b] * dhphi<N>(phi, b, a);
+        lambda[a] = m.eps * (-dA[a] + dD) - m.inv_eps * dwdphi<N>(m, phi, divphi, a, false) - dp;
+        sum_lambda += lambda[a];
+        active++;
+      }
+    }
+    if (active) sum_lambda /= R(active);
+    const R tauv = tau_of<N>(m, phi);
+#pragma unroll
+    for (int a = 0; a < N; a++)
+      if (fabs(divphi[a]) > R(0.0)) out[a] = m.dt * (lambda[a] - sum_lambda) / (tauv * m.eps);
+  }
+  project_simplex<N>(phi, divphi, out);
+}
+
+template <int N, int NC, typename R, bool P2>
+__global__ __launch_bounds__(256) void k_march(const Model<R> *__restrict__ mp, const R *__restrict__ phi,
+                                               const R *__restrict__ mu, R *__restrict__ phi_o,
+                                               R *__restrict__ mu_o, Maps maps, double *__restrict__ partials,
+                                               int64_t fs, int64_t ps, int64_t mx, int64_t my, int seg_len,
+                                               int nstrips, int nwaves) {
+  using Vw = View<R, P2>;
+  constexpr int NS = 3 * N + 3 * NC + 1;
+  const Model<R> &m = *mp;
+  const int lane = threadIdx.x & 63;
+  const int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int64_t ry = my + 6;
+
+  double st[NS];
+#pragma unroll
+  for (int a = 0; a < N; a++) {
+    st[a] = -INFINITY;
+    st[N + a] = INFINITY;
+    st[2 * N + 2 * NC + a] = 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < NC; k++) {
+    st[2 * N + k] = -INFINITY;
+    st[2 * N + NC + k] = INFINITY;
+    st[3 * N + 2 * NC + k] = 0.0;
+  }
+  st[NS - 1] = 0.0;
+
+  if (w < nwaves) {
+    const int strip = w % nstrips, seg = w / nstrips;
+    const int64_t y0 = 3 + (int64_t)strip * kMarchOut;
+    const int64_t y = y0 - 2 + lane;
+    const int64_t yl = y < ry - 1 ? y : ry - 1;
+    const int64_t xa = 3 + (int64_t)seg * seg_len;
+    const int64_t xb = (xa + seg_len - 1 < mx + 2) ? xa + seg_len - 1 : mx + 2;
+    const bool out_lane = lane >= 2 && lane <= 61 && y <= my + 2;
+    const bool img_y = out_lane && (y < 7 || y > my - 2);
+
+    RowRaw<N, NC, R> A0, A1, A2;
+    row_load(A0, phi, mu, fs, (xa - 2) * ps + yl + kYOff);
+    row_halo(A0);
+    row_load(A1, phi, mu, fs, (xa - 1) * ps + yl + kYOff);
+    row_halo(A1);
+
+    R dph_prev[N], cc_prev[N][NC], M_prev[NC][NC], jx_prev[NC], jy_prev[NC], Fx_prev[NC][NC];
+#pragma unroll
+    for (int a = 0; a < N; a++) dph_prev[a] = R(0);
+    FaceAn<N, R> fb;
+    bool have_back = false;
+
+    for (int64_t x = xa - 1; x <= xb + 1; x++) {
+      row_load(A2, phi, mu, fs, (x + 1) * ps + yl + kYOff);
+      row_halo(A2);
+
+      /* face gradients / averages and centred gradients (calculate_gradients_phasefield_2D) */
+      R gx0[N], gx1[N], sx0[N], sx1[N], cy0[N], cy1[N], cy2[N], cxc[N], cxd[N], gy1[N], gyu[N], sy1[N], syu[N];
+      R divphi[N];
+      bool lane_busy = false;
+#pragma unroll
+      for (int a = 0; a < N; a++) {
+        gx0[a] = divx<Vw>(m, A1.ph[a] - A0.ph[a]);
+        gx1[a] = divx<Vw>(m, A2.ph[a] - A1.ph[a]);
+        sx0[a] = R(0.5) * (A1.ph[a] + A0.ph[a]);
+        sx1[a] = R(0.5) * (A2.ph[a] + A1.ph[a]);
+        gy1[a] = divy<Vw>(m, A1.phd[a] - A1.ph[a]);
+        gyu[a] = divy<Vw>(m, A1.ph[a] - A1.phu[a]);
+        divphi[a] = divx<Vw>(m, (gx1[a] - gx0[a]));
+        divphi[a] += divy<Vw>(m, (gy1[a] - gyu[a]));
+        lane_busy = lane_busy || (fabs(divphi[a]) > R(0.0)) || (A1.ph[a] < R(0.0));
+      }
+      R cc1[N][NC];
+      cmu_all<N, NC>(m, A1.mu, cc1);
+      R dph[N];
+#pragma unroll
+      for (int a = 0; a < N; a++) dph[a] = R(0.0);
+      if (__any(lane_busy)) {
+#pragma unroll
+        for (int a = 0; a < N; a++) {
+          cy0[a] = div2y<Vw>(m, A0.phd[a] - A0.phu[a]);
+          cy1[a] = div2y<Vw>(m, A1.phd[a] - A1.phu[a]);
+          cy2[a] = div2y<Vw>(m, A2.phd[a] - A2.phu[a]);
+          cxc[a] = R(0.5) * (gx1[a] + gx0[a]);
+          cxd[a] = R(0.5) * (divx<Vw>(m, A2.phd[a] - A1.phd[a]) + divx<Vw>(m, A1.phd[a] - A0.phd[a]));
+          sy1[a] = R(0.5) * (A1.phd[a] + A1.ph[a]);
+          syu[a] = R(0.5) * (A1.ph[a] + A1.phu[a]);
+        }
+        if (!have_back) face_x<N>(m, sx0, gx0, cy0, cy1, fb);
+        FaceAn<N, R> ff, fr, fl;
+        face_x<N>(m, sx1, gx1, cy1, cy2, ff);
+        face_y<N>(m, sy1, gy1, cxc, cxd, fr);
+        face_shuffle_up<N>(fr, fl);
+        R psi[N];
+#pragma unroll
+        for (int b = 0; b < N; b++) {
+          R p = R(0.0);
+#pragma unroll
+          for (int k = 0; k < NC; k++) p -= A1.mu[k] * cc1[b][k];
+          p += free_energy<NC>(m, cc1[b], b);
+          psi[b] = p;
+        }
+        dphi_cell<N, NC, Vw>(m, A1.ph, divphi, psi, cxc, cy1, ff, fb, fr, fl, sx1, sx0, sy1, syu, dph);
+        fb = ff;
+        have_back = true;
+      } else {
+        have_back = false;
+      }
+
+      /* anti-trapping currents: right y-face of row x, x-face between x-1 and x */
+      R jy1[NC], jx0[NC];
+#pragma unroll
+      for (int k = 0; k < NC; k++) jy1[k] = jx0[k] = R(0.0);
+      bool jbusy = false;
+#pragma unroll
+      for (int a = 0; a < N - 1; a++) jbusy = jbusy || (dph[a] != R(0.0)) || (dph_prev[a] != R(0.0));
+      R dph_dn[N];
+#pragma unroll
+      for (int a = 0; a < N; a++) dph_dn[a] = lane_dn(dph[a]);
+#pragma unroll
+      for (int a = 0; a < N - 1; a++) jbusy = jbusy || (dph_dn[a] != R(0.0));
+      if (__any(jbusy)) {
+        R cyA0[N], cyA1[N], cxA1[N], cxA1d[N], gyA1[N];
+#pragma unroll
+        for (int a = 0; a < N; a++) {
+          cyA0[a] = div2y<Vw>(m, A0.phd[a] - A0.phu[a]);
+          cyA1[a] = div2y<Vw>(m, A1.phd[a] - A1.phu[a]);
+          cxA1[a] = R(0.5) * (gx1[a] + gx0[a]);
+          cxA1d[a] = R(0.5) * (divx<Vw>(m, A2.phd[a] - A1.phd[a]) + divx<Vw>(m, A1.phd[a] - A0.phd[a]));
+          gyA1[a] = gy1[a];
+        }
+        /* y-face (x, y+1/2): normal gradient gy, tangential average of cx over the two cells */
+        if (y != 0 && y != m.jat_skip_row) {
+          R cco[N][NC], gt[N];
+          cmu_all<N, NC>(m, A1.mud, cco);
+#pragma unroll
+          for (int a = 0; a < N; a++) gt[a] = R(0.5) * (cxA1[a] + cxA1d[a]);
+          jat_face<N, NC>(m, A1.ph, A1.phd, cc1, cco, dph, dph_dn, gyA1, gt, jy1);
+        }
+        /* x-face (x-1 .. x): normal gradient gx0, tangential average of cy over rows x-1, x */
+        if (x >= xa) {
+          R gt[N];
+#pragma unroll
+          for (int a = 0; a < N; a++) gt[a] = R(0.5) * (cyA0[a] + cyA1[a]);
+          jat_face<N, NC>(m, A0.ph, A1.ph, cc_prev, cc1, dph_prev, dph, gx0, gt, jx0);
+        }
+      }
+
+      /* mobility fluxes (calculate_gradients_concentration_2D) */
+      R M1[NC][NC], Fx0[NC][NC];
+      mobility_cell<N, NC>(m, A1.ph, M1);
+      if (x >= xa) {
+#pragma unroll
+        for (int k = 0; k < NC; k++)
+#pragma unroll
+          for (int l = 0; l < NC; l++)
+            Fx0[k][l] = ((M1[k][l] + M_prev[k][l]) * R(0.5)) * divx<Vw>(m, (A1.mu[l] - A0.mu[l]));
+      }
+
+      /* update of row x-1 (calculate_divergence_concentration_2D body) */
+      if (x >= xa + 1) {
+        R Fy[NC][NC], Fyl[NC][NC], jyl[NC];
+#pragma unroll
+        for (int k = 0; k < NC; k++) {
+          jyl[k] = lane_up(jy_prev[k]);
+#pragma unroll
+          for (int l = 0; l < NC; l++) {
+            const R Md = lane_dn(M_prev[k][l]);
+            Fy[k][l] = ((Md + M_prev[k][l]) * R(0.5)) * divy<Vw>(m, (A0.mud[l] - A0.mu[l]));
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < NC; k++)
+#pragma unroll
+          for (int l = 0; l < NC; l++) Fyl[k][l] = lane_up(Fy[k][l]);
+
+        const R *phc = A0.ph, *muc = A0.mu;
+        R sum[NC], divflux[NC], divjat[NC], deltamu[NC], munew[NC];
+#pragma unroll
+        for (int k = 0; k < NC; k++) {
+          sum[k] = R(0.0);
+          deltamu[k] = R(0.0);
+          munew[k] = muc[k];
+        }
+#pragma unroll
+        for (int a = 0; a < N; a++) {
+          R sdh = R(0.0);
+#pragma unroll
+          for (int b = 0; b < N; b++) sdh += dhphi<N>(phc, a, b) * dph_prev[b];
+#pragma unroll
+          for (int k = 0; k < NC; k++) sum[k] += cc_prev[a][k] * sdh;
+        }
+#pragma unroll
+        for (int k = 0; k < NC; k++) {
+          divflux[k] = R(0.0);
+#pragma unroll
+          for (int l = 0; l < NC; l++) {
+            divflux[k] += divx<Vw>(m, (Fx0[k][l] - Fx_prev[k][l]));
+            divflux[k] += divy<Vw>(m, (Fy[k][l] - Fyl[k][l]));
+          }
+          divjat[k] = divx<Vw>(m, (jx0[k] - jx_prev[k]) * m.jat_fac);
+          divjat[k] += divy<Vw>(m, (jy_prev[k] - jyl[k]) * m.jat_fac);
+        }
+        int bulk = -1;
+#pragma unroll
+        for (int a = 0; a < N; a++)
+          if (bulk < 0 && phc[a] == R(1.0)) bulk = a;
+        if (bulk >= 0) {
+          R c_new[NC];
+#pragma unroll
+          for (int k = 0; k < NC; k++) {
+            R c_old = R(0);
+#pragma unroll
+            for (int a = 0; a < N; a++)
+              if (a == bulk) c_old = cc_prev[a][k];
+            c_new[k] = c_old + m.dt * (divflux[k]) - divjat[k];
+          }
+#pragma unroll
+          for (int k = 0; k < NC; k++) {
+            R nm = R(0);
+#pragma unroll
+            for (int a = 0; a < N; a++)
+              if (a == bulk) nm = Mu<NC>(m, c_new, a, k);
+            deltamu[k] = nm - munew[k];
+            munew[k] = nm;
+          }
+        } else {
+          R deltac[NC], dcdmu[NC][NC], h[N];
+#pragma unroll
+          for (int a = 0; a < N; a++) h[a] = hphi<N>(phc, a);
+#pragma unroll
+          for (int k = 0; k < NC; k++) {
+            deltac[k] = m.dt * (divflux[k]) - divjat[k] - sum[k];
+#pragma unroll
+            for (int l = 0; l < NC; l++) {
+              dcdmu[k][l] = R(0.0);
+#pragma unroll
+              for (int a = 0; a < N; a++) dcdmu[k][l] += m.cmu[a][k][l] * h[a];
+            }
+          }
+          if (m.solve == GP_SOLVE_BINARY && NC == 1) {
+            R inv = R(1.0) / dcdmu[0][0];
+            deltamu[0] = deltac[0] * inv;
+            munew[0] += deltamu[0];
+          } else if (m.solve == GP_SOLVE_TERNARY && NC == 2) {
+            R DET = dcdmu[0][0] * dcdmu[1 % NC][1 % NC] - dcdmu[0][1 % NC] * dcdmu[1 % NC][0];
+            R inv[NC][NC];
+            inv[0][0] = dcdmu[1 % NC][1 % NC] / DET;
+            inv[1 % NC][1 % NC] = dcdmu[0][0] / DET;
+            inv[0][1 % NC] = -dcdmu[0][1 % NC] / DET;
+            inv[1 % NC][0] = -dcdmu[1 % NC][0] / DET;
+#pragma unroll
+            for (int k = 0; k < NC; k++) {
+              deltamu[k] = R(0.0);
+#pragma unroll
+              for (int l = 0; l < NC; l++) deltamu[k] += inv[k][l] * deltac[l];
+            }
+#pragma unroll
+            for (int k = 0; k < NC; k++) munew[k] = deltamu[k] + munew[k];
+          } else if (m.solve == GP_SOLVE_DILUTE) {
+#pragma unroll
+            for (int k = 0; k < NC; k++) {
+              R inv = R(1.0) / dcdmu[k][k];
+              deltamu[k] = deltac[k] * inv;
+              munew[k] += deltamu[k];
+            }
+          } else {
+            R inv[NC][NC];
+            lu_inverse<NC>(dcdmu, inv);
+#pragma unroll
+            for (int i = 0; i < NC; i++) {
+              R s = R(0);
+#pragma unroll
+              for (int j = 0; j < NC; j++) s = s + inv[i][j] * deltac[j];
+              deltamu[i] = s;
+            }
+#pragma unroll
+            for (int k = 0; k < NC; k++) munew[k] = deltamu[k] + munew[k];
+          }
+        }
+        R pn[N];
+#pragma unroll
+        for (int b = 0; b < N; b++) pn[b] = phc[b] + dph_prev[b];
+        if (out_lane) {
+          const int64_t xo = x - 1;
+          const int64_t c = xo * ps + y + kYOff;
+#pragma unroll
+          for (int a = 0; a < N; a++) phi_o[a * fs + c] = pn[a];
+#pragma unroll
+          for (int k = 0; k < NC; k++) mu_o[k * fs + c] = munew[k];
+          if (img_y || xo < 7 || xo > mx - 2) {
+            write_images(phi_o, N, pn, 0, maps, xo, y, fs, ps);
+            write_images(mu_o, NC, munew, 1, maps, xo, y, fs, ps);
+          }
+#pragma unroll
+          for (int a = 0; a < N; a++) {
+            const double pv = (double)pn[a], dv = (double)dph_prev[a];
+            st[a] = fmax(st[a], pv);
+            st[N + a] = fmin(st[N + a], pv);
+            st[2 * N + 2 * NC + a] += dv * dv;
+            if (!isfinite(pv)) st[NS - 1] += 1.0;
+          }
+#pragma unroll
+          for (int k = 0; k < NC; k++) {
+            const double mv = (double)munew[k], dv = (double)deltamu[k];
+            st[2 * N + k] = fmax(st[2 * N + k], mv);
+            st[2 * N + NC + k] = fmin(st[2 * N + NC + k], mv);
+            st[3 * N + 2 * NC + k] += dv * dv;
+            if (!isfinite(mv)) st[NS - 1] += 1.0;
+          }
+        }
+      }
+
+      /* carry row x into the next step */
+#pragma unroll
+      for (int a = 0; a < N; a++) dph_prev[a] = dph[a];
+#pragma unroll
+      for (int a = 0; a < N; a++)
+#pragma unroll
+        for (int k = 0; k < NC; k++) cc_prev[a][k] = cc1[a][k];
+#pragma unroll
+      for (int k = 0; k < NC; k++) {
+        jy_prev[k] = jy1[k];
+        if (x >= xa) jx_prev[k] = jx0[k];
+#pragma unroll
+        for (int l = 0; l < NC; l++) {
+          M_prev[k][l] = M1[k][l];
+          if (x >= xa) Fx_prev[k][l] = Fx0[k][l];
+        }
+      }
+      A0 = A1;
+      A1 = A2;
+    }
+  }
+
+  /* statistics: wavefront reduction, one partial per wavefront (statistic-major) */
+#pragma unroll
+  for (int i = 0; i < NS; i++) {
+    double r;
+    if (i < N || (i >= 2 * N && i < 2 * N + NC)) r = wave_max(st[i]);
+    else if (i < 2 * N + 2 * NC) r = wave_min(st[i]);
+    else r = wave_sum(st[i]);
+    if (lane == 0 && w < nwaves) partials[(int64_t)i * nwaves + w] = r;
+  }
+}
+
+}  // namespace gpk

@@ -49,6 +49,8 @@ struct Model {
   R dx, dy, dt, eps, tau;
   R inv_eps;     /* 1.0/epsilon */
   R two_dy;      /* 2.0*deltay */
+  R inv_dx, inv_dy, inv_2dy; /* exact reciprocals when dx, dy are powers of two (pow2 != 0) */
+  int32_t pow2, pad2;
   R w_obst;      /* 16.0/(M_PI*M_PI) */
   R jat_well;    /* 0.5/sqrt(2.0) */
   R jat_fac;     /* -0.25*M_PI*epsilon (obstacle) or -1.0*epsilon (well) */
@@ -57,11 +59,15 @@ struct Model {
   R rot[P][P][2][2], rot_inv[P][P][2][2];
   R D[P][NCM][NCM], A[P][NCM][NCM], cmu[P][NCM][NCM], B[P][NCM], C[P], Bmu[P][NCM];
   R dratio[P][NCM]; /* 1.0 - D[a][k][k]/D[L][k][k] */
+  R DC[P][NCM][NCM][NCM]; /* D[b][k][q]*cmu[b][q][l] at [b][k][l][q]: the mobility product, formed once */
 };
 
-/* Stencil accessor over SoA planes: field a at (x+i, y+j) relative to the centre index c. */
-template <typename R>
+/* Stencil accessor over SoA planes: field a at (x+i, y+j) relative to the centre index c.
+ * POW2: dx and dy are powers of two, so v/dx == v*(1/dx) exactly (the quotient and the product are the
+ * same real number, rounded once) and the divisions by the spacings become multiplications. */
+template <typename R, bool POW2 = false>
 struct View {
+  static constexpr bool kPow2 = POW2;
   const R *phi, *mu, *dphi;
   int64_t fs, ps, c;
   GP_HD R ph(int a, int i, int j) const { return phi[a * fs + c + i * ps + j]; }
@@ -74,17 +80,34 @@ struct View {
   }
 };
 
+/* v/dx, v/dy, v/(2 dy) */
+template <class V, typename R>
+GP_HD R divx(const Model<R> &m, R v) {
+  if constexpr (V::kPow2) return v * m.inv_dx;
+  else return v / m.dx;
+}
+template <class V, typename R>
+GP_HD R divy(const Model<R> &m, R v) {
+  if constexpr (V::kPow2) return v * m.inv_dy;
+  else return v / m.dy;
+}
+template <class V, typename R>
+GP_HD R div2y(const Model<R> &m, R v) {
+  if constexpr (V::kPow2) return v * m.inv_2dy;
+  else return v / m.two_dy;
+}
+
 /* ------------------------------------------------------------------ face quantities at offset (i,j) */
 template <typename R, class V>
-GP_HD R gx(const Model<R> &m, const V &v, int a, int i, int j) { return (v.ph(a, i + 1, j) - v.ph(a, i, j)) / m.dx; }
+GP_HD R gx(const Model<R> &m, const V &v, int a, int i, int j) { return divx<V>(m, v.ph(a, i + 1, j) - v.ph(a, i, j)); }
 template <typename R, class V>
 GP_HD R sx(const Model<R> &, const V &v, int a, int i, int j) { return R(0.5) * (v.ph(a, i + 1, j) + v.ph(a, i, j)); }
 template <typename R, class V>
-GP_HD R gy(const Model<R> &m, const V &v, int a, int i, int j) { return (v.ph(a, i, j + 1) - v.ph(a, i, j)) / m.dy; }
+GP_HD R gy(const Model<R> &m, const V &v, int a, int i, int j) { return divy<V>(m, v.ph(a, i, j + 1) - v.ph(a, i, j)); }
 template <typename R, class V>
 GP_HD R sy(const Model<R> &, const V &v, int a, int i, int j) { return R(0.5) * (v.ph(a, i, j + 1) + v.ph(a, i, j)); }
 template <typename R, class V>
-GP_HD R cy(const Model<R> &m, const V &v, int a, int i, int j) { return (v.ph(a, i, j + 1) - v.ph(a, i, j - 1)) / m.two_dy; }
+GP_HD R cy(const Model<R> &m, const V &v, int a, int i, int j) { return div2y<V>(m, v.ph(a, i, j + 1) - v.ph(a, i, j - 1)); }
 template <typename R, class V>
 GP_HD R cx(const Model<R> &m, const V &v, int a, int i, int j) { return R(0.5) * (gx(m, v, a, i, j) + gx(m, v, a, i - 1, j)); }
 
@@ -274,8 +297,8 @@ GP_HD R divdAdgradphi_A01(const Model<R> &m, const V &v, int a) {
     R Bsxa = sx(m, v, a, -1, 0), Bsxb = sx(m, v, b, -1, 0), Bgxa = gx(m, v, a, -1, 0), Bgxb = gx(m, v, b, -1, 0);
     R sya = sy(m, v, a, 0, 0), syb = sy(m, v, b, 0, 0), gya = gy(m, v, a, 0, 0), gyb = gy(m, v, b, 0, 0);
     R Lsya = sy(m, v, a, 0, -1), Lsyb = sy(m, v, b, 0, -1), Lgya = gy(m, v, a, 0, -1), Lgyb = gy(m, v, b, 0, -1);
-    R d1 = ((acf2 * (sxa * gxb - sxb * gxa) * (-sxb)) - (acb2 * (Bsxa * Bgxb - Bsxb * Bgxa) * (-Bsxb))) / m.dx;
-    d1 += ((acr2 * (sya * gyb - syb * gya) * (-syb)) - (acl2 * (Lsya * Lgyb - Lsyb * Lgya) * (-Lsyb))) / m.dy;
+    R d1 = divx<V>(m, ((acf2 * (sxa * gxb - sxb * gxa) * (-sxb)) - (acb2 * (Bsxa * Bgxb - Bsxb * Bgxa) * (-Bsxb))));
+    d1 += divy<V>(m, ((acr2 * (sya * gyb - syb * gya) * (-syb)) - (acl2 * (Lsya * Lgyb - Lsyb * Lgya) * (-Lsyb))));
     d1 *= m.gamma[a][b];
     sum1 += d1;
     R df[2], db[2], dr[2], dl[2], rf[2], rb[2], rr[2], rl[2];
@@ -288,8 +311,8 @@ GP_HD R divdAdgradphi_A01(const Model<R> &m, const V &v, int a) {
     aniso_dadq(dab, ql, dl);
     rot2(m.rot_inv[a][b], dl, rl);
     R qf2 = qnorm2(qf), qb2 = qnorm2(qb), qr2 = qnorm2(qr), ql2 = qnorm2(ql);
-    R d2 = (acf * qf2 * rf[0] * (-sxb) - acb * qb2 * rb[0] * (-Bsxb)) / m.dx;
-    d2 += (acr * qr2 * rr[1] * (-syb) - acl * ql2 * rl[1] * (-Lsyb)) / m.dy;
+    R d2 = divx<V>(m, (acf * qf2 * rf[0] * (-sxb) - acb * qb2 * rb[0] * (-Bsxb)));
+    d2 += divy<V>(m, (acr * qr2 * rr[1] * (-syb) - acl * ql2 * rl[1] * (-Lsyb)));
     d2 *= m.gamma[a][b];
     sum2 += d2;
   }
@@ -320,8 +343,8 @@ GP_HD R divdAdgradphi_iso(const Model<R> &m, const V &v, int a) {
     R Bsxa = sx(m, v, a, -1, 0), Bsxb = sx(m, v, b, -1, 0), Bgxa = gx(m, v, a, -1, 0), Bgxb = gx(m, v, b, -1, 0);
     R sya = sy(m, v, a, 0, 0), syb = sy(m, v, b, 0, 0), gya = gy(m, v, a, 0, 0), gyb = gy(m, v, b, 0, 0);
     R Lsya = sy(m, v, a, 0, -1), Lsyb = sy(m, v, b, 0, -1), Lgya = gy(m, v, a, 0, -1), Lgyb = gy(m, v, b, 0, -1);
-    R d = ((sxa * gxb - sxb * gxa) * (-sxb) - ((Bsxa * Bgxb - Bsxb * Bgxa) * (-Bsxb))) / m.dx;
-    d += ((sya * gyb - syb * gya) * (-syb) - ((Lsya * Lgyb - Lsyb * Lgya) * (-Lsyb))) / m.dy;
+    R d = divx<V>(m, ((sxa * gxb - sxb * gxa) * (-sxb) - ((Bsxa * Bgxb - Bsxb * Bgxa) * (-Bsxb))));
+    d += divy<V>(m, ((sya * gyb - syb * gya) * (-syb) - ((Lsya * Lgyb - Lsyb * Lgya) * (-Lsyb))));
     d *= m.gamma[a][b];
     sum += d;
   }
@@ -424,8 +447,8 @@ GP_HD void phi_delta(const Model<R> &m, const V &v, bool smooth, R *out) {
   for (int k = 0; k < NC; k++) mu[k] = v.m(k, 0, 0);
   bool any = false;
   for (int a = 0; a < N; a++) {
-    divphi[a] = (gx(m, v, a, 0, 0) - gx(m, v, a, -1, 0)) / m.dx;
-    divphi[a] += (gy(m, v, a, 0, 0) - gy(m, v, a, 0, -1)) / m.dy;
+    divphi[a] = divx<V>(m, (gx(m, v, a, 0, 0) - gx(m, v, a, -1, 0)));
+    divphi[a] += divy<V>(m, (gy(m, v, a, 0, 0) - gy(m, v, a, 0, -1)));
     any = any || (fabs(divphi[a]) > R(0.0));
   }
   if (!any) {
@@ -491,6 +514,11 @@ GP_HD void jat_x(const Model<R> &m, const V &v0, int i, int j, R *jx) {
   const V v = v0.shift(i, j);
   const int L = N - 1;
   for (int k = 0; k < NC; k++) jx[k] = R(0.0);
+  /* Every term carries a factor delta-phi_a of one of the two face cells: with both zero each term is a
+   * signed zero and j stays +0, so the face is skipped (same value). */
+  bool any = false;
+  for (int a = 0; a < N - 1; a++) any = any || (v.dp(a, 0, 0) != R(0.0)) || (v.dp(a, 1, 0) != R(0.0));
+  if (!any) return;
   R phc[N], phf[N], muc[NC], muf[NC];
   for (int a = 0; a < N; a++) {
     phc[a] = v.ph(a, 0, 0);
@@ -535,6 +563,9 @@ GP_HD void jat_y(const Model<R> &m, const V &v0, int i, int j, int64_t yglob, R 
   const int L = N - 1;
   for (int k = 0; k < NC; k++) jy[k] = R(0.0);
   if (yglob == 0 || yglob == m.jat_skip_row) return;
+  bool any = false;
+  for (int a = 0; a < N - 1; a++) any = any || (v.dp(a, 0, 0) != R(0.0)) || (v.dp(a, 0, 1) != R(0.0));
+  if (!any) return;
   R phc[N], phr[N], muc[NC], mur[NC];
   for (int a = 0; a < N; a++) {
     phc[a] = v.ph(a, 0, 0);
@@ -577,7 +608,7 @@ template <int N, int NC, typename R, class V>
 GP_HD R mobility(const Model<R> &m, const V &v, int i, int j, int k, int l) {
   R s = R(0.0);
   for (int b = 0; b < N; b++)
-    for (int q = 0; q < NC; q++) s += m.D[b][k][q] * m.cmu[b][q][l] * v.ph(b, i, j);
+    for (int q = 0; q < NC; q++) s += m.DC[b][k][l][q] * v.ph(b, i, j);
   return s;
 }
 
@@ -585,7 +616,7 @@ GP_HD R mobility(const Model<R> &m, const V &v, int i, int j, int k, int l) {
 template <int N, int NC, typename R, class V>
 GP_HD void mu_face_x(const Model<R> &m, const V &v, int i, int j, R (*Dm)[NC], R *gm) {
   for (int k = 0; k < NC; k++) {
-    gm[k] = (v.m(k, i + 1, j) - v.m(k, i, j)) / m.dx;
+    gm[k] = divx<V>(m, (v.m(k, i + 1, j) - v.m(k, i, j)));
     for (int l = 0; l < NC; l++) Dm[k][l] = (mobility<N, NC>(m, v, i + 1, j, k, l) + mobility<N, NC>(m, v, i, j, k, l)) * R(0.5);
   }
 }
@@ -593,7 +624,7 @@ GP_HD void mu_face_x(const Model<R> &m, const V &v, int i, int j, R (*Dm)[NC], R
 template <int N, int NC, typename R, class V>
 GP_HD void mu_face_y(const Model<R> &m, const V &v, int i, int j, R (*Dm)[NC], R *gm) {
   for (int k = 0; k < NC; k++) {
-    gm[k] = (v.m(k, i, j + 1) - v.m(k, i, j)) / m.dy;
+    gm[k] = divy<V>(m, (v.m(k, i, j + 1) - v.m(k, i, j)));
     for (int l = 0; l < NC; l++) Dm[k][l] = (mobility<N, NC>(m, v, i, j + 1, k, l) + mobility<N, NC>(m, v, i, j, k, l)) * R(0.5);
   }
 }
@@ -680,12 +711,12 @@ GP_HD void mu_update(const Model<R> &m, const V &v, bool smooth, const R *dph, c
   for (int k = 0; k < NC; k++) {
     divflux[k] = R(0.0);
     for (int l = 0; l < NC; l++) {
-      divflux[k] += (Dx[k][l] * gmx[l] - Dxb[k][l] * gmxb[l]) / m.dx;
-      divflux[k] += (Dy[k][l] * gmy[l] - Dyl[k][l] * gmyl[l]) / m.dy;
+      divflux[k] += divx<V>(m, (Dx[k][l] * gmx[l] - Dxb[k][l] * gmxb[l]));
+      divflux[k] += divy<V>(m, (Dy[k][l] * gmy[l] - Dyl[k][l] * gmyl[l]));
     }
     if (!smooth) {
-      divjat[k] = (jx0[k] - jxb[k]) * m.jat_fac / m.dx;
-      divjat[k] += (jy0[k] - jyl[k]) * m.jat_fac / m.dy;
+      divjat[k] = divx<V>(m, (jx0[k] - jxb[k]) * m.jat_fac);
+      divjat[k] += divy<V>(m, (jy0[k] - jyl[k]) * m.jat_fac);
     }
   }
   int bulk = -1;
@@ -769,6 +800,14 @@ inline int make_model(const gp_params *p, Model<R> *m) {
   m->tau = (R)p->tau;
   m->inv_eps = (R)(1.0 / p->epsilon);
   m->two_dy = (R)(2.0 * p->dy);
+  {
+    int ex, ey;
+    const bool px = frexp(p->dx, &ex) == 0.5, py = frexp(p->dy, &ey) == 0.5;
+    m->pow2 = (px && py) ? 1 : 0;
+    m->inv_dx = (R)(1.0 / p->dx);
+    m->inv_dy = (R)(1.0 / p->dy);
+    m->inv_2dy = (R)(1.0 / (2.0 * p->dy));
+  }
   m->w_obst = (R)(16.0 / (kPi * kPi));
   m->jat_well = (R)(0.5 / sqrt(2.0));
   m->jat_fac = p->fn_w == 1 ? (R)(-0.25 * kPi * p->epsilon) : (R)(-1.0 * p->epsilon);
@@ -801,6 +840,9 @@ inline int make_model(const gp_params *p, Model<R> *m) {
       m->Bmu[a][i] = (R)(p->Beq[a * NCM + i] + p->dBbdT[a * NCM + i] * (p->T - p->Teq));
       if (a < N && i < NC && a < L)
         m->dratio[a][i] = (R)(1.0 - p->diffusivity[(a * NCM + i) * NCM + i] / p->diffusivity[(L * NCM + i) * NCM + i]);
+      for (int l = 0; l < NCM; l++)
+        for (int q = 0; q < NCM; q++)
+          m->DC[a][i][l][q] = (R)(p->diffusivity[(a * NCM + i) * NCM + q] * p->cmu[(a * NCM + q) * NCM + l]);
     }
     m->C[a] = (R)p->C[a];
   }

@@ -9,15 +9,16 @@
 
 using namespace gpk;
 
-template <int N, int NC>
+template <int N, int NC, bool P2>
 static int run(const gp_params *p, double *phi, double *mu, double *dphi, int64_t nsteps, int smooth) {
   Model<double> m;
   if (make_model(p, &m)) return -1;
+  if (P2 && !m.pow2) return -3;
   const int64_t rx = p->mesh_x + 6, ry = p->mesh_y + 6, n = rx * ry;
   std::vector<double> pn(phi, phi + N * n), mn(mu, mu + NC * n);
   std::vector<double> dp(N * n, 0.0);
   for (int64_t t = 0; t < nsteps; t++) {
-    View<double> v{phi, mu, dp.data(), n, ry, 0};
+    View<double, P2> v{phi, mu, dp.data(), n, ry, 0};
     for (int64_t x = 2; x <= p->mesh_x + 3; x++)
       for (int64_t y = 2; y <= p->mesh_y + 3; y++) {
         double d[N];
@@ -26,7 +27,7 @@ static int run(const gp_params *p, double *phi, double *mu, double *dphi, int64_
       }
     for (int64_t x = 3; x <= p->mesh_x + 2; x++)
       for (int64_t y = 3; y <= p->mesh_y + 2; y++) {
-        View<double> c = v.shift((int)x, (int)y);
+        View<double, P2> c = v.shift((int)x, (int)y);
         double jx0[NC], jxb[NC], jy0[NC], jyl[NC], d[N], pnew[N], mnew[NC], dm[NC];
         if (!smooth) {
           jat_x<N, NC>(m, c, 0, 0, jx0);
@@ -55,9 +56,12 @@ static int run(const gp_params *p, double *phi, double *mu, double *dphi, int64_
   return 0;
 }
 
-extern "C" int model_step(const gp_params *p, double *phi, double *mu, double *dphi, int64_t nsteps, int smooth) {
+/* pow2: 1 forces the power-of-two spacing path (error -3 when dx, dy are not powers of two), 0 the division path */
+extern "C" int model_step(const gp_params *p, double *phi, double *mu, double *dphi, int64_t nsteps, int smooth,
+                          int pow2) {
   const int N = p->nphases, NC = p->ncomp - 1;
-#define CASE(a, b) if (N == a && NC == b) return run<a, b>(p, phi, mu, dphi, nsteps, smooth);
+#define CASE(a, b) \
+  if (N == a && NC == b) return pow2 ? run<a, b, true>(p, phi, mu, dphi, nsteps, smooth) : run<a, b, false>(p, phi, mu, dphi, nsteps, smooth);
   CASE(2, 1) CASE(3, 2) CASE(3, 1) CASE(2, 2) CASE(4, 2) CASE(4, 1) CASE(3, 3)
 #undef CASE
   return -2;

@@ -76,6 +76,30 @@ def test_gpu_matches_oracle_dphi_and_halos(name):
     assert max_rel_err(dphi[sl], odphi[sl]) <= GATE
 
 
+@pytest.mark.parametrize("path", ["march", "twopass"])
+@pytest.mark.parametrize("name", ["tern_c1_mini", "quad_k3", "bin_well_rot", "tern_aniso_per"])
+def test_gpu_step_paths_match_oracle(name, path, monkeypatch):
+    """The single-pass march kernel (default) and the two-pass kernels both reproduce the oracle."""
+    _say("[gpu] path", path, name)
+    if path == "twopass":
+        monkeypatch.setenv("GP_PATH", "twopass")
+    else:
+        monkeypatch.delenv("GP_PATH", raising=False)
+    case, _ = load_golden(name)
+    s = _solver(case.params)
+    try:
+        s.upload(case.phi, case.mu)
+        s.step(23)
+        phi, mu = s.download()
+    finally:
+        s.close()
+    ophi, omu = case.phi.copy(), case.mu.copy()
+    oracle_step(case.params, ophi, omu, 23)
+    ep, em = max_rel_err(phi, ophi), max_rel_err(mu, omu)
+    _say("   max rel err phi %.3g mu %.3g" % (ep, em))
+    assert ep <= GATE and em <= GATE
+
+
 def test_gpu_stats_match_oracle():
     from microsim_amd._abi import GPStats
     import gp_testlib
