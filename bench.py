@@ -66,7 +66,7 @@ def run_mode(case, precision: int, steps: int, warmup: int):
         el = time.perf_counter() - t0
         prof = s.profile_read()
         s.profile_enable(False)
-        st = s.stats()
+        st = s.step(1, stats=True)  # one reporting step after the timed region (non-finite check)
     step_ms = sum(ms for name, (ms, cnt) in prof.items() if name != "k_stats")
     dom = max((k for k in prof if k != "k_stats"), key=lambda k: prof[k][0])
     b_alg = 2 * (N + NC) * es

@@ -149,8 +149,11 @@ int gp_download_fields(gp_handle *h, gp_cell *cells, int64_t ncells);
 /* nsteps pre-conditioning steps (the NSMOOTH-1 loop): no driving force, no anti-trapping current. */
 int gp_smooth(gp_handle *h, int64_t nsteps);
 
-/* nsteps explicit time steps (phi sweep, BC, mu sweep, BC). stats may be NULL; when given it receives the
- * diagnostics of the last step (synchronises the stream). */
+/* nsteps explicit time steps (phi sweep, BC, mu sweep, BC). stats may be NULL; when given, the statistics of
+ * the last of these steps are folded into the cumulative ones and returned (synchronises the stream), and
+ * GP_E_NAN is returned if that step produced a non-finite value. Steps of a call without stats skip the
+ * statistics work, so cumulative max/min cover the steps that reported (the reference accumulates them every
+ * step; reporting every step, nsteps = 1, gives the same cumulative values). */
 int gp_step(gp_handle *h, int64_t nsteps, gp_stats *stats);
 
 /* Diagnostics of the current state without stepping (max/min, solute, energy; rel_change of the last step). */
@@ -175,7 +178,8 @@ const char *gp_version(void);
 #define GP_KERNEL_MU 1       /* anti-trapping + mu/phi update kernel */
 #define GP_KERNEL_STATS 2    /* statistics fold */
 #define GP_KERNEL_FUSED 3    /* single-pass step kernel */
-#define GP_NKERNELS 4
+#define GP_KERNEL_FUSED_STATS 4 /* single-pass step kernel with the statistics reduction */
+#define GP_NKERNELS 5
 int gp_profile_enable(gp_handle *h, int enable);
 int gp_profile_read(gp_handle *h, double *avg_ms, int64_t *count);
 const char *gp_kernel_name(int kind);

@@ -237,7 +237,7 @@ __global__ __launch_bounds__(kStatThreads) void k_stats(const double *__restrict
 
 /* ------------------------------------------------------------------------------------------------ dispatch */
 
-enum { MODE_SMOOTH = 0, MODE_TWOPASS = 1, MODE_MARCH = 2, MODE_DPHI_PREV = 3 };
+enum { MODE_SMOOTH = 0, MODE_TWOPASS = 1, MODE_MARCH = 2, MODE_DPHI_PREV = 3, MODE_MARCH_STATS = 4 };
 
 /* Segment length of the march: enough wavefronts for about two rounds of the resident capacity, at least
  * 16 rows per segment (each segment recomputes two rows of delta-phi). */
@@ -246,7 +246,8 @@ static void march_plan(gp_handle *h) {
   const int64_t mx = h->p.mesh_x, my = h->p.mesh_y;
   int ncu = 0, nb = 0;
   hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device);
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_march<N, NC, R, P2>, 256, 0) != hipSuccess || nb < 1) nb = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_march<N, NC, R, P2, false>, 256, 0) != hipSuccess || nb < 1)
+    nb = 1;
   if (ncu < 1) ncu = 1;
   const int64_t nstrips = (my + kMarchOut - 1) / kMarchOut;
   const int64_t target = 2 * (int64_t)nb * 4 * ncu;
@@ -280,16 +281,21 @@ static int launch_step_p(gp_handle *h, int mode) {
     HIPCHK(h, hipGetLastError());
     return GP_OK;
   }
-  if (mode == MODE_MARCH) {
+  if (mode == MODE_MARCH || mode == MODE_MARCH_STATS) {
     if (h->march_nwaves == 0) march_plan<N, NC, R, P2>(h);
     const unsigned nblk = (unsigned)((h->march_nwaves + 3) / 4);
-    {
+    if (mode == MODE_MARCH) {
       ProfScope ps(h, GP_KERNEL_FUSED);
-      hipLaunchKernelGGL((k_march<N, NC, R, P2>), dim3(nblk), dim3(256), 0, h->stream, m, phi, mu, phi_o, mu_o,
-                         h->maps, h->partials, h->fs, h->ps, mx, my, h->march_seg, h->march_nstrips,
+      hipLaunchKernelGGL((k_march<N, NC, R, P2, false>), dim3(nblk), dim3(256), 0, h->stream, m, phi, mu, phi_o,
+                         mu_o, h->maps, h->partials, h->fs, h->ps, mx, my, h->march_seg, h->march_nstrips,
                          (int)h->march_nwaves);
-    }
-    {
+    } else {
+      {
+        ProfScope ps(h, GP_KERNEL_FUSED_STATS);
+        hipLaunchKernelGGL((k_march<N, NC, R, P2, true>), dim3(nblk), dim3(256), 0, h->stream, m, phi, mu, phi_o,
+                           mu_o, h->maps, h->partials, h->fs, h->ps, mx, my, h->march_seg, h->march_nstrips,
+                           (int)h->march_nwaves);
+      }
       ProfScope ps(h, GP_KERNEL_STATS);
       hipLaunchKernelGGL((k_stats<N, NC>), dim3(3 * N + 3 * NC + 1), dim3(kStatThreads), 0, h->stream,
                          h->partials, (int64_t)h->march_nwaves, h->stats_dev);
@@ -650,7 +656,8 @@ int gp_step(gp_handle *h, int64_t nsteps, gp_stats *stats) {
   if (!h || nsteps < 0) return h ? fail(h, GP_E_ARG, "bad nsteps") : GP_E_ARG;
   step_fn f = pick_any(h);
   for (int64_t t = 0; t < nsteps; t++) {
-    int rc = f(h, h->use_march ? MODE_MARCH : MODE_TWOPASS);
+    const bool report = stats != nullptr && t == nsteps - 1;
+    int rc = f(h, h->use_march ? (report ? MODE_MARCH_STATS : MODE_MARCH) : MODE_TWOPASS);
     if (rc) return rc;
     h->steps++;
     h->prev_valid = h->use_march;
@@ -670,7 +677,7 @@ int gp_profile_enable(gp_handle *h, int enable) {
 }
 
 const char *gp_kernel_name(int kind) {
-  static const char *names[GP_NKERNELS] = {"k_phi", "k_mu", "k_stats", "k_step_fused"};
+  static const char *names[GP_NKERNELS] = {"k_phi", "k_mu", "k_stats", "k_march", "k_march_stats"};
   return (kind >= 0 && kind < GP_NKERNELS) ? names[kind] : "?";
 }
 

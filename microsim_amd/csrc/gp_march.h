@@ -40,7 +40,7 @@ struct Pairs {
 template <int N, int NC, typename R>
 struct RowRaw {
   R ph[N], phu[N], phd[N];
-  R mu[NC], muu[NC], mud[NC];
+  R mu[NC], mud[NC];
 };
 
 template <int N, int NC, typename R>
@@ -61,7 +61,6 @@ __device__ __forceinline__ void row_halo(RowRaw<N, NC, R> &r) {
   }
 #pragma unroll
   for (int k = 0; k < NC; k++) {
-    r.muu[k] = lane_up(r.mu[k]);
     r.mud[k] = lane_dn(r.mu[k]);
   }
 }
@@ -325,7 +324,8 @@ __device__ __forceinline__ void dphi_cell(const Model<R> &m, const R *phi, const
   project_simplex<N>(phi, divphi, out);
 }
 
-template <int N, int NC, typename R, bool P2>
+/* STATS: accumulate the step statistics (one partial per wavefront); off for the steps that do not report */
+template <int N, int NC, typename R, bool P2, bool STATS>
 __global__ __launch_bounds__(256) void k_march(const Model<R> *__restrict__ mp, const R *__restrict__ phi,
                                                const R *__restrict__ mu, R *__restrict__ phi_o,
                                                R *__restrict__ mu_o, Maps maps, double *__restrict__ partials,
@@ -338,20 +338,23 @@ __global__ __launch_bounds__(256) void k_march(const Model<R> *__restrict__ mp, 
   const int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const int64_t ry = my + 6;
 
-  double st[NS];
+  constexpr int NSA = STATS ? NS : 1;
+  double st[NSA];
+  if (STATS) {
 #pragma unroll
-  for (int a = 0; a < N; a++) {
-    st[a] = -INFINITY;
-    st[N + a] = INFINITY;
-    st[2 * N + 2 * NC + a] = 0.0;
-  }
+    for (int a = 0; a < N; a++) {
+      st[a] = -INFINITY;
+      st[N + a] = INFINITY;
+      st[2 * N + 2 * NC + a] = 0.0;
+    }
 #pragma unroll
-  for (int k = 0; k < NC; k++) {
-    st[2 * N + k] = -INFINITY;
-    st[2 * N + NC + k] = INFINITY;
-    st[3 * N + 2 * NC + k] = 0.0;
+    for (int k = 0; k < NC; k++) {
+      st[2 * N + k] = -INFINITY;
+      st[2 * N + NC + k] = INFINITY;
+      st[3 * N + 2 * NC + k] = 0.0;
+    }
+    st[NSA - 1] = 0.0;
   }
-  st[NS - 1] = 0.0;
 
   if (w < nwaves) {
     const int strip = w % nstrips, seg = w / nstrips;
@@ -368,6 +371,12 @@ __global__ __launch_bounds__(256) void k_march(const Model<R> *__restrict__ mp, 
     row_halo(A0);
     row_load(A1, phi, mu, fs, (xa - 1) * ps + yl + kYOff);
     row_halo(A1);
+    /* software pipeline: the raw row x+2 is in flight while step x computes */
+    R nph[N], nmu[NC];
+#pragma unroll
+    for (int a = 0; a < N; a++) nph[a] = phi[a * fs + xa * ps + yl + kYOff];
+#pragma unroll
+    for (int k = 0; k < NC; k++) nmu[k] = mu[k * fs + xa * ps + yl + kYOff];
 
     R dph_prev[N], cc_prev[N][NC], M_prev[NC][NC], jx_prev[NC], jy_prev[NC], Fx_prev[NC][NC];
 #pragma unroll
@@ -376,8 +385,18 @@ __global__ __launch_bounds__(256) void k_march(const Model<R> *__restrict__ mp, 
     bool have_back = false;
 
     for (int64_t x = xa - 1; x <= xb + 1; x++) {
-      row_load(A2, phi, mu, fs, (x + 1) * ps + yl + kYOff);
+#pragma unroll
+      for (int a = 0; a < N; a++) A2.ph[a] = nph[a];
+#pragma unroll
+      for (int k = 0; k < NC; k++) A2.mu[k] = nmu[k];
       row_halo(A2);
+      if (x + 2 <= xb + 2) {
+        const int64_t ni = (x + 2) * ps + yl + kYOff;
+#pragma unroll
+        for (int a = 0; a < N; a++) nph[a] = phi[a * fs + ni];
+#pragma unroll
+        for (int k = 0; k < NC; k++) nmu[k] = mu[k * fs + ni];
+      }
 
       /* face gradients / averages and centred gradients (calculate_gradients_phasefield_2D) */
       R gx0[N], gx1[N], sx0[N], sx1[N], cy0[N], cy1[N], cy2[N], cxc[N], cxd[N], gy1[N], gyu[N], sy1[N], syu[N];
@@ -617,21 +636,23 @@ __global__ __launch_bounds__(256) void k_march(const Model<R> *__restrict__ mp, 
             write_images(phi_o, N, pn, 0, maps, xo, y, fs, ps);
             write_images(mu_o, NC, munew, 1, maps, xo, y, fs, ps);
           }
+          if (STATS) {
 #pragma unroll
-          for (int a = 0; a < N; a++) {
-            const double pv = (double)pn[a], dv = (double)dph_prev[a];
-            st[a] = fmax(st[a], pv);
-            st[N + a] = fmin(st[N + a], pv);
-            st[2 * N + 2 * NC + a] += dv * dv;
-            if (!isfinite(pv)) st[NS - 1] += 1.0;
-          }
+            for (int a = 0; a < N; a++) {
+              const double pv = (double)pn[a], dv = (double)dph_prev[a];
+              st[a] = fmax(st[a], pv);
+              st[N + a] = fmin(st[N + a], pv);
+              st[2 * N + 2 * NC + a] += dv * dv;
+              if (!isfinite(pv)) st[NSA - 1] += 1.0;
+            }
 #pragma unroll
-          for (int k = 0; k < NC; k++) {
-            const double mv = (double)munew[k], dv = (double)deltamu[k];
-            st[2 * N + k] = fmax(st[2 * N + k], mv);
-            st[2 * N + NC + k] = fmin(st[2 * N + NC + k], mv);
-            st[3 * N + 2 * NC + k] += dv * dv;
-            if (!isfinite(mv)) st[NS - 1] += 1.0;
+            for (int k = 0; k < NC; k++) {
+              const double mv = (double)munew[k], dv = (double)deltamu[k];
+              st[2 * N + k] = fmax(st[2 * N + k], mv);
+              st[2 * N + NC + k] = fmin(st[2 * N + NC + k], mv);
+              st[3 * N + 2 * NC + k] += dv * dv;
+              if (!isfinite(mv)) st[NSA - 1] += 1.0;
+            }
           }
         }
       }
@@ -659,8 +680,9 @@ __global__ __launch_bounds__(256) void k_march(const Model<R> *__restrict__ mp, 
   }
 
   /* statistics: wavefront reduction, one partial per wavefront (statistic-major) */
+  if (!STATS) return;
 #pragma unroll
-  for (int i = 0; i < NS; i++) {
+  for (int i = 0; i < NSA; i++) {
     double r;
     if (i < N || (i >= 2 * N && i < 2 * N + NC)) r = wave_max(st[i]);
     else if (i < 2 * N + 2 * NC) r = wave_min(st[i]);
