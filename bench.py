@@ -5,15 +5,19 @@
 Workload (the headline configuration): ternary three-phase grand-potential model, two solids + liquid,
 4096 x 4096 interior cells (16.8 M), 1024 circular seeds (radius 6-12, Poisson-disc spacing 60, seeded RNG),
 parabolic free energies with random SPD Hessians, 4-fold anisotropy, obstacle potential, periodic sides,
-dt at 0.8 of the explicit limit. A step is one phi sweep + one mu sweep over the whole grid; one
-cell-update is one interior cell advanced by one step.
+dt at 0.8 of the explicit limit; interface energy 0.2 and a liquid fill at driving force 0.04 for both solids,
+so the seeds are supercritical and grow (configs.config_ternary). A step is one phi sweep + one mu sweep over
+the whole grid; one cell-update is one interior cell advanced by one step.
 
-Prints one JSON line. ``value`` is cell-updates/s of the first mode in --modes with the state resident in HBM;
-the other modes appear under ``modes``. ``roofline`` uses the algorithmic bytes 2 (N + NC) sizeof(real) per
-cell-update (phi and mu read once and written once) over the mean duration of the step's kernels, timed with
-HIP events on the solver's stream inside the timed region. ``cpu_baseline`` times the serial reference binary
-(oracle/_ref, built from the reference sources) on a bounded sample: the same ternary model on 1024 x 1024 with
-64 seeds, per-step wall time from two runs that differ only in their step count.
+Prints one JSON line. ``value`` is cell-updates/s of the first mode in --modes with the state resident in HBM,
+every step including the reference loop's per-step statistics (``no_stats`` gives the same steps without
+them); the other modes appear under ``modes``. ``roofline`` is for the dominant kernel: the algorithmic bytes
+2 (N + NC) sizeof(real) per cell-update (phi and mu read once and written once) times the cells of one launch,
+over that kernel's mean duration, timed with HIP events on the solver's stream inside the timed region.
+A streaming-copy kernel is timed on the same device as a measured bandwidth reference. ``cpu_baseline`` times
+the serial reference binary (oracle/_ref, built from the reference sources) on a bounded sample: the same
+ternary model on 1024 x 1024 with 64 seeds, per-step wall time from two runs that differ only in their step
+count.
 """
 from __future__ import annotations
 
@@ -50,38 +54,127 @@ def make_case(mesh: int, seeds: int, seed: int = 1234):
 
 
 def run_mode(case, precision: int, steps: int, warmup: int):
+    """Time `steps` solver steps with the reference loop's per-step statistics (cumulative max/min, sums of
+    squared increments) folded on the device every step, as the driver runs them."""
     from microsim_amd.solver import GPSolver
     N, NC = case.N, case.NC
     cells = int(case.params.mesh_x) * int(case.params.mesh_y)
     es = 8 if precision == 0 else 4
     with GPSolver(case.params, precision) as s:
         s.upload(case.phi, case.mu)
-        s.step(warmup)
+        s.step(warmup, stats=True)
         s.synchronize()
         s.profile_enable(True)
         s.profile_read()
         t0 = time.perf_counter()
-        s.step(steps)
+        st = s.step(steps, stats=True)
         s.synchronize()
         el = time.perf_counter() - t0
         prof = s.profile_read()
+        # the same steps without the statistics work (a caller that does not ask for diagnostics)
+        t0 = time.perf_counter()
+        s.step(steps)
+        s.synchronize()
+        el_ns = time.perf_counter() - t0
+        prof_ns = s.profile_read()
         s.profile_enable(False)
-        st = s.step(1, stats=True)  # one reporting step after the timed region (non-finite check)
-    step_ms = sum(ms for name, (ms, cnt) in prof.items() if name != "k_stats")
-    dom = max((k for k in prof if k != "k_stats"), key=lambda k: prof[k][0])
+    step_ms = sum(ms for ms, cnt in prof.values())
+    dom = max(prof, key=lambda k: prof[k][0])
     b_alg = 2 * (N + NC) * es
-    achieved = cells * b_alg / (step_ms * 1e-3) / 1e9 if step_ms > 0 else 0.0
+    # algorithmic bytes of one launch of the dominant kernel (one step over every interior cell) / its mean time
+    achieved = cells * b_alg / (prof[dom][0] * 1e-3) / 1e9 if prof[dom][0] > 0 else 0.0
     return {
         "value": cells * steps / el,
         "ms_per_step": el * 1e3 / steps,
         "kernels_ms": {k: round(v[0], 4) for k, v in prof.items()},
+        "step_kernels_ms": round(step_ms, 4),
         "dominant": dom,
+        "no_stats": {"value": cells * steps / el_ns, "ms_per_step": el_ns * 1e3 / steps,
+                     "kernels_ms": {k: round(v[0], 4) for k, v in prof_ns.items()}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "+".join(k for k in prof if k != "k_stats"),
-                     "alg_bytes_per_cell_update": b_alg},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_per_launch(dom, precision),
+                     "kernel": dom, "alg_bytes_per_cell_update": b_alg, "cells_per_launch": cells},
         "nonfinite": st["nonfinite"],
     }
+
+
+def copy_bandwidth(nbytes: int = 1 << 31, reps: int = 10):
+    """Streaming-copy reference on the same device: y = -x over nbytes (one elementwise kernel reads nbytes and
+    writes nbytes), mean over reps. Reported next to the vendor peak."""
+    import torch
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        torch.neg(x, out=y)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del x, y
+    return {"bytes_per_launch": 2 * nbytes, "ms": round(ms, 4), "GBps": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}
+
+
+def traffic_per_launch(kernel: str, precision: int):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by the
+    same counters on the copy kernel of known size (profiles/traffic_tern4096.txt); None when not measured."""
+    path = os.path.join(REPO, "profiles", "traffic_tern4096.txt")
+    if not os.path.exists(path):
+        return None
+    key = "%s %s bytes_per_launch" % (kernel, "fp64" if precision == 0 else "fp32")
+    for line in open(path):
+        if line.startswith(key):
+            return float(line.split("=")[1])
+    return None
+
+
+def precision_errors(case, steps: int, checkpoints=(100, 500, 1000, 2000, 5000, 10000)):
+    """Max and L2 (root-mean-square) error against the FP64 mode, over interior cells, for phi, mu and the
+    composition c = sum_b c_b(mu) h_b(phi), at each checkpoint up to `steps`, of
+
+      fp32:              the FP32 production mode from the same initial state;
+      fp64_rounded_init: the FP64 mode started from the initial state rounded to FP32 (the perturbation that
+                         storing the state in FP32 makes before the first step; a floor for any FP32 storage).
+
+    The FP64 mode reproduces the FP64 oracle bit for bit (tests/test_gpu_parity.py), so these are errors against
+    the oracle. Each chunk of steps is issued with statistics (non-finite check)."""
+    from microsim_amd import _abi, host
+    from microsim_amd.solver import GPSolver
+    import numpy as np
+    t0 = time.perf_counter()
+    runs = {"fp64": GPSolver(case.params, _abi.GP_FP64), "fp32": GPSolver(case.params, _abi.GP_FP32),
+            "fp64_rounded_init": GPSolver(case.params, _abi.GP_FP64)}
+    res = {"steps": steps, "mesh": [int(case.params.mesh_x), int(case.params.mesh_y)], "at": []}
+    sl = (slice(None), slice(3, -3), slice(3, -3))
+    try:
+        runs["fp64"].upload(case.phi, case.mu)
+        runs["fp32"].upload(case.phi, case.mu)
+        runs["fp64_rounded_init"].upload(case.phi.astype(np.float32).astype(np.float64),
+                                          case.mu.astype(np.float32).astype(np.float64))
+        done = 0
+        for cp in sorted(c for c in set(checkpoints) | {steps} if c <= steps):
+            for r in runs.values():
+                r.step(cp - done, stats=True)
+            done = cp
+            ref = [np.ascontiguousarray(a[sl]) for a in runs["fp64"].download()]
+            cref = host.composition(case.params, ref[0], ref[1])
+            row = {"step": cp}
+            for name in ("fp32", "fp64_rounded_init"):
+                got = [np.ascontiguousarray(a[sl]) for a in runs[name].download()]
+                cgot = host.composition(case.params, got[0], got[1])
+                row[name] = {}
+                for f, a, b in (("phi", got[0], ref[0]), ("mu", got[1], ref[1]), ("c", cgot, cref)):
+                    d = a - b
+                    row[name][f] = {"max": float(np.abs(d).max()), "l2": float(np.sqrt(np.mean(d * d)))}
+            res["at"].append(row)
+            log("precision", json.dumps(row))
+    finally:
+        for r in runs.values():
+            r.close()
+    res["seconds"] = round(time.perf_counter() - t0, 1)
+    return res
 
 
 def cpu_baseline(sample_steps: int = 20):
@@ -130,6 +223,8 @@ def main():
     ap.add_argument("--seeds", type=int, default=1024)
     ap.add_argument("--modes", default="fp32,fp64")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--error-steps", type=int, default=10000,
+                    help="steps of the FP32-vs-FP64 error study at the bench mesh and at 1024^2 (0: skip)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build runs the solver loop on one GPU; --gpus %d is not supported "
@@ -137,6 +232,8 @@ def main():
         sys.exit(2)
     from microsim_amd import _abi
     case, _ = make_case(a.mesh, a.seeds)
+    copy = copy_bandwidth()
+    log("copy kernel", json.dumps(copy))
     res = {}
     for mode in a.modes.split(","):
         prec = _abi.GP_FP32 if mode == "fp32" else _abi.GP_FP64
@@ -151,11 +248,18 @@ def main():
         "config": {"workload": "GP ternary 3-phase %dx%d, %d seeds, %s mode" % (a.mesh, a.mesh, a.seeds, first),
                    "mesh": [a.mesh, a.mesh], "phases": 3, "components": 3, "seeds": a.seeds,
                    "parallelism": "single-gpu"},
-        "roofline": r0["roofline"],
+        "roofline": dict(r0["roofline"], measured_copy_GBps=copy["GBps"]),
+        "copy_kernel": copy,
         "modes": {m: {k: v for k, v in r.items()} for m, r in res.items()},
     }
     if "fp32" in res and "fp64" in res:
         out["fp32_over_fp64"] = res["fp32"]["value"] / res["fp64"]["value"]
+    if a.error_steps > 0:
+        errs = [precision_errors(case, a.error_steps)]
+        if a.mesh != 1024:
+            errs.append(precision_errors(make_case(1024, 64)[0], a.error_steps))
+        out["fp32_error_vs_fp64"] = errs
+        log("fp32 error", json.dumps(errs))
     if not a.no_cpu:
         cb = cpu_baseline()
         if cb is not None:

@@ -149,11 +149,11 @@ int gp_download_fields(gp_handle *h, gp_cell *cells, int64_t ncells);
 /* nsteps pre-conditioning steps (the NSMOOTH-1 loop): no driving force, no anti-trapping current. */
 int gp_smooth(gp_handle *h, int64_t nsteps);
 
-/* nsteps explicit time steps (phi sweep, BC, mu sweep, BC). stats may be NULL; when given, the statistics of
- * the last of these steps are folded into the cumulative ones and returned (synchronises the stream), and
- * GP_E_NAN is returned if that step produced a non-finite value. Steps of a call without stats skip the
- * statistics work, so cumulative max/min cover the steps that reported (the reference accumulates them every
- * step; reporting every step, nsteps = 1, gives the same cumulative values). */
+/* nsteps explicit time steps (phi sweep, BC, mu sweep, BC). stats may be NULL. When given, every one of these
+ * steps folds its max/min into the cumulative ones on the device (as the reference does every step), the sums
+ * of the last step are returned with them (one stream synchronisation at the end), and GP_E_NAN is returned if
+ * a step produced a non-finite value. Steps of a call without stats skip the statistics work entirely; the
+ * cumulative max/min then cover only the steps of calls that passed stats. */
 int gp_step(gp_handle *h, int64_t nsteps, gp_stats *stats);
 
 /* Diagnostics of the current state without stepping (max/min, solute, energy; rel_change of the last step). */

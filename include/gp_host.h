@@ -97,6 +97,9 @@ int gp_write_echo(const gp_input *in, const char *input_path);
 /* Composition field sum_b c_b(mu) h_b(phi) of one cell (the Composition_<k> output). */
 double gp_composition(const gp_params *p, const double *phi_cell, const double *mu_cell, int k);
 
+/* The same for every cell of a halo-inclusive SoA state (phi [N][n], mu [NC][n]) into c [NC][n], n = ncells. */
+void gp_composition_grid(const gp_params *p, const double *phi, const double *mu, double *c, int64_t ncells);
+
 /* sizeof checks for foreign-function mirrors of these structs. */
 size_t gp_sizeof_input(void);
 size_t gp_sizeof_params(void);

@@ -114,6 +114,8 @@ def _declare_host(lib):
     lib.gp_write_echo.argtypes = [C.POINTER(GPInput), C.c_char_p]
     lib.gp_composition.argtypes = [C.POINTER(GPParams), _dp, _dp, C.c_int]
     lib.gp_composition.restype = C.c_double
+    lib.gp_composition_grid.argtypes = [C.POINTER(GPParams), _dp, _dp, _dp, C.c_int64]
+    lib.gp_composition_grid.restype = None
     for f in ("gp_sizeof_input", "gp_sizeof_params", "gp_sizeof_stats"):
         getattr(lib, f).restype = C.c_size_t
         getattr(lib, f).argtypes = []

@@ -347,10 +347,39 @@ def binary_material(inp: GPInput, c_s: float = 0.2, c_l: float = 0.4, c_inf: flo
     return inp
 
 
+def hessian(a: Sequence[float], nc: int) -> np.ndarray:
+    """Hessian of f = sum_{i<=j} A_ij c_i c_j from the ``A`` key tuple (diagonal entries, then i<j pairs)."""
+    H = np.diag([2.0 * a[i] for i in range(nc)])
+    k = nc
+    for i in range(nc):
+        for j in range(i + 1, nc):
+            H[i, j] = H[j, i] = a[k]
+            k += 1
+    return H
+
+
+def fill_for_driving_force(inp: GPInput, driving: float) -> np.ndarray:
+    """Liquid fill composition c that gives every solid a the same linearised grand-potential driving force
+
+        psi_L(mu) - psi_a(mu) ~= (c_a - c_aL) . (mu - mu_a,eq) = (c_a - c_aL)^T H_L (c - c_aL) = driving,
+
+    with c_a = ceq[a][a], c_aL = ceq[a][L] the two-phase equilibrium pair (the reference builds B_a, C_a so
+    that they share mu and psi at Teq, functions/functionF_01.h:62-104) and mu_L(c) = H_L c. With as many
+    solids as components this is a square solve; otherwise the minimum-norm least-squares solution."""
+    N, NC = inp.nphases, inp.ncomp - 1
+    L = N - 1
+    HL = hessian(inp.A[L], NC)
+    R = np.array([HL @ (np.asarray(inp.ceq[(a, a)]) - np.asarray(inp.ceq[(a, L)])) for a in range(N - 1)])
+    rhs = np.array([R[a] @ np.asarray(inp.ceq[(a, L)]) + driving for a in range(N - 1)])
+    return np.linalg.lstsq(R, rhs, rcond=None)[0]
+
+
 def ternary_material(inp: GPInput, rng: np.random.Generator, D_l: float = 1.0, D_s: float = 0.0,
-                     supersat: float = 0.3) -> GPInput:
+                     supersat: float = 0.3, driving: Optional[float] = None) -> GPInput:
     """N-phase ternary material: random SPD Hessians (eigenvalues in [0.5, 2]) and equilibrium compositions
-    uniform in [0.05, 0.45]; the liquid is filled between its equilibrium compositions and the solids'."""
+    uniform in [0.05, 0.45]. The liquid is filled between its equilibrium compositions and the solids'
+    (``supersat``), or, when ``driving`` is given, at the composition that gives every solid that linearised
+    driving force (fill_for_driving_force)."""
     N, NC = inp.nphases, inp.ncomp - 1
     L = N - 1
     inp.A = {p: random_spd_A(rng, NC) for p in range(N)}
@@ -377,6 +406,11 @@ def ternary_material(inp: GPInput, rng: np.random.Generator, D_l: float = 1.0, D
     inp.ceq, inp.cfill, inp.slopes = ceq, cfill, slopes
     inp.diffusivity = {p: [D_s] * NC for p in range(N - 1)}
     inp.diffusivity[L] = [D_l] * NC
+    if driving is not None:
+        cf = fill_for_driving_force(inp, driving)
+        if not np.all((cf > 0.0) & (cf < 1.0)) or cf.sum() >= 1.0:
+            raise ValueError("driving force %g needs an unphysical liquid fill %s" % (driving, cf))
+        cfill[(L, L)] = list(np.round(cf, 4))
     return inp
 
 
@@ -407,15 +441,22 @@ def config_c0(mesh: int = 200, steps: int = 20000, savet: int = 2000) -> Tuple[G
 
 
 def config_ternary(mesh: int, nseeds: int, seed: int = 1234, steps: int = 1000, savet: int = 1000,
-                   dt_frac: float = 0.8, min_spacing: float = 60.0) -> Tuple[GPInput, List[Fill], float]:
+                   dt_frac: float = 0.8, min_spacing: float = 60.0, gamma: float = 0.2,
+                   driving: Optional[float] = 0.04) -> Tuple[GPInput, List[Fill], float]:
     """Ternary three-phase random-seed workload (two solids + liquid), periodic boundaries, dt at ``dt_frac``
-    of the explicit stability limit. Returns (input, fills, dt_limit)."""
+    of the explicit stability limit. Returns (input, fills, dt_limit).
+
+    Interface energy and supersaturation are chosen so that the seeds (radius 6-12) are supercritical and keep
+    growing: the liquid is filled at a driving force of 0.04 for both solids (critical radius gamma/driving =
+    5 cells at gamma = 0.2). With gamma = 1 and compositions in [0.05, 0.45], every seed dissolves within about
+    a thousand steps and the run degenerates into pure diffusion. ``driving=None`` gives the older
+    supersaturation-0.3 fill."""
     rng = np.random.default_rng(seed)
     inp = GPInput(mesh_x=mesh, mesh_y=mesh, nphases=3, ncomp=3, dx=1.0, dy=1.0, ntimesteps=steps, savet=savet,
-                  nsmooth=1, epsilon=4.0, tau=0.3, gamma=upper(3, 1.0), tau_ab=upper(3, 0.3),
+                  nsmooth=1, epsilon=4.0, tau=0.3, gamma=upper(3, gamma), tau_ab=upper(3, 0.3),
                   dab=upper(3, 0.02), gamma_abc=[0.0], components=["Al", "Cu", "Ni"],
                   phases=["alpha", "beta", "liquid"], bc_phi=(3, 3, 3, 3), track_progress=max(1, savet))
-    ternary_material(inp, rng)
+    ternary_material(inp, rng, driving=driving)
     dtl = explicit_dt_limit(inp)
     inp.dt = float("%.6g" % (dt_frac * dtl))
     fills = poisson_disc_seeds(rng, mesh, mesh, nseeds, 6.0, 12.0, min_spacing, 2, margin=13)

@@ -50,7 +50,7 @@ struct gp_handle {
   int device;
   void *phi[2], *mu[2], *dphi;
   void *model;              /* Model<R> in device memory */
-  double *partials;         /* [nblocks][kNStatMax] */
+  double *partials;         /* [statistic][nparts_max], statistic-major */
   double *stats_dev;        /* cumulative stats (kNStatMax) */
   int cur;
   int64_t nblocks_mu;
@@ -73,14 +73,21 @@ struct ProfScope {
   hipEvent_t e0, e1;
   int kind;
   ProfScope(gp_handle *hh, int k) : h(hh), e0(nullptr), e1(nullptr), kind(k) {
-    if (h->profiling && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess)
-      hipEventRecord(e0, h->stream);
+    if (h->profiling && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess &&
+        hipEventRecord(e0, h->stream) != hipSuccess)
+      release();
   }
   ~ProfScope() {
     if (e1) {
-      hipEventRecord(e1, h->stream);
-      h->prof.push_back({kind, {e0, e1}});
+      if (hipEventRecord(e1, h->stream) == hipSuccess) h->prof.push_back({kind, {e0, e1}});
+      else release();
     }
+  }
+  /* a launch whose events could not be recorded is simply not timed */
+  void release() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    e0 = e1 = nullptr;
   }
 };
 
@@ -245,7 +252,7 @@ template <int N, int NC, typename R, bool P2>
 static void march_plan(gp_handle *h) {
   const int64_t mx = h->p.mesh_x, my = h->p.mesh_y;
   int ncu = 0, nb = 0;
-  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device);
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device) != hipSuccess) ncu = 256;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_march<N, NC, R, P2, false>, 256, 0) != hipSuccess || nb < 1)
     nb = 1;
   if (ncu < 1) ncu = 1;
@@ -480,7 +487,10 @@ int gp_create(const gp_params *params, int precision, gp_handle **out) {
     h->maps.hx[k] = (int32_t)hx[k];
     h->maps.hy[k] = (int32_t)hy[k];
   }
-  hipGetDevice(&h->device);
+  if (hipGetDevice(&h->device) != hipSuccess) {
+    delete h;
+    return fail(nullptr, GP_E_NODEV, "hipGetDevice failed");
+  }
   h->nblocks_mu = ((params->mesh_y + kBX - 1) / kBX) * ((params->mesh_x + kBY - 1) / kBY);
   {
     /* the march uses at most one wavefront per (60-cell strip, 16-row segment) */
@@ -538,21 +548,17 @@ int gp_create(const gp_params *params, int precision, gp_handle **out) {
 
 void gp_destroy(gp_handle *h) {
   if (!h) return;
-  if (h->stream) hipStreamSynchronize(h->stream);
+  /* teardown: errors here have no caller to report to, so they are ignored explicitly */
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
   for (auto &r : h->prof) {
-    hipEventDestroy(r.second.first);
-    hipEventDestroy(r.second.second);
+    (void)hipEventDestroy(r.second.first);
+    (void)hipEventDestroy(r.second.second);
   }
   h->prof.clear();
-  for (int i = 0; i < 2; i++) {
-    if (h->phi[i]) hipFree(h->phi[i]);
-    if (h->mu[i]) hipFree(h->mu[i]);
-  }
-  if (h->dphi) hipFree(h->dphi);
-  if (h->partials) hipFree(h->partials);
-  if (h->stats_dev) hipFree(h->stats_dev);
-  if (h->model) hipFree(h->model);
-  if (h->stream) hipStreamDestroy(h->stream);
+  void *bufs[] = {h->phi[0], h->phi[1], h->mu[0], h->mu[1], h->dphi, h->partials, h->stats_dev, h->model};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }
 
@@ -656,8 +662,7 @@ int gp_step(gp_handle *h, int64_t nsteps, gp_stats *stats) {
   if (!h || nsteps < 0) return h ? fail(h, GP_E_ARG, "bad nsteps") : GP_E_ARG;
   step_fn f = pick_any(h);
   for (int64_t t = 0; t < nsteps; t++) {
-    const bool report = stats != nullptr && t == nsteps - 1;
-    int rc = f(h, h->use_march ? (report ? MODE_MARCH_STATS : MODE_MARCH) : MODE_TWOPASS);
+    int rc = f(h, h->use_march ? (stats ? MODE_MARCH_STATS : MODE_MARCH) : MODE_TWOPASS);
     if (rc) return rc;
     h->steps++;
     h->prev_valid = h->use_march;
@@ -692,8 +697,8 @@ int gp_profile_read(gp_handle *h, double *avg_ms, int64_t *count) {
       tot[r.first] += ms;
       cnt[r.first]++;
     }
-    hipEventDestroy(r.second.first);
-    hipEventDestroy(r.second.second);
+    (void)hipEventDestroy(r.second.first);
+    (void)hipEventDestroy(r.second.second);
   }
   h->prof.clear();
   for (int k = 0; k < GP_NKERNELS; k++) {

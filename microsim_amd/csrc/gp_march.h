@@ -41,7 +41,19 @@ template <int N, int NC, typename R>
 struct RowRaw {
   R ph[N], phu[N], phd[N];
   R mu[NC], mud[NC];
+  int pure; /* b when the cell is purely phase b (phi_b == 1, every other phi == 0), else -1 */
 };
+
+template <int N, typename R>
+__device__ __forceinline__ int pure_phase(const R *ph) {
+  int code = -1, nz = 0;
+#pragma unroll
+  for (int a = 0; a < N; a++) {
+    if (ph[a] == R(1.0)) code = a;
+    nz += (ph[a] != R(0.0));
+  }
+  return nz == 1 ? code : -1;
+}
 
 template <int N, int NC, typename R>
 __device__ __forceinline__ void row_load(RowRaw<N, NC, R> &r, const R *__restrict__ phi, const R *__restrict__ mu,
@@ -54,6 +66,7 @@ __device__ __forceinline__ void row_load(RowRaw<N, NC, R> &r, const R *__restric
 
 template <int N, int NC, typename R>
 __device__ __forceinline__ void row_halo(RowRaw<N, NC, R> &r) {
+  r.pure = pure_phase<N>(r.ph);
 #pragma unroll
   for (int a = 0; a < N; a++) {
     r.phu[a] = lane_up(r.ph[a]);
@@ -381,8 +394,43 @@ __global__ __launch_bounds__(256) void k_march(const Model<R> *__restrict__ mp, 
     R dph_prev[N], cc_prev[N][NC], M_prev[NC][NC], jx_prev[NC], jy_prev[NC], Fx_prev[NC][NC];
 #pragma unroll
     for (int a = 0; a < N; a++) dph_prev[a] = R(0);
+#pragma unroll
+    for (int k = 0; k < NC; k++) jx_prev[k] = jy_prev[k] = R(0);
+    bool cc_valid = false; /* cc_prev holds c_mu of row x-1 (the pure-phase steps do not form it) */
     FaceAn<N, R> fb;
     bool have_back = false;
+
+    /* writes the updated row x-1 (and its halo images) and accumulates the statistics */
+    auto emit = [&](int64_t xo, const R *pn, const R *munew, const R *dph_o, const R *deltamu) {
+      if (!out_lane) return;
+      const int64_t c = xo * ps + y + kYOff;
+#pragma unroll
+      for (int a = 0; a < N; a++) phi_o[a * fs + c] = pn[a];
+#pragma unroll
+      for (int k = 0; k < NC; k++) mu_o[k * fs + c] = munew[k];
+      if (img_y || xo < 7 || xo > mx - 2) {
+        write_images(phi_o, N, pn, 0, maps, xo, y, fs, ps);
+        write_images(mu_o, NC, munew, 1, maps, xo, y, fs, ps);
+      }
+      if (STATS) {
+#pragma unroll
+        for (int a = 0; a < N; a++) {
+          const double pv = (double)pn[a], dv = (double)dph_o[a];
+          st[a] = fmax(st[a], pv);
+          st[N + a] = fmin(st[N + a], pv);
+          st[2 * N + 2 * NC + a] += dv * dv;
+          if (!isfinite(pv)) st[NSA - 1] += 1.0;
+        }
+#pragma unroll
+        for (int k = 0; k < NC; k++) {
+          const double mv = (double)munew[k], dv = (double)deltamu[k];
+          st[2 * N + k] = fmax(st[2 * N + k], mv);
+          st[2 * N + NC + k] = fmin(st[2 * N + NC + k], mv);
+          st[3 * N + 2 * NC + k] += dv * dv;
+          if (!isfinite(mv)) st[NSA - 1] += 1.0;
+        }
+      }
+    };
 
     for (int64_t x = xa - 1; x <= xb + 1; x++) {
 #pragma unroll
@@ -397,6 +445,96 @@ __global__ __launch_bounds__(256) void k_march(const Model<R> *__restrict__ mp, 
 #pragma unroll
         for (int k = 0; k < NC; k++) nmu[k] = mu[k * fs + ni];
       }
+
+      /* Pure-phase step: every lane of rows x-1, x, x+1 is purely the same phase b, and row x-1 carries no
+       * delta-phi and no anti-trapping current. Then delta-phi of row x and every anti-trapping current of the
+       * step are exactly zero, each mobility is Mpure[b] and each face average (M + M) * 0.5 equals M, so the
+       * general step below reduces to the mu fluxes and the bulk update of phase b; this path forms exactly
+       * those values, with the same operations in the same order. */
+      {
+        bool lp = A0.pure >= 0 && A0.pure == A1.pure && A1.pure == A2.pure;
+#pragma unroll
+        for (int a = 0; a < N; a++) lp = lp && (dph_prev[a] == R(0.0));
+#pragma unroll
+        for (int k = 0; k < NC; k++) lp = lp && (jx_prev[k] == R(0.0)) && (jy_prev[k] == R(0.0));
+        const int b = __shfl(A1.pure, 0, 64);
+        if (__all(lp && A1.pure == b)) {
+          R Fx0[NC][NC];
+          if (x >= xa) {
+#pragma unroll
+            for (int k = 0; k < NC; k++)
+#pragma unroll
+              for (int l = 0; l < NC; l++) Fx0[k][l] = m.Mpure[b][k][l] * divx<Vw>(m, (A1.mu[l] - A0.mu[l]));
+          }
+          if (x >= xa + 1) {
+            R Fy[NC][NC], Fyl[NC][NC];
+#pragma unroll
+            for (int k = 0; k < NC; k++)
+#pragma unroll
+              for (int l = 0; l < NC; l++) Fy[k][l] = m.Mpure[b][k][l] * divy<Vw>(m, (A0.mud[l] - A0.mu[l]));
+#pragma unroll
+            for (int k = 0; k < NC; k++)
+#pragma unroll
+              for (int l = 0; l < NC; l++) Fyl[k][l] = lane_up(Fy[k][l]);
+            const R zj = R(0.0) * m.jat_fac; /* (j - j_prev) * jat_fac with both currents +0 */
+            R divflux[NC], divjat[NC], deltamu[NC], munew[NC];
+#pragma unroll
+            for (int k = 0; k < NC; k++) {
+              divflux[k] = R(0.0);
+#pragma unroll
+              for (int l = 0; l < NC; l++) {
+                divflux[k] += divx<Vw>(m, (Fx0[k][l] - Fx_prev[k][l]));
+                divflux[k] += divy<Vw>(m, (Fy[k][l] - Fyl[k][l]));
+              }
+              divjat[k] = divx<Vw>(m, zj);
+              divjat[k] += divy<Vw>(m, zj);
+            }
+            if (kDirectBulkMu<R>) {
+              R dc[NC];
+#pragma unroll
+              for (int k = 0; k < NC; k++) dc[k] = m.dt * (divflux[k]) - divjat[k];
+#pragma unroll
+              for (int k = 0; k < NC; k++) {
+                deltamu[k] = Mu_lin<NC>(m, dc, b, k);
+                munew[k] = A0.mu[k] + deltamu[k];
+              }
+            } else {
+              R c_new[NC];
+#pragma unroll
+              for (int k = 0; k < NC; k++) c_new[k] = c_mu<NC>(m, A0.mu, b, k) + m.dt * (divflux[k]) - divjat[k];
+#pragma unroll
+              for (int k = 0; k < NC; k++) {
+                munew[k] = Mu<NC>(m, c_new, b, k);
+                deltamu[k] = munew[k] - A0.mu[k];
+              }
+            }
+            R pn[N];
+#pragma unroll
+            for (int a = 0; a < N; a++) pn[a] = A0.ph[a] + dph_prev[a];
+            emit(x - 1, pn, munew, dph_prev, deltamu);
+          }
+          /* carry: row x has delta-phi 0, no currents, mobility Mpure[b] */
+#pragma unroll
+          for (int a = 0; a < N; a++) dph_prev[a] = R(0.0);
+#pragma unroll
+          for (int k = 0; k < NC; k++) {
+            jy_prev[k] = R(0.0);
+            if (x >= xa) jx_prev[k] = R(0.0);
+#pragma unroll
+            for (int l = 0; l < NC; l++) {
+              M_prev[k][l] = m.Mpure[b][k][l];
+              if (x >= xa) Fx_prev[k][l] = Fx0[k][l];
+            }
+          }
+          cc_valid = false;
+          have_back = false;
+          A0 = A1;
+          A1 = A2;
+          continue;
+        }
+      }
+      if (!cc_valid) cmu_all<N, NC>(m, A0.mu, cc_prev);
+      cc_valid = true;
 
       /* face gradients / averages and centred gradients (calculate_gradients_phasefield_2D) */
       R gx0[N], gx1[N], sx0[N], sx1[N], cy0[N], cy1[N], cy2[N], cxc[N], cxd[N], gy1[N], gyu[N], sy1[N], syu[N];
@@ -549,7 +687,16 @@ __global__ __launch_bounds__(256) void k_march(const Model<R> *__restrict__ mp, 
 #pragma unroll
         for (int a = 0; a < N; a++)
           if (bulk < 0 && phc[a] == R(1.0)) bulk = a;
-        if (bulk >= 0) {
+        if (bulk >= 0 && kDirectBulkMu<R>) {
+          R dc[NC];
+#pragma unroll
+          for (int k = 0; k < NC; k++) dc[k] = m.dt * (divflux[k]) - divjat[k];
+#pragma unroll
+          for (int k = 0; k < NC; k++) {
+            deltamu[k] = Mu_lin<NC>(m, dc, bulk, k);
+            munew[k] += deltamu[k];
+          }
+        } else if (bulk >= 0) {
           R c_new[NC];
 #pragma unroll
           for (int k = 0; k < NC; k++) {
@@ -625,36 +772,7 @@ __global__ __launch_bounds__(256) void k_march(const Model<R> *__restrict__ mp, 
         R pn[N];
 #pragma unroll
         for (int b = 0; b < N; b++) pn[b] = phc[b] + dph_prev[b];
-        if (out_lane) {
-          const int64_t xo = x - 1;
-          const int64_t c = xo * ps + y + kYOff;
-#pragma unroll
-          for (int a = 0; a < N; a++) phi_o[a * fs + c] = pn[a];
-#pragma unroll
-          for (int k = 0; k < NC; k++) mu_o[k * fs + c] = munew[k];
-          if (img_y || xo < 7 || xo > mx - 2) {
-            write_images(phi_o, N, pn, 0, maps, xo, y, fs, ps);
-            write_images(mu_o, NC, munew, 1, maps, xo, y, fs, ps);
-          }
-          if (STATS) {
-#pragma unroll
-            for (int a = 0; a < N; a++) {
-              const double pv = (double)pn[a], dv = (double)dph_prev[a];
-              st[a] = fmax(st[a], pv);
-              st[N + a] = fmin(st[N + a], pv);
-              st[2 * N + 2 * NC + a] += dv * dv;
-              if (!isfinite(pv)) st[NSA - 1] += 1.0;
-            }
-#pragma unroll
-            for (int k = 0; k < NC; k++) {
-              const double mv = (double)munew[k], dv = (double)deltamu[k];
-              st[2 * N + k] = fmax(st[2 * N + k], mv);
-              st[2 * N + NC + k] = fmin(st[2 * N + NC + k], mv);
-              st[3 * N + 2 * NC + k] += dv * dv;
-              if (!isfinite(mv)) st[NSA - 1] += 1.0;
-            }
-          }
-        }
+        emit(x - 1, pn, munew, dph_prev, deltamu);
       }
 
       /* carry row x into the next step */

@@ -60,6 +60,7 @@ struct Model {
   R D[P][NCM][NCM], A[P][NCM][NCM], cmu[P][NCM][NCM], B[P][NCM], C[P], Bmu[P][NCM];
   R dratio[P][NCM]; /* 1.0 - D[a][k][k]/D[L][k][k] */
   R DC[P][NCM][NCM][NCM]; /* D[b][k][q]*cmu[b][q][l] at [b][k][l][q]: the mobility product, formed once */
+  R Mpure[P][NCM][NCM];   /* mobility of a cell that is purely phase b */
 };
 
 /* Stencil accessor over SoA planes: field a at (x+i, y+j) relative to the centre index c.
@@ -154,6 +155,24 @@ GP_HD R Mu(const Model<R> &m, const R *c, int a, int i) {
     if (i != j) s += m.A[a][i][j] * c[j];
   return s;
 }
+
+/* Linear part of Mu: the change of mu_i for a change dc of the composition in phase a (Mu is affine in c). */
+template <int NC, typename R>
+GP_HD R Mu_lin(const Model<R> &m, const R *dc, int a, int i) {
+  R s = R(2.0) * m.A[a][i][i] * dc[i];
+  for (int j = 0; j < NC; j++)
+    if (i != j) s += m.A[a][i][j] * dc[j];
+  return s;
+}
+
+/* Whether the bulk-cell mu increment takes the direct form dmu = Mu_lin(dc). The reference's form
+ * (c_new = c(mu) + dc, mu_new = Mu(c_new); calculate_divergence_concentration.h:71-79) is kept in FP64 for
+ * bit-exact parity. In FP32 its mu -> c -> mu round trip costs about one ulp of mu per step in every bulk cell,
+ * independent of the increment, and those errors random-walk through a long run (about 20x the phi error after
+ * 2000 steps of the ternary seed workload); the direct form, identical in exact arithmetic, rounds relative to
+ * the increment only. */
+template <typename R>
+constexpr bool kDirectBulkMu = sizeof(R) < sizeof(double);
 
 template <int NC, typename R>
 GP_HD R free_energy(const Model<R> &m, const R *c, int a) {
@@ -724,7 +743,14 @@ GP_HD void mu_update(const Model<R> &m, const V &v, bool smooth, const R *dph, c
     if (bulk < 0 && phi[a] == R(1.0)) bulk = a;
   R munew[NC];
   for (int k = 0; k < NC; k++) munew[k] = mu[k];
-  if (bulk >= 0) {
+  if (bulk >= 0 && kDirectBulkMu<R>) {
+    R dc[NC];
+    for (int k = 0; k < NC; k++) dc[k] = smooth ? m.dt * (divflux[k]) : m.dt * (divflux[k]) - divjat[k];
+    for (int k = 0; k < NC; k++) {
+      deltamu[k] = Mu_lin<NC>(m, dc, bulk, k);
+      munew[k] = munew[k] + deltamu[k];
+    }
+  } else if (bulk >= 0) {
     R c_new[NC];
     for (int k = 0; k < NC; k++) {
       R c_old = c_mu<NC>(m, mu, bulk, k);
@@ -846,6 +872,17 @@ inline int make_model(const gp_params *p, Model<R> *m) {
     }
     m->C[a] = (R)p->C[a];
   }
+  /* mobility of a cell that is purely phase b (phi_b = 1, every other phase 0), formed with the same
+   * operations, in the same order and precision, as the per-cell sum in mobility(): bit-identical to it */
+  for (int b = 0; b < P; b++)
+    for (int k = 0; k < NCM; k++)
+      for (int l = 0; l < NCM; l++) {
+        R s = R(0.0);
+        if (b < N && k < NC && l < NC)
+          for (int a = 0; a < N; a++)
+            for (int q = 0; q < NC; q++) s += m->DC[a][k][l][q] * (a == b ? R(1.0) : R(0.0));
+        m->Mpure[b][k][l] = s;
+      }
   return 0;
 }
 

@@ -91,3 +91,13 @@ def write_vtk(case_inp: _abi.GPInput, path: str, phi: np.ndarray, mu: np.ndarray
     rc = lib.gp_write_vtk(C.byref(case_inp), path.encode(), _abi.dptr(phi), _abi.dptr(mu))
     if rc != 0:
         raise IOError("cannot write %s" % path)
+
+
+def composition(params: _abi.GPParams, phi: np.ndarray, mu: np.ndarray) -> np.ndarray:
+    """c_k = sum_b c_b^k(mu) h_b(phi) for every cell (the Composition_<k> output) as [NC, ...] float64."""
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    mu = np.ascontiguousarray(mu, dtype=np.float64)
+    n = int(np.prod(phi.shape[1:]))
+    c = np.empty((mu.shape[0],) + phi.shape[1:])
+    _abi.host_lib().gp_composition_grid(C.byref(params), _abi.dptr(phi), _abi.dptr(mu), _abi.dptr(c), n)
+    return c

@@ -634,6 +634,16 @@ double gp_composition(const gp_params *p, const double *phi_cell, const double *
   return c;
 }
 
+void gp_composition_grid(const gp_params *p, const double *phi, const double *mu, double *c, int64_t ncells) {
+  const int N = p->nphases, NC = p->ncomp - 1;
+  for (int64_t i = 0; i < ncells; i++) {
+    double pc[GP_MAXP], mc[GP_MAXNC];
+    for (int a = 0; a < N; a++) pc[a] = phi[(int64_t)a * ncells + i];
+    for (int k = 0; k < NC; k++) mc[k] = mu[(int64_t)k * ncells + i];
+    for (int k = 0; k < NC; k++) c[(int64_t)k * ncells + i] = gp_composition(p, pc, mc, k);
+  }
+}
+
 /* ------------------------------------------------------------------------------------------------------------
  * fill_domain (GPs/functions/fill_domain.h, GPs/functions/filling.h)
  * ---------------------------------------------------------------------------------------------------------- */

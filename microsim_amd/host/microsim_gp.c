@@ -80,8 +80,8 @@ int main(int argc, char *argv[]) {
     if (ns < next) next = ns;
     if (nt < next) next = nt;
     const int report = (next % track == 0);
-    gp_stats st;
-    int rc = gp_step(h, next - t, report ? &st : NULL);
+    gp_stats st; /* every step accumulates the cumulative max/min, as the reference loop does */
+    int rc = gp_step(h, next - t, &st);
     if (rc) die("gp_step", gp_last_error(h));
     t = next;
     if (t % saveT == 0) {

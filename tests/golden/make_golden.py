@@ -98,7 +98,7 @@ def cases():
     inp = _bin_base(32, 36, bc_phi=(1, 1, 1, 1), fn_anisotropy=0, writeformat="ASCII")
     out["bin_iso_cube_ascii"] = (inp, [C.Fill("cube", 0, (0, 0, 31, 8))], None, [0, 1])
 
-    inp, fills, _ = C.config_ternary(96, 4, seed=5, steps=200, savet=1, min_spacing=30.0)
+    inp, fills, _ = C.config_ternary(96, 4, seed=5, steps=200, savet=1, min_spacing=30.0, gamma=1.0, driving=None)
     out["tern_c1_mini"] = (inp, fills, None, [0, 1, 10, 200])
     return out
 
