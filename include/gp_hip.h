@@ -57,7 +57,8 @@ extern "C" {
 
 /* precision modes */
 #define GP_FP64 0            /* FP64 storage and arithmetic, same operation order as the reference */
-#define GP_FP32 1            /* FP32 phi/mu storage and stencil arithmetic, compensated increments */
+#define GP_FP32 1            /* FP32 phi/mu storage and stencil arithmetic (bulk mu increments in direct form),
+                                FP64 statistics and set-up */
 
 /* closed-form solve of d(mu) = (dc/dmu)^-1 dc (GPs/solverloop/calculate_divergence_concentration.h:93-125) */
 #define GP_SOLVE_LU 0
@@ -179,7 +180,8 @@ const char *gp_version(void);
 #define GP_KERNEL_STATS 2    /* statistics fold */
 #define GP_KERNEL_FUSED 3    /* single-pass step kernel */
 #define GP_KERNEL_FUSED_STATS 4 /* single-pass step kernel with the statistics reduction */
-#define GP_NKERNELS 5
+#define GP_KERNEL_BULK 5     /* single-pass step kernel of the tiles inside one pure phase */
+#define GP_NKERNELS 6
 int gp_profile_enable(gp_handle *h, int enable);
 int gp_profile_read(gp_handle *h, double *avg_ms, int64_t *count);
 const char *gp_kernel_name(int kind);

@@ -6,9 +6,11 @@
  *   0..mesh_y+5 starting at element kYOff, so the first interior cell y = 3 sits on a 64-element boundary;
  *   the row pitch ps is a multiple of 64 elements. phi (N planes) and mu (NC planes) are ping-ponged,
  *   dphi (N planes) holds delta-phi when it is asked for.
- * A time step is one launch of k_march (gp_march.h): a single pass that reads phi and mu once, recomputes
- * delta-phi and the anti-trapping currents on the fly and writes the new phi and mu (and the halo images of
- * boundary cells), followed by the statistics fold k_stats.
+ * A time step is a single pass over the state in tiles (gp_march.h): k_march updates the tiles near an
+ * interface -- it reads phi and mu once, recomputes delta-phi and the anti-trapping currents on the fly and
+ * writes the new phi and mu (and the halo images of boundary cells) -- and k_bulk the tiles whose neighbourhood
+ * was purely one phase (from the tile flags both kernels write for the next step); the statistics fold k_stats
+ * follows when statistics are asked for.
  * The two-pass form (k_phi: delta-phi on x,y in [2, mesh+3]; k_mu: currents and update on the interior) runs
  * the smoothing steps, recomputes delta-phi for gp_download, and is selectable for steps with GP_PATH=twopass.
  * Reference routines replaced: see include/gp_hip.h.
@@ -52,12 +54,16 @@ struct gp_handle {
   void *model;              /* Model<R> in device memory */
   double *partials;         /* [statistic][nparts_max], statistic-major */
   double *stats_dev;        /* cumulative stats (kNStatMax) */
+  double *inv_dev;          /* solute[NC], energy of the current state */
   int cur;
   int64_t nblocks_mu;
   int64_t steps;
   int64_t nparts_max;       /* rows of the partials buffer */
-  int march_seg, march_nstrips;
-  int64_t march_nwaves;     /* 0 until the march plan is made */
+  int march_seg, march_nstrips, march_nsegs; /* tile grid of the single-pass step (see gp_march.h) */
+  int64_t march_nwaves;     /* tiles = wavefronts of k_march and of k_bulk */
+  int *flags[2];            /* per-tile purity flags of the last two steps */
+  int fcur;                 /* flags[fcur] describes the current state */
+  bool flags_valid;         /* flags[fcur] is valid for the current state (else no tile is taken as deep) */
   bool use_march;           /* single-pass steps (default) or the two-pass form */
   bool prev_valid;          /* the other buffer holds the state before the last step */
   Maps maps;
@@ -210,8 +216,8 @@ __global__ __launch_bounds__(kBX *kBY) void k_mu(const Model<R> *__restrict__ mp
 constexpr int kStatThreads = 1024;
 
 template <int N, int NC>
-__global__ __launch_bounds__(kStatThreads) void k_stats(const double *__restrict__ partials, int64_t nparts,
-                                                        double *__restrict__ cum) {
+__global__ __launch_bounds__(kStatThreads) void k_stats(const double *__restrict__ partials, int64_t stride,
+                                                        int64_t nparts, double *__restrict__ cum) {
   constexpr int NS = 3 * N + 3 * NC + 1;
   __shared__ double red[kStatThreads];
   const int i = blockIdx.x;
@@ -219,7 +225,7 @@ __global__ __launch_bounds__(kStatThreads) void k_stats(const double *__restrict
   const bool mx = (i < N || (i >= 2 * N && i < 2 * N + NC));
   const bool mn = !mx && i < 2 * N + 2 * NC;
   double r = mx ? -INFINITY : mn ? INFINITY : 0.0;
-  const double *row = partials + (int64_t)i * nparts;
+  const double *row = partials + (int64_t)i * stride;
   for (int64_t b = threadIdx.x; b < nparts; b += kStatThreads) {
     double v = row[b];
     r = mx ? fmax(r, v) : mn ? fmin(r, v) : r + v;
@@ -242,32 +248,123 @@ __global__ __launch_bounds__(kStatThreads) void k_stats(const double *__restrict
   }
 }
 
+/* Global solute and grand-potential energy of the current state (no reference counterpart; the FP64 oracle's
+ * oracle_invariants defines them): per interior cell
+ *   c_k    = sum_a h_a(phi) c_a^k(mu) dx dy,
+ *   energy = [ sum_a h_a psi_a(mu) + eps sum_{a<b} Gamma_ab ac(q_ab)^2 |q_ab|^2 + w(phi) / eps ] dx dy,
+ * with q_ab from the centred gradients. Cells are evaluated in the state's precision and accumulated in FP64:
+ * a block sum per quantity into partials[i * nblocks + block], folded by k_fold_sum. */
+template <int N, int NC, typename R>
+__global__ __launch_bounds__(kBX *kBY) void k_invariants(const Model<R> *__restrict__ mp, const R *__restrict__ phi,
+                                                         const R *__restrict__ mu, double *__restrict__ partials,
+                                                         int64_t fs, int64_t ps, int64_t mx, int64_t my) {
+  const int64_t y = 3 + (int64_t)blockIdx.x * kBX + threadIdx.x;
+  const int64_t x = 3 + (int64_t)blockIdx.y * kBY + threadIdx.y;
+  double v[NC + 1];
+  for (int i = 0; i <= NC; i++) v[i] = 0.0;
+  if (x <= mx + 2 && y <= my + 2) {
+    const Model<R> &m = *mp;
+    const int64_t c = x * ps + y + kYOff;
+    const double area = (double)m.dx * (double)m.dy;
+    R ph[N], mv[NC], gxc[N], gyc[N], h[N];
+    for (int a = 0; a < N; a++) {
+      ph[a] = phi[a * fs + c];
+      gxc[a] = (phi[a * fs + c + ps] - phi[a * fs + c - ps]) / (R(2.0) * m.dx);
+      gyc[a] = (phi[a * fs + c + 1] - phi[a * fs + c - 1]) / m.two_dy;
+    }
+    for (int k = 0; k < NC; k++) mv[k] = mu[k * fs + c];
+    for (int a = 0; a < N; a++) h[a] = hphi<N>(ph, a);
+    R e = R(0.0);
+    for (int a = 0; a < N; a++) {
+      R cc[NC], psi = R(0.0);
+      for (int k = 0; k < NC; k++) {
+        cc[k] = c_mu<NC>(m, mv, a, k);
+        v[k] += (double)(h[a] * cc[k]) * area;
+        psi -= mv[k] * cc[k];
+      }
+      psi += free_energy<NC>(m, cc, a);
+      e += h[a] * psi;
+    }
+    R grad = R(0.0), w = R(0.0);
+    for (int a = 0; a < N; a++)
+      for (int b = a + 1; b < N; b++) {
+        R r[2] = {ph[a] * gxc[b] - ph[b] * gxc[a], ph[a] * gyc[b] - ph[b] * gyc[a]}, q[2];
+        rot2(m.rot[a][b], r, q);
+        const R ac = m.fn_aniso ? aniso_ac(m.dab[a][b], q) : R(1.0);
+        grad += m.gamma[a][b] * ac * ac * qnorm2(q);
+        if (m.fn_w == 1) w += m.w_obst * m.gamma[a][b] * ph[a] * ph[b];
+        else w += R(9.0) * m.gamma[a][b] * ph[a] * ph[a] * ph[b] * ph[b];
+        for (int cc = b + 1; cc < N; cc++) w += m.gamma_abc[a][b][cc] * ph[a] * ph[b] * ph[cc];
+      }
+    e += m.eps * grad + w / m.eps;
+    v[NC] = (double)e * area;
+  }
+  __shared__ double red[NC + 1][kBY];
+  const int lane = threadIdx.x & 63;
+  for (int i = 0; i <= NC; i++) {
+    const double r = wave_sum(v[i]);
+    if (lane == 0) red[i][threadIdx.y] = r;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    const int64_t nb = (int64_t)gridDim.x * gridDim.y, bid = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+    for (int i = 0; i <= NC; i++) {
+      double t = 0.0;
+      for (int j = 0; j < kBY; j++) t += red[i][j];
+      partials[i * nb + bid] = t;
+    }
+  }
+}
+
+/* out[i] = sum_b partials[i * count + b]: one block per quantity, fixed order (deterministic) */
+__global__ __launch_bounds__(kStatThreads) void k_fold_sum(const double *__restrict__ partials, int64_t count,
+                                                          double *__restrict__ out) {
+  __shared__ double red[kStatThreads];
+  const double *row = partials + (int64_t)blockIdx.x * count;
+  double r = 0.0;
+  for (int64_t b = threadIdx.x; b < count; b += kStatThreads) r += row[b];
+  red[threadIdx.x] = r;
+  __syncthreads();
+  for (int st = kStatThreads / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+/* solute[NC] and energy of the current state into h->inv_dev */
+template <int N, int NC, typename R>
+static int launch_invariants(gp_handle *h) {
+  const int64_t mx = h->p.mesh_x, my = h->p.mesh_y;
+  dim3 blk(kBX, kBY);
+  dim3 g2((unsigned)((my + kBX - 1) / kBX), (unsigned)((mx + kBY - 1) / kBY));
+  hipLaunchKernelGGL((k_invariants<N, NC, R>), g2, blk, 0, h->stream, (const Model<R> *)h->model,
+                     (const R *)h->phi[h->cur], (const R *)h->mu[h->cur], h->partials, h->fs, h->ps, mx, my);
+  hipLaunchKernelGGL(k_fold_sum, dim3(NC + 1), dim3(kStatThreads), 0, h->stream, h->partials, h->nblocks_mu,
+                     h->inv_dev);
+  HIPCHK(h, hipGetLastError());
+  return GP_OK;
+}
+
 /* ------------------------------------------------------------------------------------------------ dispatch */
 
 enum { MODE_SMOOTH = 0, MODE_TWOPASS = 1, MODE_MARCH = 2, MODE_DPHI_PREV = 3, MODE_MARCH_STATS = 4 };
 
-/* Segment length of the march: enough wavefronts for about two rounds of the resident capacity, at least
- * 16 rows per segment (each segment recomputes two rows of delta-phi). */
-template <int N, int NC, typename R, bool P2>
+/* Tile grid of the single-pass step: 60-cell y-strips by x-segments of 32 rows (16 when that gives fewer than
+ * 4096 tiles, to fill the device on small grids); GP_SEG_LEN overrides the segment length. Each segment
+ * recomputes two warm-up rows of delta-phi. */
 static void march_plan(gp_handle *h) {
   const int64_t mx = h->p.mesh_x, my = h->p.mesh_y;
-  int ncu = 0, nb = 0;
-  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device) != hipSuccess) ncu = 256;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_march<N, NC, R, P2, false>, 256, 0) != hipSuccess || nb < 1)
-    nb = 1;
-  if (ncu < 1) ncu = 1;
   const int64_t nstrips = (my + kMarchOut - 1) / kMarchOut;
-  const int64_t target = 2 * (int64_t)nb * 4 * ncu;
-  int64_t nseg = (target + nstrips - 1) / nstrips;
-  int64_t seg = (mx + nseg - 1) / nseg;
-  if (seg < 16) seg = 16;
+  int64_t seg = 32;
+  if (nstrips * ((mx + seg - 1) / seg) < 4096) seg = 16;
   const char *e = getenv("GP_SEG_LEN");
   if (e && atoi(e) > 0) seg = atoi(e);
   if (seg > mx) seg = mx;
-  nseg = (mx + seg - 1) / seg;
   h->march_seg = (int)seg;
   h->march_nstrips = (int)nstrips;
-  h->march_nwaves = nstrips * nseg;
+  h->march_nsegs = (int)((mx + seg - 1) / seg);
+  h->march_nwaves = nstrips * h->march_nsegs;
 }
 
 template <int N, int NC, typename R, bool P2>
@@ -289,28 +386,50 @@ static int launch_step_p(gp_handle *h, int mode) {
     return GP_OK;
   }
   if (mode == MODE_MARCH || mode == MODE_MARCH_STATS) {
-    if (h->march_nwaves == 0) march_plan<N, NC, R, P2>(h);
-    const unsigned nblk = (unsigned)((h->march_nwaves + 3) / 4);
+    /* k_march updates the tiles near an interface, k_bulk the deep tiles (none when the flags are not valid);
+     * both read the current state, write disjoint tiles of the other buffer and the tile flags of the new state */
+    const int nt = (int)h->march_nwaves;
+    const unsigned nblk = (unsigned)((nt + 3) / 4);
+    const int *fin = h->flags_valid ? h->flags[h->fcur] : nullptr;
+    int *fout = h->flags[1 - h->fcur];
+    const int64_t pstride = 2 * (int64_t)nt;
     if (mode == MODE_MARCH) {
-      ProfScope ps(h, GP_KERNEL_FUSED);
-      hipLaunchKernelGGL((k_march<N, NC, R, P2, false>), dim3(nblk), dim3(256), 0, h->stream, m, phi, mu, phi_o,
-                         mu_o, h->maps, h->partials, h->fs, h->ps, mx, my, h->march_seg, h->march_nstrips,
-                         (int)h->march_nwaves);
+      {
+        ProfScope ps(h, GP_KERNEL_FUSED);
+        hipLaunchKernelGGL((k_march<N, NC, R, P2, false>), dim3(nblk), dim3(256), 0, h->stream, m, phi, mu, phi_o,
+                           mu_o, h->maps, h->partials, pstride, fin, fout, h->fs, h->ps, mx, my, h->march_seg,
+                           h->march_nstrips, h->march_nsegs, nt);
+      }
+      if (fin) {
+        ProfScope ps(h, GP_KERNEL_BULK);
+        hipLaunchKernelGGL((k_bulk<N, NC, R, P2, false>), dim3(nblk), dim3(256), 0, h->stream, m, mu, phi_o, mu_o,
+                           h->partials + nt, pstride, fin, fout, h->fs, h->ps, mx, h->march_seg, h->march_nstrips,
+                           h->march_nsegs, nt);
+      }
     } else {
       {
         ProfScope ps(h, GP_KERNEL_FUSED_STATS);
         hipLaunchKernelGGL((k_march<N, NC, R, P2, true>), dim3(nblk), dim3(256), 0, h->stream, m, phi, mu, phi_o,
-                           mu_o, h->maps, h->partials, h->fs, h->ps, mx, my, h->march_seg, h->march_nstrips,
-                           (int)h->march_nwaves);
+                           mu_o, h->maps, h->partials, pstride, fin, fout, h->fs, h->ps, mx, my, h->march_seg,
+                           h->march_nstrips, h->march_nsegs, nt);
+      }
+      if (fin) {
+        ProfScope ps(h, GP_KERNEL_BULK);
+        hipLaunchKernelGGL((k_bulk<N, NC, R, P2, true>), dim3(nblk), dim3(256), 0, h->stream, m, mu, phi_o, mu_o,
+                           h->partials + nt, pstride, fin, fout, h->fs, h->ps, mx, h->march_seg, h->march_nstrips,
+                           h->march_nsegs, nt);
       }
       ProfScope ps(h, GP_KERNEL_STATS);
       hipLaunchKernelGGL((k_stats<N, NC>), dim3(3 * N + 3 * NC + 1), dim3(kStatThreads), 0, h->stream,
-                         h->partials, (int64_t)h->march_nwaves, h->stats_dev);
+                         h->partials, pstride, fin ? pstride : (int64_t)nt, h->stats_dev);
     }
     HIPCHK(h, hipGetLastError());
     h->cur = 1 - h->cur;
+    h->fcur = 1 - h->fcur;
+    h->flags_valid = true;
     return GP_OK;
   }
+  h->flags_valid = false; /* the two-pass kernels do not form tile flags */
   if (smooth) {
     {
       ProfScope ps(h, GP_KERNEL_PHI);
@@ -334,7 +453,7 @@ static int launch_step_p(gp_handle *h, int mode) {
     {
       ProfScope ps(h, GP_KERNEL_STATS);
       hipLaunchKernelGGL((k_stats<N, NC>), dim3(3 * N + 3 * NC + 1), dim3(kStatThreads), 0, h->stream,
-                         h->partials, h->nblocks_mu, h->stats_dev);
+                         h->partials, h->nblocks_mu, h->nblocks_mu, h->stats_dev);
     }
   }
   HIPCHK(h, hipGetLastError());
@@ -348,6 +467,16 @@ static int launch_step(gp_handle *h, int mode) {
 }
 
 typedef int (*step_fn)(gp_handle *, int);
+typedef int (*inv_fn)(gp_handle *);
+
+template <typename R>
+static inv_fn pick_inv(int N, int NC) {
+#define GP_CASE(a, b) \
+  if (N == a && NC == b) return launch_invariants<a, b, R>;
+  GP_CASE(2, 1) GP_CASE(3, 2) GP_CASE(3, 1) GP_CASE(2, 2) GP_CASE(4, 2) GP_CASE(4, 1) GP_CASE(3, 3)
+#undef GP_CASE
+  return nullptr;
+}
 
 template <typename R>
 static step_fn pick(int N, int NC) {
@@ -492,15 +621,13 @@ int gp_create(const gp_params *params, int precision, gp_handle **out) {
     return fail(nullptr, GP_E_NODEV, "hipGetDevice failed");
   }
   h->nblocks_mu = ((params->mesh_y + kBX - 1) / kBX) * ((params->mesh_x + kBY - 1) / kBY);
-  {
-    /* the march uses at most one wavefront per (60-cell strip, 16-row segment) */
-    const int64_t nw = ((params->mesh_y + kMarchOut - 1) / kMarchOut) * ((params->mesh_x + 15) / 16);
-    h->nparts_max = std::max(h->nblocks_mu, nw);
-  }
+  march_plan(h);
+  h->nparts_max = std::max(h->nblocks_mu, 2 * h->march_nwaves); /* k_march and k_bulk partials */
   const char *path = getenv("GP_PATH");
   h->use_march = !(path && strcmp(path, "twopass") == 0);
-  h->march_nwaves = 0;
   h->prev_valid = false;
+  h->fcur = 0;
+  h->flags_valid = false;
   const size_t es = precision == GP_FP64 ? 8 : 4;
   const size_t pn = (size_t)h->N * h->fs * es, mn = (size_t)h->NC * h->fs * es;
   hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
@@ -510,6 +637,9 @@ int gp_create(const gp_params *params, int precision, gp_handle **out) {
   ok = ok && hipMalloc(&h->dphi, pn) == hipSuccess;
   ok = ok && hipMalloc((void **)&h->partials, sizeof(double) * kNStatMax * h->nparts_max) == hipSuccess;
   ok = ok && hipMalloc((void **)&h->stats_dev, sizeof(double) * kNStatMax) == hipSuccess;
+  ok = ok && hipMalloc((void **)&h->inv_dev, sizeof(double) * (GP_MAXNC + 1)) == hipSuccess;
+  ok = ok && hipMalloc((void **)&h->flags[0], sizeof(int) * h->march_nwaves) == hipSuccess;
+  ok = ok && hipMalloc((void **)&h->flags[1], sizeof(int) * h->march_nwaves) == hipSuccess;
   const size_t msize = precision == GP_FP64 ? sizeof(Model<double>) : sizeof(Model<float>);
   ok = ok && hipMalloc(&h->model, msize) == hipSuccess;
   if (!ok) {
@@ -555,7 +685,8 @@ void gp_destroy(gp_handle *h) {
     (void)hipEventDestroy(r.second.second);
   }
   h->prof.clear();
-  void *bufs[] = {h->phi[0], h->phi[1], h->mu[0], h->mu[1], h->dphi, h->partials, h->stats_dev, h->model};
+  void *bufs[] = {h->phi[0], h->phi[1], h->mu[0], h->mu[1], h->dphi, h->partials, h->stats_dev, h->model,
+                  h->flags[0], h->flags[1], h->inv_dev};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -578,6 +709,7 @@ int gp_upload(gp_handle *h, const double *phi, const double *mu) {
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->steps = 0;
   h->prev_valid = false;
+  h->flags_valid = false;
   return GP_OK;
 }
 
@@ -638,8 +770,12 @@ int gp_smooth(gp_handle *h, int64_t nsteps) {
 
 int gp_get_stats(gp_handle *h, gp_stats *st) {
   if (!h || !st) return h ? fail(h, GP_E_ARG, "null argument") : GP_E_ARG;
-  double s[kNStatMax];
+  double s[kNStatMax], inv[GP_MAXNC + 1];
+  inv_fn fi = h->precision == GP_FP64 ? pick_inv<double>(h->N, h->NC) : pick_inv<float>(h->N, h->NC);
+  int rc = fi(h);
+  if (rc) return rc;
   HIPCHK(h, hipMemcpyAsync(s, h->stats_dev, sizeof s, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipMemcpyAsync(inv, h->inv_dev, sizeof(double) * (h->NC + 1), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   memset(st, 0, sizeof *st);
   const int N = h->N, NC = h->NC;
@@ -654,6 +790,8 @@ int gp_get_stats(gp_handle *h, gp_stats *st) {
     st->rel_change_mu[k] = s[3 * N + 2 * NC + k];
   }
   st->nonfinite = (int64_t)s[3 * N + 3 * NC];
+  for (int k = 0; k < NC; k++) st->solute[k] = inv[k];
+  st->energy = inv[NC];
   st->step = h->steps;
   return GP_OK;
 }
@@ -682,7 +820,7 @@ int gp_profile_enable(gp_handle *h, int enable) {
 }
 
 const char *gp_kernel_name(int kind) {
-  static const char *names[GP_NKERNELS] = {"k_phi", "k_mu", "k_stats", "k_march", "k_march_stats"};
+  static const char *names[GP_NKERNELS] = {"k_phi", "k_mu", "k_stats", "k_march", "k_march_stats", "k_bulk"};
   return (kind >= 0 && kind < GP_NKERNELS) ? names[kind] : "?";
 }
 

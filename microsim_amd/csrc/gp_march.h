@@ -14,7 +14,13 @@
  * Every value keeps the operation order of the per-cell functions in gp_model.h (which follow the serial
  * reference), so the FP64 results are identical to those of gp_model.h and of the reference.
  * A wavefront whose rows are all bulk (every Laplacian of phi zero, phi >= 0) skips the phase-field work of the
- * row: delta-phi is exactly zero there. Lanes 2..61 produce output; lanes 0, 1, 62, 63 only feed neighbours.
+ * row: delta-phi is exactly zero there; one whose three rows are purely one phase takes the pure-phase step.
+ * Lanes 2..61 produce output; lanes 0, 1, 62, 63 only feed neighbours.
+ *
+ * The step is split over tiles (strip x segment, one wavefront each): k_march takes the tiles near an interface,
+ * k_bulk the "deep" tiles, whose own cells and whose neighbour tiles' cells were all purely one phase at the
+ * start of the step. k_bulk streams mu only, with few registers and a deep row prefetch, so that the bulk of
+ * the grid runs at memory speed while the register-heavy interface work stays in k_march.
  */
 #pragma once
 
@@ -337,13 +343,67 @@ __device__ __forceinline__ void dphi_cell(const Model<R> &m, const R *phi, const
   project_simplex<N>(phi, divphi, out);
 }
 
-/* STATS: accumulate the step statistics (one partial per wavefront); off for the steps that do not report */
+/* per-lane statistics: max/min of each phi and mu, sums of squared increments, non-finite count */
+template <int N, int NC, int NSA>
+__device__ __forceinline__ void stats_init(double (&st)[NSA]) {
+#pragma unroll
+  for (int a = 0; a < N; a++) {
+    st[a] = -INFINITY;
+    st[N + a] = INFINITY;
+    st[2 * N + 2 * NC + a] = 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < NC; k++) {
+    st[2 * N + k] = -INFINITY;
+    st[2 * N + NC + k] = INFINITY;
+    st[3 * N + 2 * NC + k] = 0.0;
+  }
+  st[NSA - 1] = 0.0;
+}
+
+/* wavefront reduction of the per-lane statistics into partials[i * pstride + w] (statistic-major) */
+template <int N, int NC, int NSA>
+__device__ __forceinline__ void stats_partials(double (&st)[NSA], double *__restrict__ partials, int64_t pstride,
+                                               int w, int nwaves, int lane) {
+#pragma unroll
+  for (int i = 0; i < NSA; i++) {
+    double r;
+    if (i < N || (i >= 2 * N && i < 2 * N + NC)) r = wave_max(st[i]);
+    else if (i < 2 * N + 2 * NC) r = wave_min(st[i]);
+    else r = wave_sum(st[i]);
+    if (lane == 0 && w < nwaves) partials[(int64_t)i * pstride + w] = r;
+  }
+}
+
+/* Tiles: wavefront w of a step kernel handles tile w = (strip w % nstrips, segment w / nstrips), i.e. the 60
+ * output cells of a y-strip over the seg_len rows of an x-segment. After each step, flags[tile] holds b when
+ * every output cell of the tile is purely phase b (phi_b == 1, every other phi == 0), else -1.
+ * deep_phase: b when the tile and its eight neighbours all held purely phase b at the start of the step, else
+ * -1. Every cell the tile's update reads (up to three cells outside it) is then purely phase b, delta-phi and the
+ * anti-trapping currents vanish in the whole stencil, and the bulk kernel below updates the tile. Tiles on the
+ * edge of the tile grid (which write halo images) are never deep. */
+__device__ __forceinline__ int deep_phase(const int *__restrict__ flags, int strip, int seg, int nstrips, int nsegs) {
+  if (!flags || strip == 0 || seg == 0 || strip >= nstrips - 1 || seg >= nsegs - 1) return -1;
+  const int b = flags[seg * nstrips + strip];
+  if (b < 0) return -1;
+  bool all = true;
+#pragma unroll
+  for (int dx = -1; dx <= 1; dx++)
+#pragma unroll
+    for (int ds = -1; ds <= 1; ds++) all = all && flags[(seg + dx) * nstrips + strip + ds] == b;
+  return all ? b : -1;
+}
+
+/* STATS: accumulate the step statistics (one partial per wavefront at partials[i * pstride + w]); off for the
+ * steps that do not report. flags_in (may be NULL: no tile is deep) and flags_out: tile flags of the previous and
+ * of this step; deep tiles are left to k_bulk. */
 template <int N, int NC, typename R, bool P2, bool STATS>
 __global__ __launch_bounds__(256) void k_march(const Model<R> *__restrict__ mp, const R *__restrict__ phi,
                                                const R *__restrict__ mu, R *__restrict__ phi_o,
                                                R *__restrict__ mu_o, Maps maps, double *__restrict__ partials,
-                                               int64_t fs, int64_t ps, int64_t mx, int64_t my, int seg_len,
-                                               int nstrips, int nwaves) {
+                                               int64_t pstride, const int *__restrict__ flags_in,
+                                               int *__restrict__ flags_out, int64_t fs, int64_t ps, int64_t mx,
+                                               int64_t my, int seg_len, int nstrips, int nsegs, int nwaves) {
   using Vw = View<R, P2>;
   constexpr int NS = 3 * N + 3 * NC + 1;
   const Model<R> &m = *mp;
@@ -353,24 +413,10 @@ __global__ __launch_bounds__(256) void k_march(const Model<R> *__restrict__ mp, 
 
   constexpr int NSA = STATS ? NS : 1;
   double st[NSA];
-  if (STATS) {
-#pragma unroll
-    for (int a = 0; a < N; a++) {
-      st[a] = -INFINITY;
-      st[N + a] = INFINITY;
-      st[2 * N + 2 * NC + a] = 0.0;
-    }
-#pragma unroll
-    for (int k = 0; k < NC; k++) {
-      st[2 * N + k] = -INFINITY;
-      st[2 * N + NC + k] = INFINITY;
-      st[3 * N + 2 * NC + k] = 0.0;
-    }
-    st[NSA - 1] = 0.0;
-  }
+  if (STATS) stats_init<N, NC>(st);
 
-  if (w < nwaves) {
-    const int strip = w % nstrips, seg = w / nstrips;
+  const int strip = w % nstrips, seg = w / nstrips;
+  if (w < nwaves && deep_phase(flags_in, strip, seg, nstrips, nsegs) < 0) {
     const int64_t y0 = 3 + (int64_t)strip * kMarchOut;
     const int64_t y = y0 - 2 + lane;
     const int64_t yl = y < ry - 1 ? y : ry - 1;
@@ -399,10 +445,13 @@ __global__ __launch_bounds__(256) void k_march(const Model<R> *__restrict__ mp, 
     bool cc_valid = false; /* cc_prev holds c_mu of row x-1 (the pure-phase steps do not form it) */
     FaceAn<N, R> fb;
     bool have_back = false;
+    int oc = -2; /* purity of this lane's output cells: -2 none yet, b all purely phase b, -1 otherwise */
 
-    /* writes the updated row x-1 (and its halo images) and accumulates the statistics */
+    /* writes the updated row x-1 (and its halo images) and accumulates the statistics and the purity */
     auto emit = [&](int64_t xo, const R *pn, const R *munew, const R *dph_o, const R *deltamu) {
       if (!out_lane) return;
+      const int pc = pure_phase<N>(pn);
+      oc = (oc == -2 || oc == pc) ? pc : -1;
       const int64_t c = xo * ps + y + kYOff;
 #pragma unroll
       for (int a = 0; a < N; a++) phi_o[a * fs + c] = pn[a];
@@ -795,18 +844,171 @@ __global__ __launch_bounds__(256) void k_march(const Model<R> *__restrict__ mp, 
       A0 = A1;
       A1 = A2;
     }
+    /* tile flag: lane 2 is an output lane of every tile */
+    const int ref = __shfl(oc, 2, 64);
+    const bool same = __all(!out_lane || oc == ref);
+    if (lane == 0) flags_out[w] = (same && ref >= 0) ? ref : -1;
   }
 
-  /* statistics: wavefront reduction, one partial per wavefront (statistic-major) */
+  /* statistics: wavefront reduction, one partial per wavefront (statistic-major); a deep tile's wavefront
+   * writes the neutral values */
   if (!STATS) return;
+  stats_partials<N, NC>(st, partials, pstride, w, nwaves, lane);
+}
+
+
+
+/* Bulk step of the deep tiles (deep_phase >= 0): every cell the update of such a tile reads is purely phase b, so
+ * delta-phi and the anti-trapping currents vanish, phi is unchanged and mu follows the bulk update of phase b with
+ * the mobility Mpure[b]. The values, operations and order are those of the pure-phase step of k_march (and so of
+ * the general step), so the results are bit-identical. The kernel reads mu only and writes phi = e_b (phi + 0 of
+ * the general step), mu and the tile flag. Its few registers keep many wavefronts resident, and the mu rows are
+ * prefetched kBulkDepth march steps ahead: the work is pure streaming, bound by HBM latency and bandwidth. */
+constexpr int kBulkDepth = 4;
+
+template <int N, int NC, typename R, bool P2, bool STATS>
+__global__ __launch_bounds__(256) void k_bulk(const Model<R> *__restrict__ mp, const R *__restrict__ mu,
+                                              R *__restrict__ phi_o, R *__restrict__ mu_o,
+                                              double *__restrict__ partials, int64_t pstride,
+                                              const int *__restrict__ flags_in, int *__restrict__ flags_out,
+                                              int64_t fs, int64_t ps, int64_t mx, int seg_len, int nstrips, int nsegs,
+                                              int nwaves) {
+  using Vw = View<R, P2>;
+  constexpr int NS = 3 * N + 3 * NC + 1;
+  constexpr int NSA = STATS ? NS : 1;
+  const Model<R> &m = *mp;
+  const int lane = threadIdx.x & 63;
+  const int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  double st[NSA];
+  if (STATS) stats_init<N, NC>(st);
+  const int strip = w % nstrips, seg = w / nstrips;
+  const int b = w < nwaves ? deep_phase(flags_in, strip, seg, nstrips, nsegs) : -1;
+  if (b >= 0) {
+    /* a deep tile is not on the edge of the tile grid: all its lanes are inside the grid, no halo images */
+    const int64_t y = 3 + (int64_t)strip * kMarchOut - 2 + lane;
+    const int64_t xa = 3 + (int64_t)seg * seg_len;
+    const int64_t xb = (xa + seg_len - 1 < mx + 2) ? xa + seg_len - 1 : mx + 2;
+    const bool out_lane = lane >= 2 && lane <= 61;
+    R Mb[NC][NC];
 #pragma unroll
-  for (int i = 0; i < NSA; i++) {
-    double r;
-    if (i < N || (i >= 2 * N && i < 2 * N + NC)) r = wave_max(st[i]);
-    else if (i < 2 * N + 2 * NC) r = wave_min(st[i]);
-    else r = wave_sum(st[i]);
-    if (lane == 0 && w < nwaves) partials[(int64_t)i * nwaves + w] = r;
+    for (int k = 0; k < NC; k++)
+#pragma unroll
+      for (int l = 0; l < NC; l++) Mb[k][l] = m.Mpure[b][k][l];
+    const R zj = R(0.0) * m.jat_fac; /* (j - j_prev) * jat_fac with both currents +0 */
+    R divjat0 = divx<Vw>(m, zj);
+    divjat0 += divy<Vw>(m, zj);
+    R pe[N];
+#pragma unroll
+    for (int a = 0; a < N; a++) pe[a] = a == b ? R(1.0) : R(0.0);
+
+    R m0[NC], m1[NC], q[kBulkDepth][NC];
+    const int64_t c0 = y + kYOff;
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+      m0[k] = mu[k * fs + (xa - 1) * ps + c0];
+      m1[k] = mu[k * fs + xa * ps + c0];
+    }
+#pragma unroll
+    for (int d = 0; d < kBulkDepth; d++)
+      if (xa + 1 + d <= xb + 1) {
+#pragma unroll
+        for (int k = 0; k < NC; k++) q[d][k] = mu[k * fs + (xa + 1 + d) * ps + c0];
+      }
+    R Fb[NC][NC];
+#pragma unroll
+    for (int k = 0; k < NC; k++)
+#pragma unroll
+      for (int l = 0; l < NC; l++) Fb[k][l] = Mb[k][l] * divx<Vw>(m, (m1[l] - m0[l]));
+
+    for (int64_t x = xa; x <= xb; x++) {
+      R m2[NC];
+#pragma unroll
+      for (int k = 0; k < NC; k++) m2[k] = q[0][k];
+#pragma unroll
+      for (int d = 0; d + 1 < kBulkDepth; d++)
+#pragma unroll
+        for (int k = 0; k < NC; k++) q[d][k] = q[d + 1][k];
+      if (x + 1 + kBulkDepth <= xb + 1) {
+#pragma unroll
+        for (int k = 0; k < NC; k++) q[kBulkDepth - 1][k] = mu[k * fs + (x + 1 + kBulkDepth) * ps + c0];
+      }
+      R Ff[NC][NC], Fy[NC][NC], Fyl[NC][NC], mud[NC], muu[NC];
+#pragma unroll
+      for (int k = 0; k < NC; k++) {
+        mud[k] = lane_dn(m1[k]);
+        muu[k] = lane_up(m1[k]);
+      }
+#pragma unroll
+      for (int k = 0; k < NC; k++)
+#pragma unroll
+        for (int l = 0; l < NC; l++) {
+          Ff[k][l] = Mb[k][l] * divx<Vw>(m, (m2[l] - m1[l]));
+          Fy[k][l] = Mb[k][l] * divy<Vw>(m, (mud[l] - m1[l]));
+          Fyl[k][l] = Mb[k][l] * divy<Vw>(m, (m1[l] - muu[l])); /* = Fy of the lane below, bit for bit */
+        }
+      R divflux[NC], deltamu[NC], munew[NC];
+#pragma unroll
+      for (int k = 0; k < NC; k++) {
+        divflux[k] = R(0.0);
+#pragma unroll
+        for (int l = 0; l < NC; l++) {
+          divflux[k] += divx<Vw>(m, (Ff[k][l] - Fb[k][l]));
+          divflux[k] += divy<Vw>(m, (Fy[k][l] - Fyl[k][l]));
+        }
+      }
+      if (kDirectBulkMu<R>) {
+        R dc[NC];
+#pragma unroll
+        for (int k = 0; k < NC; k++) dc[k] = m.dt * (divflux[k]) - divjat0;
+#pragma unroll
+        for (int k = 0; k < NC; k++) {
+          deltamu[k] = Mu_lin<NC>(m, dc, b, k);
+          munew[k] = m1[k] + deltamu[k];
+        }
+      } else {
+        R c_new[NC];
+#pragma unroll
+        for (int k = 0; k < NC; k++) c_new[k] = c_mu<NC>(m, m1, b, k) + m.dt * (divflux[k]) - divjat0;
+#pragma unroll
+        for (int k = 0; k < NC; k++) {
+          munew[k] = Mu<NC>(m, c_new, b, k);
+          deltamu[k] = munew[k] - m1[k];
+        }
+      }
+      if (out_lane) {
+        const int64_t c = x * ps + c0;
+#pragma unroll
+        for (int a = 0; a < N; a++) phi_o[a * fs + c] = pe[a];
+#pragma unroll
+        for (int k = 0; k < NC; k++) mu_o[k * fs + c] = munew[k];
+        if (STATS) {
+#pragma unroll
+          for (int a = 0; a < N; a++) {
+            st[a] = fmax(st[a], (double)pe[a]);
+            st[N + a] = fmin(st[N + a], (double)pe[a]);
+          }
+#pragma unroll
+          for (int k = 0; k < NC; k++) {
+            const double mv = (double)munew[k], dv = (double)deltamu[k];
+            st[2 * N + k] = fmax(st[2 * N + k], mv);
+            st[2 * N + NC + k] = fmin(st[2 * N + NC + k], mv);
+            st[3 * N + 2 * NC + k] += dv * dv;
+            if (!isfinite(mv)) st[NSA - 1] += 1.0;
+          }
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < NC; k++) {
+        m0[k] = m1[k];
+        m1[k] = m2[k];
+#pragma unroll
+        for (int l = 0; l < NC; l++) Fb[k][l] = Ff[k][l];
+      }
+    }
+    if (lane == 0) flags_out[w] = b; /* phi is unchanged: the tile stays purely phase b */
   }
+  if (!STATS) return;
+  stats_partials<N, NC>(st, partials, pstride, w, nwaves, lane);
 }
 
 }  // namespace gpk

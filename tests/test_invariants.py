@@ -1,0 +1,101 @@
+"""Global solute and grand-potential energy (no counterpart in the reference; defined by oracle_invariants).
+
+The oracle's definition is pinned by closed forms on states where every term is known:
+  - a uniform pure phase b: solute_k = cells * area * c_b^k(mu), energy = cells * area * psi_b(mu),
+    psi_b = f_b(c_b) - mu . c_b, no gradient or potential term;
+  - a two-phase state whose phase fields are linear in x (phi_a = s x + t, phi_b = 1 - phi_a): then
+    q_ab = phi_a grad phi_b - phi_b grad phi_a = -s x_hat exactly, |q|^2 = s^2, the 4-fold a_c of a q along
+    an axis is 1 + dab, and the obstacle term is (16/pi^2) Gamma phi_a phi_b;
+and the device kernel (libgp_hip, gp_get_stats) is checked against the oracle on evolved states.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from gp_testlib import load_golden, oracle, oracle_step
+from microsim_amd import _abi
+
+
+def _inv(params, phi, mu):
+    nc = params.ncomp - 1
+    sol = (C.c_double * 4)()
+    en = C.c_double()
+    oracle().oracle_invariants(C.byref(params), _abi.dptr(phi), _abi.dptr(mu), sol, C.byref(en))
+    return np.array(sol[:nc]), en.value
+
+
+def _c_and_psi(params, mu, b):
+    """c_b(mu) and psi_b(mu) from the parameters in numpy (parabolic free energy)."""
+    N, NC = params.nphases, params.ncomp - 1
+    cmu = np.array(params.cmu).reshape(6, 4, 4)[b, :NC, :NC]
+    Bmu = np.array([params.Beq[b * 4 + k] + params.dBbdT[b * 4 + k] * (params.T - params.Teq) for k in range(NC)])
+    c = cmu @ (mu - Bmu)
+    A = np.array(params.A).reshape(6, 4, 4)[b, :NC, :NC]
+    f = sum(A[i, j] * c[i] * c[j] for i in range(NC) for j in range(NC) if i <= j)
+    f += sum(params.B[b * 4 + i] * c[i] for i in range(NC)) + params.C[b]
+    return c, f - mu @ c
+
+
+def test_invariants_uniform_phase_closed_form():
+    case, _ = load_golden("tern_c1_mini")
+    p = case.params
+    N, NC = case.N, case.NC
+    mu_val = np.array([0.31, -0.12])
+    for b in range(N):
+        phi = np.zeros_like(case.phi)
+        phi[b] = 1.0
+        mu = np.empty_like(case.mu)
+        for k in range(NC):
+            mu[k] = mu_val[k]
+        sol, en = _inv(p, phi, mu)
+        cells = int(p.mesh_x) * int(p.mesh_y)
+        c, psi = _c_and_psi(p, mu_val, b)
+        area = p.dx * p.dy
+        np.testing.assert_allclose(sol, cells * area * c, rtol=1e-12)
+        assert en == pytest.approx(cells * area * psi, rel=1e-12)
+
+
+def test_invariants_linear_profile_closed_form():
+    case, _ = load_golden("bin_aniso_neu")
+    p = case.params
+    assert p.nphases == 2 and p.fn_aniso == 1 and p.fn_w == 1
+    rx, ry = case.phi.shape[1:]
+    s, t = 0.01, 0.2
+    x = np.arange(rx, dtype=np.float64)[:, None] * np.ones((1, ry))
+    phi = np.stack([s * x + t, 1.0 - (s * x + t)])
+    mu = np.zeros_like(case.mu)
+    sol, en = _inv(p, phi, mu)
+    area = p.dx * p.dy
+    inner = (slice(3, -3), slice(3, -3))
+    pa, pb = phi[0][inner], phi[1][inner]
+    # bulk part: h-weighted psi of both phases at mu = 0
+    h = lambda a, o: 3 * a * a - 2 * a ** 3  # two phases: the triple term is absent
+    _, psi0 = _c_and_psi(p, np.zeros(1), 0)
+    _, psi1 = _c_and_psi(p, np.zeros(1), 1)
+    bulk = (h(pa, pb) * psi0 + h(pb, pa) * psi1).sum()
+    g = p.gamma[0 * 6 + 1]
+    dab = p.dab[0 * 6 + 1]
+    slope = s / p.dx
+    grad = pa.size * g * (1.0 + dab) ** 2 * slope ** 2
+    w = (16.0 / math.pi ** 2) * g * (pa * pb).sum()
+    want = (bulk + p.epsilon * grad + w / p.epsilon) * area
+    assert en == pytest.approx(want, rel=1e-10)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision,rtol", [(_abi.GP_FP64, 1e-12), (_abi.GP_FP32, 2e-5)])
+@pytest.mark.parametrize("name", ["tern_c1_mini", "bin_aniso_neu", "quad_k3"])
+def test_device_invariants_match_oracle(name, precision, rtol):
+    from microsim_amd.solver import GPSolver
+    print("[gpu] invariants", name, precision, flush=True)
+    case, _ = load_golden(name)
+    with GPSolver(case.params, precision) as s:
+        s.upload(case.phi, case.mu)
+        got = s.step(17, stats=True)
+    phi, mu = case.phi.copy(), case.mu.copy()
+    oracle_step(case.params, phi, mu, 17)
+    sol, en = _inv(case.params, phi, mu)
+    np.testing.assert_allclose(got["solute"], sol, rtol=rtol)
+    assert got["energy"] == pytest.approx(en, rel=rtol, abs=rtol * np.abs(sol).max())
