@@ -94,12 +94,13 @@ struct FaceAn {
 /* x-face between rows r and r+1: sx, gx of the face, cy of both rows (q_divx, function_A_01.h:135-206) */
 template <int N, typename R>
 __device__ __forceinline__ void face_x(const Model<R> &m, const R *sxv, const R *gxv, const R *cy0, const R *cy1,
-                                       FaceAn<N, R> &f) {
+                                       FaceAn<N, R> &f, unsigned need) {
 #pragma unroll
   for (int a = 0; a < N; a++)
 #pragma unroll
     for (int b = a + 1; b < N; b++) {
       const int p = Pairs<N>::idx(a, b);
+      if (!((need >> p) & 1u)) continue;
       const R r0 = (sxv[a] * gxv[b] - sxv[b] * gxv[a]);
       if (!m.fn_aniso) {
         f.U[p] = r0;
@@ -122,12 +123,13 @@ __device__ __forceinline__ void face_x(const Model<R> &m, const R *sxv, const R 
 /* y-face between lanes y and y+1 of one row: sy, gy of the face, cx at y and y+1 (q_divy) */
 template <int N, typename R>
 __device__ __forceinline__ void face_y(const Model<R> &m, const R *syv, const R *gyv, const R *cx0, const R *cx1,
-                                       FaceAn<N, R> &f) {
+                                       FaceAn<N, R> &f, unsigned need) {
 #pragma unroll
   for (int a = 0; a < N; a++)
 #pragma unroll
     for (int b = a + 1; b < N; b++) {
       const int p = Pairs<N>::idx(a, b);
+      if (!((need >> p) & 1u)) continue;
       const R r1 = (syv[a] * gyv[b] - syv[b] * gyv[a]);
       if (!m.fn_aniso) {
         f.U[p] = r1;
@@ -148,9 +150,10 @@ __device__ __forceinline__ void face_y(const Model<R> &m, const R *syv, const R 
 }
 
 template <int N, typename R>
-__device__ __forceinline__ void face_shuffle_up(const FaceAn<N, R> &src, FaceAn<N, R> &dst) {
+__device__ __forceinline__ void face_shuffle_up(const FaceAn<N, R> &src, FaceAn<N, R> &dst, unsigned need) {
 #pragma unroll
   for (int p = 0; p < Pairs<N>::n; p++) {
+    if (!((need >> p) & 1u)) continue;
     dst.U[p] = lane_up(src.U[p]);
     dst.V[p] = lane_up(src.V[p]);
   }
@@ -160,12 +163,13 @@ __device__ __forceinline__ void face_shuffle_up(const FaceAn<N, R> &src, FaceAn<
 template <int N, class V, typename R>
 __device__ __forceinline__ R div_dadgrad(const Model<R> &m, int a, const FaceAn<N, R> &ff, const FaceAn<N, R> &fb,
                                          const FaceAn<N, R> &fr, const FaceAn<N, R> &fl, const R *sxf, const R *sxb,
-                                         const R *syr, const R *syl) {
+                                         const R *syr, const R *syl, unsigned need) {
   R sum1 = R(0.0), sum2 = R(0.0);
 #pragma unroll
   for (int b = 0; b < N; b++) {
     if (b == a) continue;
     const int p = a < b ? Pairs<N>::idx(a, b) : Pairs<N>::idx(b, a);
+    if (!((need >> p) & 1u)) continue; /* partner absent: the pair adds a signed zero */
     const bool s = a < b;
     const R Uf = s ? ff.U[p] : -ff.U[p], Ub = s ? fb.U[p] : -fb.U[p];
     const R Ur = s ? fr.U[p] : -fr.U[p], Ul = s ? fl.U[p] : -fl.U[p];
@@ -193,7 +197,8 @@ __device__ __forceinline__ R div_dadgrad(const Model<R> &m, int a, const FaceAn<
 
 /* d a / d phi_a at the cell centre with per-pair sharing (function_A_01_dAdphi) */
 template <int N, typename R>
-__device__ __forceinline__ void dadphi_all(const Model<R> &m, const R *phi, const R *cxc, const R *cyc, R *out) {
+__device__ __forceinline__ void dadphi_all(const Model<R> &m, const R *phi, const R *cxc, const R *cyc, R *out,
+                                           unsigned need) {
   if (!m.fn_aniso) {
 #pragma unroll
     for (int a = 0; a < N; a++) out[a] = dAdphi_A00<N>(m, phi, cxc, cyc, a);
@@ -206,6 +211,7 @@ __device__ __forceinline__ void dadphi_all(const Model<R> &m, const R *phi, cons
 #pragma unroll
     for (int b = a + 1; b < N; b++) {
       const int p = Pairs<N>::idx(a, b);
+      if (!((need >> p) & 1u)) continue;
       R r[2];
       r[0] = (phi[a] * cxc[b] - phi[b] * cxc[a]);
       r[1] = (phi[a] * cyc[b] - phi[b] * cyc[a]);
@@ -222,6 +228,7 @@ __device__ __forceinline__ void dadphi_all(const Model<R> &m, const R *phi, cons
       if (b == a) continue;
       const bool s = a < b;
       const int p = s ? Pairs<N>::idx(a, b) : Pairs<N>::idx(b, a);
+      if (!((need >> p) & 1u)) continue;
       R q[2], d[2], v[2], vr[2];
       q[0] = s ? qp[p][0] : -qp[p][0];
       q[1] = s ? qp[p][1] : -qp[p][1];
@@ -250,7 +257,7 @@ __device__ __forceinline__ void dadphi_all(const Model<R> &m, const R *phi, cons
 template <int N, int NC, typename R>
 __device__ __forceinline__ void jat_face(const Model<R> &m, const R *phc, const R *pho, const R (*ccc)[NC],
                                          const R (*cco)[NC], const R *dc, const R *dov, const R *gnorm,
-                                         const R *gtan, R *j) {
+                                         const R *gtan, R *j, unsigned phmask) {
   const int L = N - 1;
 #pragma unroll
   for (int k = 0; k < NC; k++) j[k] = R(0.0);
@@ -258,10 +265,11 @@ __device__ __forceinline__ void jat_face(const Model<R> &m, const R *phc, const 
 #pragma unroll
   for (int a = 0; a < N - 1; a++) any = any || (dc[a] != R(0.0)) || (dov[a] != R(0.0));
   if (!any) return;
-  const R nl = sqrt(gtan[L] * gtan[L] + gnorm[L] * gnorm[L]);
+  const R nl = qsqrt(gtan[L] * gtan[L] + gnorm[L] * gnorm[L]);
 #pragma unroll
   for (int a = 0; a < N - 1; a++) {
-    const R ng = sqrt(gtan[a] * gtan[a] + gnorm[a] * gnorm[a]);
+    if (!((phmask >> a) & 1u)) continue; /* phase absent: its gradient is zero on every lane */
+    const R ng = qsqrt(gtan[a] * gtan[a] + gnorm[a] * gnorm[a]);
     if (!(ng != R(0.0))) continue;
     R s_c = R(0), s_o = R(0);
     if (m.fn_w == 1) {
@@ -273,11 +281,11 @@ __device__ __forceinline__ void jat_face(const Model<R> &m, const R *phc, const 
       const R cl_o = cco[L][k], cl_c = ccc[L][k], cs_o = cco[a][k], cs_c = ccc[a][k];
       R sp = gnorm[a] * gnorm[L];
       sp += gtan[a] * gtan[L];
-      if (nl > R(0.0)) sp /= (nl * ng);
+      if (nl > R(0.0)) sp = qdiv(sp, (nl * ng));
       if (m.fn_w == 2)
-        j[k] += m.dratio[a][k] * fabs(sp) * m.jat_well * ((cl_o - cs_o) * dov[a] + (cl_c - cs_c) * dc[a]) * gnorm[a] / ng;
+        j[k] += qdiv(m.dratio[a][k] * fabs(sp) * m.jat_well * ((cl_o - cs_o) * dov[a] + (cl_c - cs_c) * dc[a]) * gnorm[a], ng);
       else
-        j[k] += m.dratio[a][k] * fabs(sp) * R(0.5) * (s_o * (cl_o - cs_o) * dov[a] + s_c * (cl_c - cs_c) * dc[a]) * gnorm[a] / ng;
+        j[k] += qdiv(m.dratio[a][k] * fabs(sp) * R(0.5) * (s_o * (cl_o - cs_o) * dov[a] + s_c * (cl_c - cs_c) * dc[a]) * gnorm[a], ng);
     }
   }
 }
@@ -310,7 +318,8 @@ template <int N, int NC, class V, typename R>
 __device__ __forceinline__ void dphi_cell(const Model<R> &m, const R *phi, const R *divphi, const R *psi,
                                           const R *cxc, const R *cyc, const FaceAn<N, R> &ff,
                                           const FaceAn<N, R> &fb, const FaceAn<N, R> &fr, const FaceAn<N, R> &fl,
-                                          const R *sxf, const R *sxb, const R *syr, const R *syl, R *out) {
+                                          const R *sxf, const R *sxb, const R *syr, const R *syl, R *out,
+                                          unsigned need) {
   bool any = false;
 #pragma unroll
   for (int a = 0; a < N; a++) any = any || (fabs(divphi[a]) > R(0.0));
@@ -318,14 +327,14 @@ __device__ __forceinline__ void dphi_cell(const Model<R> &m, const R *phi, const
   for (int a = 0; a < N; a++) out[a] = R(0.0);
   if (any) {
     R dA[N];
-    dadphi_all<N>(m, phi, cxc, cyc, dA);
+    dadphi_all<N>(m, phi, cxc, cyc, dA, need);
     R lambda[N], sum_lambda = R(0.0);
     int active = 0;
 #pragma unroll
     for (int a = 0; a < N; a++) {
       lambda[a] = R(0);
       if (fabs(divphi[a]) > R(0.0)) {
-        const R dD = div_dadgrad<N, V>(m, a, ff, fb, fr, fl, sxf, sxb, syr, syl);
+        const R dD = div_dadgrad<N, V>(m, a, ff, fb, fr, fl, sxf, sxb, syr, syl, need);
         R dp = R(0.0);
 #pragma unroll
         for (int b = 0; b < N; b++) dp += psi[b] * dhphi<N>(phi, b, a);
@@ -334,11 +343,11 @@ __device__ __forceinline__ void dphi_cell(const Model<R> &m, const R *phi, const
         active++;
       }
     }
-    if (active) sum_lambda /= R(active);
+    if (active) sum_lambda = qdiv(sum_lambda, R(active));
     const R tauv = tau_of<N>(m, phi);
 #pragma unroll
     for (int a = 0; a < N; a++)
-      if (fabs(divphi[a]) > R(0.0)) out[a] = m.dt * (lambda[a] - sum_lambda) / (tauv * m.eps);
+      if (fabs(divphi[a]) > R(0.0)) out[a] = qdiv(m.dt * (lambda[a] - sum_lambda), (tauv * m.eps));
   }
   project_simplex<N>(phi, divphi, out);
 }
@@ -444,7 +453,7 @@ __global__ __launch_bounds__(256) void k_march(const Model<R> *__restrict__ mp, 
     for (int k = 0; k < NC; k++) jx_prev[k] = jy_prev[k] = R(0);
     bool cc_valid = false; /* cc_prev holds c_mu of row x-1 (the pure-phase steps do not form it) */
     FaceAn<N, R> fb;
-    bool have_back = false;
+    unsigned back_mask = 0; /* pairs whose entries of fb hold the face between rows x-1 and x */
     int oc = -2; /* purity of this lane's output cells: -2 none yet, b all purely phase b, -1 otherwise */
 
     /* writes the updated row x-1 (and its halo images) and accumulates the statistics and the purity */
@@ -576,7 +585,7 @@ __global__ __launch_bounds__(256) void k_march(const Model<R> *__restrict__ mp, 
             }
           }
           cc_valid = false;
-          have_back = false;
+          back_mask = 0;
           A0 = A1;
           A1 = A2;
           continue;
@@ -584,6 +593,11 @@ __global__ __launch_bounds__(256) void k_march(const Model<R> *__restrict__ mp, 
       }
       if (!cc_valid) cmu_all<N, NC>(m, A0.mu, cc_prev);
       cc_valid = true;
+      /* phases present (nonzero on some lane) in rows x-1, x, x+1: wavefront-uniform */
+      unsigned phmask = 0;
+#pragma unroll
+      for (int a = 0; a < N; a++)
+        if (__any(A0.ph[a] != R(0.0) || A1.ph[a] != R(0.0) || A2.ph[a] != R(0.0))) phmask |= 1u << a;
 
       /* face gradients / averages and centred gradients (calculate_gradients_phasefield_2D) */
       R gx0[N], gx1[N], sx0[N], sx1[N], cy0[N], cy1[N], cy2[N], cxc[N], cxd[N], gy1[N], gyu[N], sy1[N], syu[N];
@@ -617,11 +631,20 @@ __global__ __launch_bounds__(256) void k_march(const Model<R> *__restrict__ mp, 
           sy1[a] = R(0.5) * (A1.phd[a] + A1.ph[a]);
           syu[a] = R(0.5) * (A1.ph[a] + A1.phu[a]);
         }
-        if (!have_back) face_x<N>(m, sx0, gx0, cy0, cy1, fb);
+        /* Phase pairs with both phases present in rows x-1..x+1 of the wavefront. For any other pair one
+         * phase is zero on the whole stencil: its q vanishes, and its terms add signed zeros to the sums of the
+         * active phases (which start at +0), so skipping them leaves every result unchanged. */
+        unsigned need = 0;
+#pragma unroll
+        for (int a = 0; a < N; a++)
+#pragma unroll
+          for (int b = a + 1; b < N; b++)
+            if (((phmask >> a) & 1u) && ((phmask >> b) & 1u)) need |= 1u << Pairs<N>::idx(a, b);
+        face_x<N>(m, sx0, gx0, cy0, cy1, fb, need & ~back_mask);
         FaceAn<N, R> ff, fr, fl;
-        face_x<N>(m, sx1, gx1, cy1, cy2, ff);
-        face_y<N>(m, sy1, gy1, cxc, cxd, fr);
-        face_shuffle_up<N>(fr, fl);
+        face_x<N>(m, sx1, gx1, cy1, cy2, ff, need);
+        face_y<N>(m, sy1, gy1, cxc, cxd, fr, need);
+        face_shuffle_up<N>(fr, fl, need);
         R psi[N];
 #pragma unroll
         for (int b = 0; b < N; b++) {
@@ -631,11 +654,11 @@ __global__ __launch_bounds__(256) void k_march(const Model<R> *__restrict__ mp, 
           p += free_energy<NC>(m, cc1[b], b);
           psi[b] = p;
         }
-        dphi_cell<N, NC, Vw>(m, A1.ph, divphi, psi, cxc, cy1, ff, fb, fr, fl, sx1, sx0, sy1, syu, dph);
+        dphi_cell<N, NC, Vw>(m, A1.ph, divphi, psi, cxc, cy1, ff, fb, fr, fl, sx1, sx0, sy1, syu, dph, need);
         fb = ff;
-        have_back = true;
+        back_mask = need;
       } else {
-        have_back = false;
+        back_mask = 0;
       }
 
       /* anti-trapping currents: right y-face of row x, x-face between x-1 and x */
@@ -666,14 +689,14 @@ __global__ __launch_bounds__(256) void k_march(const Model<R> *__restrict__ mp, 
           cmu_all<N, NC>(m, A1.mud, cco);
 #pragma unroll
           for (int a = 0; a < N; a++) gt[a] = R(0.5) * (cxA1[a] + cxA1d[a]);
-          jat_face<N, NC>(m, A1.ph, A1.phd, cc1, cco, dph, dph_dn, gyA1, gt, jy1);
+          jat_face<N, NC>(m, A1.ph, A1.phd, cc1, cco, dph, dph_dn, gyA1, gt, jy1, phmask);
         }
         /* x-face (x-1 .. x): normal gradient gx0, tangential average of cy over rows x-1, x */
         if (x >= xa) {
           R gt[N];
 #pragma unroll
           for (int a = 0; a < N; a++) gt[a] = R(0.5) * (cyA0[a] + cyA1[a]);
-          jat_face<N, NC>(m, A0.ph, A1.ph, cc_prev, cc1, dph_prev, dph, gx0, gt, jx0);
+          jat_face<N, NC>(m, A0.ph, A1.ph, cc_prev, cc1, dph_prev, dph, gx0, gt, jx0, phmask);
         }
       }
 
@@ -779,16 +802,16 @@ __global__ __launch_bounds__(256) void k_march(const Model<R> *__restrict__ mp, 
             }
           }
           if (m.solve == GP_SOLVE_BINARY && NC == 1) {
-            R inv = R(1.0) / dcdmu[0][0];
+            R inv = qdiv(R(1.0), dcdmu[0][0]);
             deltamu[0] = deltac[0] * inv;
             munew[0] += deltamu[0];
           } else if (m.solve == GP_SOLVE_TERNARY && NC == 2) {
             R DET = dcdmu[0][0] * dcdmu[1 % NC][1 % NC] - dcdmu[0][1 % NC] * dcdmu[1 % NC][0];
             R inv[NC][NC];
-            inv[0][0] = dcdmu[1 % NC][1 % NC] / DET;
-            inv[1 % NC][1 % NC] = dcdmu[0][0] / DET;
-            inv[0][1 % NC] = -dcdmu[0][1 % NC] / DET;
-            inv[1 % NC][0] = -dcdmu[1 % NC][0] / DET;
+            inv[0][0] = qdiv(dcdmu[1 % NC][1 % NC], DET);
+            inv[1 % NC][1 % NC] = qdiv(dcdmu[0][0], DET);
+            inv[0][1 % NC] = qdiv(-dcdmu[0][1 % NC], DET);
+            inv[1 % NC][0] = qdiv(-dcdmu[1 % NC][0], DET);
 #pragma unroll
             for (int k = 0; k < NC; k++) {
               deltamu[k] = R(0.0);
@@ -800,7 +823,7 @@ __global__ __launch_bounds__(256) void k_march(const Model<R> *__restrict__ mp, 
           } else if (m.solve == GP_SOLVE_DILUTE) {
 #pragma unroll
             for (int k = 0; k < NC; k++) {
-              R inv = R(1.0) / dcdmu[k][k];
+              R inv = qdiv(R(1.0), dcdmu[k][k]);
               deltamu[k] = deltac[k] * inv;
               munew[k] += deltamu[k];
             }

@@ -81,21 +81,41 @@ struct View {
   }
 };
 
+/* Quotient and square root of the step arithmetic. FP64: IEEE (correctly rounded), as the reference computes
+ * them, so the FP64 results are bit-identical to the reference. FP32, the production mode: the hardware
+ * reciprocal and square root (v_rcp_f32, v_sqrt_f32, about 1 ulp), two instructions where the correctly rounded
+ * FP32 sequences take about ten; the FP32 mode's error is set by its FP32 storage, far above 1 ulp per
+ * operation. Host builds (the CPU checks) use the IEEE forms. */
+template <typename R>
+GP_HD R qdiv(R a, R b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (sizeof(R) == sizeof(float)) return a * __builtin_amdgcn_rcpf(b);
+#endif
+  return a / b;
+}
+template <typename R>
+GP_HD R qsqrt(R a) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (sizeof(R) == sizeof(float)) return __builtin_amdgcn_sqrtf(a);
+#endif
+  return sqrt(a);
+}
+
 /* v/dx, v/dy, v/(2 dy) */
 template <class V, typename R>
 GP_HD R divx(const Model<R> &m, R v) {
   if constexpr (V::kPow2) return v * m.inv_dx;
-  else return v / m.dx;
+  else return qdiv(v, m.dx);
 }
 template <class V, typename R>
 GP_HD R divy(const Model<R> &m, R v) {
   if constexpr (V::kPow2) return v * m.inv_dy;
-  else return v / m.dy;
+  else return qdiv(v, m.dy);
 }
 template <class V, typename R>
 GP_HD R div2y(const Model<R> &m, R v) {
   if constexpr (V::kPow2) return v * m.inv_2dy;
-  else return v / m.two_dy;
+  else return qdiv(v, m.two_dy);
 }
 
 /* ------------------------------------------------------------------ face quantities at offset (i,j) */
@@ -210,7 +230,7 @@ GP_HD R tau_of(const Model<R> &m, const R *phi) {
         s += m.tau_ab[a][b] * phi[a] * phi[b];
         s1 += phi[a] * phi[b];
       }
-  if (s1 != R(0.0)) return s / s1;
+  if (s1 != R(0.0)) return qdiv(s, s1);
   return m.tau;
 }
 
@@ -230,7 +250,7 @@ GP_HD R aniso_ac(R dab, const R *q) {
   R qy2 = q[1] * q[1], qy4 = qy2 * qy2;
   R qz2 = R(0.0), qz4 = qz2 * qz2;
   R q2 = qx2 + qy2 + qz2;
-  if (fabs(q2) > R(1.0e-15)) return R(1.0) - dab * (R(3.0) - R(4.0) * (qx4 + qy4 + qz4) / (q2 * q2));
+  if (fabs(q2) > R(1.0e-15)) return R(1.0) - dab * (R(3.0) - qdiv(R(4.0) * (qx4 + qy4 + qz4), (q2 * q2)));
   return R(1.0);
 }
 
@@ -242,8 +262,8 @@ GP_HD void aniso_dadq(R dab, const R *q, R *d) {
   R q22 = q2 * q2, q23 = q2 * q2 * q2;
   R q4 = qx3 * q[0] + qy3 * q[1] + qz3 * qz;
   if (fabs(q2) > R(1.0e-15)) {
-    d[0] = R(16.0) * dab * (qx3 / q22 - q[0] * q4 / q23);
-    d[1] = R(16.0) * dab * (qy3 / q22 - q[1] * q4 / q23);
+    d[0] = R(16.0) * dab * (qdiv(qx3, q22) - qdiv(q[0] * q4, q23));
+    d[1] = R(16.0) * dab * (qdiv(qy3, q22) - qdiv(q[1] * q4, q23));
   } else {
     d[0] = R(0.0);
     d[1] = R(0.0);
@@ -420,8 +440,8 @@ GP_HD void project_simplex(const R *phi, const R *divphi, R *dphi) {
   if (Deltaphi > R(0.0))
     for (int b = 0; b < N; b++)
       if ((fabs(divphi[b]) > R(0.0)) && ((phi[b] + dphi[b]) > R(0.0)) && ((phi[b] + dphi[b]) < R(1.0))) {
-        if (fabs(sum_phib) > R(0.0)) dphi[b] -= Deltaphi * (phi[b]) / (sum_phib);
-        else dphi[b] -= Deltaphi / R(count);
+        if (fabs(sum_phib) > R(0.0)) dphi[b] -= qdiv(Deltaphi * (phi[b]), (sum_phib));
+        else dphi[b] -= qdiv(Deltaphi, R(count));
       }
   bool done = false;
   for (int a = 0; a < N; a++)
@@ -453,7 +473,7 @@ GP_HD void project_simplex_ww(const R *phi, const R *divphi, R *dphi) {
       dphi[a] += d;
       for (int b = 0; b < N; b++)
         if (((fabs(divphi[b]) > R(0.0)) && b != a) && (phi[b] + dphi[b]) > R(0.0) && (phi[b] + dphi[b]) < R(1.0))
-          dphi[b] -= d / R(count);
+          dphi[b] -= qdiv(d, R(count));
     }
   }
 }
@@ -510,10 +530,10 @@ GP_HD void phi_delta(const Model<R> &m, const V &v, bool smooth, R *out) {
       active++;
     }
   }
-  if (active) sum_lambda /= R(active);
+  if (active) sum_lambda = qdiv(sum_lambda, R(active));
   const R tauv = tau_of<N>(m, phi);
   for (int a = 0; a < N; a++) {
-    if (fabs(divphi[a]) > R(0.0)) out[a] = m.dt * (lambda[a] - sum_lambda) / (tauv * m.eps);
+    if (fabs(divphi[a]) > R(0.0)) out[a] = qdiv(m.dt * (lambda[a] - sum_lambda), (tauv * m.eps));
     else out[a] = R(0.0);
   }
   if (smooth) project_simplex_ww<N>(phi, divphi, out);
@@ -523,7 +543,7 @@ GP_HD void phi_delta(const Model<R> &m, const V &v, bool smooth, R *out) {
 /* ------------------------------------------------------------------ anti-trapping current */
 template <int N, typename R>
 GP_HD R s_obst(const R *phi, int a) {
-  if (phi[a] * (R(1.0) - phi[a]) > R(0.0)) return phi[a] * (R(1.0) - hphi<N>(phi, a)) / (sqrt(phi[a] * (R(1.0) - phi[a])));
+  if (phi[a] * (R(1.0) - phi[a]) > R(0.0)) return qdiv(phi[a] * (R(1.0) - hphi<N>(phi, a)), (qsqrt(phi[a] * (R(1.0) - phi[a]))));
   return R(0.0);
 }
 
@@ -549,11 +569,11 @@ GP_HD void jat_x(const Model<R> &m, const V &v0, int i, int j, R *jx) {
   }
   R gyl = R(0.5) * (cy(m, v, L, 0, 0) + cy(m, v, L, 1, 0));
   R gxL = gx(m, v, L, 0, 0);
-  R nxl = sqrt(gyl * gyl + gxL * gxL);
+  R nxl = qsqrt(gyl * gyl + gxL * gxL);
   for (int a = 0; a < N - 1; a++) {
     R gpy = R(0.5) * (cy(m, v, a, 0, 0) + cy(m, v, a, 1, 0));
     R gxa = gx(m, v, a, 0, 0);
-    R ng = sqrt(gpy * gpy + gxa * gxa);
+    R ng = qsqrt(gpy * gpy + gxa * gxa);
     if (!(ng != R(0.0))) continue;
     R s_c = R(0), s_f = R(0);
     if (m.fn_w == 1) {
@@ -566,11 +586,11 @@ GP_HD void jat_x(const Model<R> &m, const V &v0, int i, int j, R *jx) {
       R cs_f = c_mu<NC>(m, muf, a, k), cs_c = c_mu<NC>(m, muc, a, k);
       R sp = gxa * gxL;
       sp += gpy * gyl;
-      if (nxl > R(0.0)) sp /= (nxl * ng);
+      if (nxl > R(0.0)) sp = qdiv(sp, (nxl * ng));
       if (m.fn_w == 2)
-        jx[k] += m.dratio[a][k] * fabs(sp) * m.jat_well * ((cl_f - cs_f) * df + (cl_c - cs_c) * dc) * (gxa) / ng;
+        jx[k] += qdiv(m.dratio[a][k] * fabs(sp) * m.jat_well * ((cl_f - cs_f) * df + (cl_c - cs_c) * dc) * (gxa), ng);
       else
-        jx[k] += m.dratio[a][k] * fabs(sp) * R(0.5) * (s_f * (cl_f - cs_f) * df + s_c * (cl_c - cs_c) * dc) * (gxa) / ng;
+        jx[k] += qdiv(m.dratio[a][k] * fabs(sp) * R(0.5) * (s_f * (cl_f - cs_f) * df + s_c * (cl_c - cs_c) * dc) * (gxa), ng);
     }
   }
 }
@@ -596,11 +616,11 @@ GP_HD void jat_y(const Model<R> &m, const V &v0, int i, int j, int64_t yglob, R 
   }
   R gxl = R(0.5) * (cx(m, v, L, 0, 0) + cx(m, v, L, 0, 1));
   R gyL = gy(m, v, L, 0, 0);
-  R nyl = sqrt(gxl * gxl + gyL * gyL);
+  R nyl = qsqrt(gxl * gxl + gyL * gyL);
   for (int a = 0; a < N - 1; a++) {
     R gpx = R(0.5) * (cx(m, v, a, 0, 0) + cx(m, v, a, 0, 1));
     R gya = gy(m, v, a, 0, 0);
-    R ng = sqrt(gpx * gpx + gya * gya);
+    R ng = qsqrt(gpx * gpx + gya * gya);
     if (!(ng != R(0.0))) continue;
     R s_c = R(0), s_r = R(0);
     if (m.fn_w == 1) {
@@ -613,11 +633,11 @@ GP_HD void jat_y(const Model<R> &m, const V &v0, int i, int j, int64_t yglob, R 
       R cs_r = c_mu<NC>(m, mur, a, k), cs_c = c_mu<NC>(m, muc, a, k);
       R sp = gya * gyL;
       sp += gpx * gxl;
-      if (nyl > R(0.0)) sp /= (nyl * ng);
+      if (nyl > R(0.0)) sp = qdiv(sp, (nyl * ng));
       if (m.fn_w == 2)
-        jy[k] += m.dratio[a][k] * fabs(sp) * m.jat_well * ((cl_r - cs_r) * dr + (cl_c - cs_c) * dc) * gya / ng;
+        jy[k] += qdiv(m.dratio[a][k] * fabs(sp) * m.jat_well * ((cl_r - cs_r) * dr + (cl_c - cs_c) * dc) * gya, ng);
       else
-        jy[k] += m.dratio[a][k] * fabs(sp) * R(0.5) * (s_r * (cl_r - cs_r) * dr + s_c * (cl_c - cs_c) * dc) * gya / ng;
+        jy[k] += qdiv(m.dratio[a][k] * fabs(sp) * R(0.5) * (s_r * (cl_r - cs_r) * dr + s_c * (cl_c - cs_c) * dc) * gya, ng);
     }
   }
 }
@@ -678,7 +698,7 @@ GP_HD void lu_inverse(R (*a)[n], R (*inv)[n]) {
       factor[t1][i] = s;
     }
     for (int i = k + 1; i < n; i++) {
-      R fact = -a[i][k] / a[k][k];
+      R fact = qdiv(-a[i][k], a[k][k]);
       factor[i][k] = -fact;
       for (int j = k; j <= n - 1; j++) a[i][j] = fact * a[k][j] + a[i][j];
     }
@@ -691,11 +711,11 @@ GP_HD void lu_inverse(R (*a)[n], R (*inv)[n]) {
       for (int j = 0; j < i; j++) s = s - factor[i][j] * v1[j];
       v1[i] = ((i == col) ? R(1.0) : R(0.0)) + s;
     }
-    v[n - 1] = v1[n - 1] * (R(1.0) / a[n - 1][n - 1]);
+    v[n - 1] = v1[n - 1] * qdiv(R(1.0), a[n - 1][n - 1]);
     for (int i = n - 2; i >= 0; i--) {
       R s = R(0);
       for (int j = i + 1; j < n; j++) s = s - a[i][j] * v[j];
-      v[i] = (v1[i] + s) * (R(1.0) / a[i][i]);
+      v[i] = (v1[i] + s) * qdiv(R(1.0), a[i][i]);
     }
     for (int j = 0; j < n; j++) inv1[j][col] = v[j];
   }
@@ -774,16 +794,16 @@ GP_HD void mu_update(const Model<R> &m, const V &v, bool smooth, const R *dph, c
     }
     const int mode = smooth ? GP_SOLVE_LU : m.solve;
     if (mode == GP_SOLVE_BINARY && NC == 1) {
-      R inv = R(1.0) / dcdmu[0][0];
+      R inv = qdiv(R(1.0), dcdmu[0][0]);
       deltamu[0] = deltac[0] * inv;
       munew[0] += deltamu[0];
     } else if (mode == GP_SOLVE_TERNARY && NC == 2) {
       R DET = dcdmu[0][0] * dcdmu[1 % NC][1 % NC] - dcdmu[0][1 % NC] * dcdmu[1 % NC][0];
       R inv[NC][NC];
-      inv[0][0] = dcdmu[1 % NC][1 % NC] / DET;
-      inv[1 % NC][1 % NC] = dcdmu[0][0] / DET;
-      inv[0][1 % NC] = -dcdmu[0][1 % NC] / DET;
-      inv[1 % NC][0] = -dcdmu[1 % NC][0] / DET;
+      inv[0][0] = qdiv(dcdmu[1 % NC][1 % NC], DET);
+      inv[1 % NC][1 % NC] = qdiv(dcdmu[0][0], DET);
+      inv[0][1 % NC] = qdiv(-dcdmu[0][1 % NC], DET);
+      inv[1 % NC][0] = qdiv(-dcdmu[1 % NC][0], DET);
       for (int k = 0; k < NC; k++) {
         deltamu[k] = R(0.0);
         for (int l = 0; l < NC; l++) deltamu[k] += inv[k][l] * deltac[l];
@@ -791,7 +811,7 @@ GP_HD void mu_update(const Model<R> &m, const V &v, bool smooth, const R *dph, c
       for (int k = 0; k < NC; k++) munew[k] = deltamu[k] + munew[k];
     } else if (mode == GP_SOLVE_DILUTE) {
       for (int k = 0; k < NC; k++) {
-        R inv = R(1.0) / dcdmu[k][k];
+        R inv = qdiv(R(1.0), dcdmu[k][k]);
         deltamu[k] = deltac[k] * inv;
         munew[k] += deltamu[k];
       }

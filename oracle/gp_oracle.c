@@ -30,7 +30,8 @@
  * binary built from its own sources (oracle/Makefile "ref"), at steps 0/1/10/N for eleven input decks covering
  * every branch (BINARY/TERNARY/DILUTE/LU solves, obstacle/well, isotropic/anisotropic with rotation, Neumann/
  * periodic/mixed sides, undercooling, smoothing, 4 phases with third-order terms). The invariants
- * (solute, energy) are not in the reference: they are pinned by closed forms in tests/test_oracle_math.py.
+ * (solute, energy) are not in the reference: they are pinned by closed forms in tests/test_invariants.py
+ * (uniform pure phases; a linear two-phase profile with a known gradient, anisotropy and potential term).
  */
 #include "gp_oracle.h"
 
