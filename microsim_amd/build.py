@@ -20,8 +20,10 @@ ARCH = os.environ.get("GP_OFFLOAD_ARCH", "gfx950")
 
 HOST_CFLAGS = ["-O2", "-std=gnu11", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-Wno-unused-function"]
 # FP64 parity needs IEEE division/sqrt and no contraction; packed FP32 instructions are switched off.
+# -fno-hip-fp32-correctly-rounded-divide-sqrt affects single precision only (the FP32 production mode).
 HIP_FLAGS = ["-O3", "-fPIC", "-shared", "-std=c++17", "--offload-arch=%s" % ARCH, "-ffp-contract=off",
-             "-fno-fast-math", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-fno-fast-math", "-fno-hip-fp32-correctly-rounded-divide-sqrt",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-result", "-munsafe-fp-atomics"]
 
 

@@ -81,23 +81,20 @@ struct View {
   }
 };
 
-/* Quotient and square root of the step arithmetic. FP64: IEEE (correctly rounded), as the reference computes
- * them, so the FP64 results are bit-identical to the reference. FP32, the production mode: the hardware
- * reciprocal and square root (v_rcp_f32, v_sqrt_f32, about 1 ulp), two instructions where the correctly rounded
- * FP32 sequences take about ten; the FP32 mode's error is set by its FP32 storage, far above 1 ulp per
- * operation. Host builds (the CPU checks) use the IEEE forms. */
+/* Quotient and square root of the step arithmetic (the one place that chooses their form). FP64: IEEE,
+ * correctly rounded, as the reference computes them, so the FP64 results are bit-identical to it. FP32 (the
+ * production mode) takes the device build's fast single-precision forms (hipcc
+ * -fno-hip-fp32-correctly-rounded-divide-sqrt: reciprocal-based quotient and hardware square root of about
+ * 1-2.5 ulp, with denormal operands rescaled), about half the instructions of the correctly rounded
+ * sequences; the FP32 mode's error is set by its FP32 storage, far above that. A bare v_rcp_f32 is not used:
+ * it flushes denormal operands, and FP32 denominators such as |q|^6 of the anisotropy derivative reach the
+ * denormal range. */
 template <typename R>
 GP_HD R qdiv(R a, R b) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  if constexpr (sizeof(R) == sizeof(float)) return a * __builtin_amdgcn_rcpf(b);
-#endif
   return a / b;
 }
 template <typename R>
 GP_HD R qsqrt(R a) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  if constexpr (sizeof(R) == sizeof(float)) return __builtin_amdgcn_sqrtf(a);
-#endif
   return sqrt(a);
 }
 
