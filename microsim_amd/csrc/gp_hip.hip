@@ -62,6 +62,8 @@ struct gp_handle {
   int march_seg, march_nstrips, march_nsegs; /* tile grid of the single-pass step (see gp_march.h) */
   int64_t march_nwaves;     /* tiles = wavefronts of k_march and of k_bulk */
   int *flags[2];            /* per-tile purity flags of the last two steps */
+  int8_t *codes;            /* phase code of every cell of the current state (pure phase b, or -1) */
+  bool codes_valid;         /* codes match the current state */
   int fcur;                 /* flags[fcur] describes the current state */
   bool flags_valid;         /* flags[fcur] is valid for the current state (else no tile is taken as deep) */
   bool use_march;           /* single-pass steps (default) or the two-pass form */
@@ -393,31 +395,42 @@ static int launch_step_p(gp_handle *h, int mode) {
     const int *fin = h->flags_valid ? h->flags[h->fcur] : nullptr;
     int *fout = h->flags[1 - h->fcur];
     const int64_t pstride = 2 * (int64_t)nt;
+    if (!h->codes_valid) {
+      hipLaunchKernelGGL((k_codes<N, R>), dim3((unsigned)((h->ry + 63) / 64), (unsigned)((h->rx + 3) / 4)), dim3(256),
+                         0, h->stream, phi, h->codes, h->fs, h->ps, h->rx, h->ry);
+    }
+    {
+      /* delta-phi of the candidate cells, tiles of 64 y-cells by kDTileRows rows over x, y in [2, mesh+3] */
+      const int nty = (int)((my + 2 + 63) / 64), ntx = (int)((mx + 2 + kDTileRows - 1) / kDTileRows);
+      ProfScope ps(h, GP_KERNEL_DPHI);
+      hipLaunchKernelGGL((k_dphi<N, NC, R, P2>), dim3((unsigned)((nty * ntx + 3) / 4)), dim3(256), 0, h->stream, m,
+                         phi, mu, h->codes, dphi, h->fs, h->ps, mx, my, nty, nty * ntx);
+    }
     if (mode == MODE_MARCH) {
       {
         ProfScope ps(h, GP_KERNEL_FUSED);
-        hipLaunchKernelGGL((k_march<N, NC, R, P2, false>), dim3(nblk), dim3(256), 0, h->stream, m, phi, mu, phi_o,
-                           mu_o, h->maps, h->partials, pstride, fin, fout, h->fs, h->ps, mx, my, h->march_seg,
-                           h->march_nstrips, h->march_nsegs, nt);
+        hipLaunchKernelGGL((k_march<N, NC, R, P2, false>), dim3(nblk), dim3(256), 0, h->stream, m, phi, mu, dphi,
+                           phi_o, mu_o, h->codes, h->maps, h->partials, pstride, fin, fout, h->fs, h->ps, mx, my,
+                           h->march_seg, h->march_nstrips, h->march_nsegs, nt);
       }
       if (fin) {
         ProfScope ps(h, GP_KERNEL_BULK);
         hipLaunchKernelGGL((k_bulk<N, NC, R, P2, false>), dim3(nblk), dim3(256), 0, h->stream, m, mu, phi_o, mu_o,
-                           h->partials + nt, pstride, fin, fout, h->fs, h->ps, mx, h->march_seg, h->march_nstrips,
-                           h->march_nsegs, nt);
+                           h->codes, h->partials + nt, pstride, fin, fout, h->fs, h->ps, mx, h->march_seg,
+                           h->march_nstrips, h->march_nsegs, nt);
       }
     } else {
       {
         ProfScope ps(h, GP_KERNEL_FUSED_STATS);
-        hipLaunchKernelGGL((k_march<N, NC, R, P2, true>), dim3(nblk), dim3(256), 0, h->stream, m, phi, mu, phi_o,
-                           mu_o, h->maps, h->partials, pstride, fin, fout, h->fs, h->ps, mx, my, h->march_seg,
-                           h->march_nstrips, h->march_nsegs, nt);
+        hipLaunchKernelGGL((k_march<N, NC, R, P2, true>), dim3(nblk), dim3(256), 0, h->stream, m, phi, mu, dphi,
+                           phi_o, mu_o, h->codes, h->maps, h->partials, pstride, fin, fout, h->fs, h->ps, mx, my,
+                           h->march_seg, h->march_nstrips, h->march_nsegs, nt);
       }
       if (fin) {
         ProfScope ps(h, GP_KERNEL_BULK);
         hipLaunchKernelGGL((k_bulk<N, NC, R, P2, true>), dim3(nblk), dim3(256), 0, h->stream, m, mu, phi_o, mu_o,
-                           h->partials + nt, pstride, fin, fout, h->fs, h->ps, mx, h->march_seg, h->march_nstrips,
-                           h->march_nsegs, nt);
+                           h->codes, h->partials + nt, pstride, fin, fout, h->fs, h->ps, mx, h->march_seg,
+                           h->march_nstrips, h->march_nsegs, nt);
       }
       ProfScope ps(h, GP_KERNEL_STATS);
       hipLaunchKernelGGL((k_stats<N, NC>), dim3(3 * N + 3 * NC + 1), dim3(kStatThreads), 0, h->stream,
@@ -427,9 +440,11 @@ static int launch_step_p(gp_handle *h, int mode) {
     h->cur = 1 - h->cur;
     h->fcur = 1 - h->fcur;
     h->flags_valid = true;
+    h->codes_valid = true;
     return GP_OK;
   }
-  h->flags_valid = false; /* the two-pass kernels do not form tile flags */
+  h->flags_valid = false; /* the two-pass kernels form neither tile flags nor phase codes */
+  h->codes_valid = false;
   if (smooth) {
     {
       ProfScope ps(h, GP_KERNEL_PHI);
@@ -628,6 +643,7 @@ int gp_create(const gp_params *params, int precision, gp_handle **out) {
   h->prev_valid = false;
   h->fcur = 0;
   h->flags_valid = false;
+  h->codes_valid = false;
   const size_t es = precision == GP_FP64 ? 8 : 4;
   const size_t pn = (size_t)h->N * h->fs * es, mn = (size_t)h->NC * h->fs * es;
   hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
@@ -640,6 +656,7 @@ int gp_create(const gp_params *params, int precision, gp_handle **out) {
   ok = ok && hipMalloc((void **)&h->inv_dev, sizeof(double) * (GP_MAXNC + 1)) == hipSuccess;
   ok = ok && hipMalloc((void **)&h->flags[0], sizeof(int) * h->march_nwaves) == hipSuccess;
   ok = ok && hipMalloc((void **)&h->flags[1], sizeof(int) * h->march_nwaves) == hipSuccess;
+  ok = ok && hipMalloc((void **)&h->codes, (size_t)h->rx * h->ps) == hipSuccess;
   const size_t msize = precision == GP_FP64 ? sizeof(Model<double>) : sizeof(Model<float>);
   ok = ok && hipMalloc(&h->model, msize) == hipSuccess;
   if (!ok) {
@@ -686,7 +703,7 @@ void gp_destroy(gp_handle *h) {
   }
   h->prof.clear();
   void *bufs[] = {h->phi[0], h->phi[1], h->mu[0], h->mu[1], h->dphi, h->partials, h->stats_dev, h->model,
-                  h->flags[0], h->flags[1], h->inv_dev};
+                  h->flags[0], h->flags[1], h->inv_dev, h->codes};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -710,6 +727,7 @@ int gp_upload(gp_handle *h, const double *phi, const double *mu) {
   h->steps = 0;
   h->prev_valid = false;
   h->flags_valid = false;
+  h->codes_valid = false;
   return GP_OK;
 }
 
@@ -820,7 +838,8 @@ int gp_profile_enable(gp_handle *h, int enable) {
 }
 
 const char *gp_kernel_name(int kind) {
-  static const char *names[GP_NKERNELS] = {"k_phi", "k_mu", "k_stats", "k_march", "k_march_stats", "k_bulk"};
+  static const char *names[GP_NKERNELS] = {"k_phi", "k_mu", "k_stats", "k_march", "k_march_stats", "k_bulk",
+                                           "k_dphi"};
   return (kind >= 0 && kind < GP_NKERNELS) ? names[kind] : "?";
 }
 

@@ -31,17 +31,28 @@ namespace gpk {
 
 constexpr int kMarchLanes = 64;
 constexpr int kMarchOut = 60;
+/* rows the march keeps in flight ahead of the current one (registers: (N + NC) per row) */
+template <typename R>
+constexpr int kMarchDepth = sizeof(R) == sizeof(float) ? 3 : 2;
 
-__device__ __forceinline__ double lane_up(double v) { return __shfl_up(v, 1, 64); }   /* value of lane-1 */
-__device__ __forceinline__ double lane_dn(double v) { return __shfl_down(v, 1, 64); } /* value of lane+1 */
-__device__ __forceinline__ float lane_up(float v) { return __shfl_up(v, 1, 64); }
-__device__ __forceinline__ float lane_dn(float v) { return __shfl_down(v, 1, 64); }
-
-template <int N>
-struct Pairs {
-  static constexpr int n = N * (N - 1) / 2;
-  static constexpr GP_HD int idx(int a, int b) { return a * (2 * N - a - 1) / 2 + (b - a - 1); } /* a < b */
-};
+/* Neighbour lanes through DPP wavefront shifts (GFX9 wave_shr:1 / wave_shl:1): a VALU move that takes the value
+ * of lane-1 (lane_up) or lane+1 (lane_dn) without the LDS round trip of a bpermute; the lane without a source
+ * (0, resp. 63) keeps its own value, like __shfl_up / __shfl_down. */
+constexpr int kDppWaveShr1 = 0x138, kDppWaveShl1 = 0x130;
+__device__ __forceinline__ int lane_up(int v) { return __builtin_amdgcn_update_dpp(v, v, kDppWaveShr1, 0xf, 0xf, false); }
+__device__ __forceinline__ int lane_dn(int v) { return __builtin_amdgcn_update_dpp(v, v, kDppWaveShl1, 0xf, 0xf, false); }
+__device__ __forceinline__ float lane_up(float v) { return __int_as_float(lane_up(__float_as_int(v))); }
+__device__ __forceinline__ float lane_dn(float v) { return __int_as_float(lane_dn(__float_as_int(v))); }
+__device__ __forceinline__ double lane_up(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = lane_up((int)(b & 0xffffffffll)), hi = lane_up((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double lane_dn(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = lane_dn((int)(b & 0xffffffffll)), hi = lane_dn((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 
 template <int N, int NC, typename R>
 struct RowRaw {
@@ -81,174 +92,6 @@ __device__ __forceinline__ void row_halo(RowRaw<N, NC, R> &r) {
 #pragma unroll
   for (int k = 0; k < NC; k++) {
     r.mud[k] = lane_dn(r.mu[k]);
-  }
-}
-
-/* per-face anisotropy data of one face, per unordered pair: U = a_c^2 r (or r, isotropic), V = (a_c |q|^2) rd */
-template <int N, typename R>
-struct FaceAn {
-  R U[Pairs<N>::n > 0 ? Pairs<N>::n : 1];
-  R V[Pairs<N>::n > 0 ? Pairs<N>::n : 1];
-};
-
-/* x-face between rows r and r+1: sx, gx of the face, cy of both rows (q_divx, function_A_01.h:135-206) */
-template <int N, typename R>
-__device__ __forceinline__ void face_x(const Model<R> &m, const R *sxv, const R *gxv, const R *cy0, const R *cy1,
-                                       FaceAn<N, R> &f, unsigned need) {
-#pragma unroll
-  for (int a = 0; a < N; a++)
-#pragma unroll
-    for (int b = a + 1; b < N; b++) {
-      const int p = Pairs<N>::idx(a, b);
-      if (!((need >> p) & 1u)) continue;
-      const R r0 = (sxv[a] * gxv[b] - sxv[b] * gxv[a]);
-      if (!m.fn_aniso) {
-        f.U[p] = r0;
-        f.V[p] = R(0);
-        continue;
-      }
-      R r[2], q[2], dq[2], rd[2];
-      r[0] = r0;
-      r[1] = R(0.5) * (sxv[a] * (cy0[b] + cy1[b]) - sxv[b] * (cy0[a] + cy1[a]));
-      rot2(m.rot[a][b], r, q);
-      const R ac = aniso_ac(m.dab[a][b], q);
-      aniso_dadq(m.dab[a][b], q, dq);
-      rot2(m.rot_inv[a][b], dq, rd);
-      const R ac2 = ac * ac;
-      f.U[p] = ac2 * r0;
-      f.V[p] = ac * qnorm2(q) * rd[0];
-    }
-}
-
-/* y-face between lanes y and y+1 of one row: sy, gy of the face, cx at y and y+1 (q_divy) */
-template <int N, typename R>
-__device__ __forceinline__ void face_y(const Model<R> &m, const R *syv, const R *gyv, const R *cx0, const R *cx1,
-                                       FaceAn<N, R> &f, unsigned need) {
-#pragma unroll
-  for (int a = 0; a < N; a++)
-#pragma unroll
-    for (int b = a + 1; b < N; b++) {
-      const int p = Pairs<N>::idx(a, b);
-      if (!((need >> p) & 1u)) continue;
-      const R r1 = (syv[a] * gyv[b] - syv[b] * gyv[a]);
-      if (!m.fn_aniso) {
-        f.U[p] = r1;
-        f.V[p] = R(0);
-        continue;
-      }
-      R r[2], q[2], dq[2], rd[2];
-      r[0] = R(0.5) * (syv[a] * (cx0[b] + cx1[b]) - syv[b] * (cx0[a] + cx1[a]));
-      r[1] = r1;
-      rot2(m.rot[a][b], r, q);
-      const R ac = aniso_ac(m.dab[a][b], q);
-      aniso_dadq(m.dab[a][b], q, dq);
-      rot2(m.rot_inv[a][b], dq, rd);
-      const R ac2 = ac * ac;
-      f.U[p] = ac2 * r1;
-      f.V[p] = ac * qnorm2(q) * rd[1];
-    }
-}
-
-template <int N, typename R>
-__device__ __forceinline__ void face_shuffle_up(const FaceAn<N, R> &src, FaceAn<N, R> &dst, unsigned need) {
-#pragma unroll
-  for (int p = 0; p < Pairs<N>::n; p++) {
-    if (!((need >> p) & 1u)) continue;
-    dst.U[p] = lane_up(src.U[p]);
-    dst.V[p] = lane_up(src.V[p]);
-  }
-}
-
-/* div(da/dgrad phi_a) from the four faces (function_A_01_divdAdgradphi / function_A_00_divdAdgradphi) */
-template <int N, class V, typename R>
-__device__ __forceinline__ R div_dadgrad(const Model<R> &m, int a, const FaceAn<N, R> &ff, const FaceAn<N, R> &fb,
-                                         const FaceAn<N, R> &fr, const FaceAn<N, R> &fl, const R *sxf, const R *sxb,
-                                         const R *syr, const R *syl, unsigned need) {
-  R sum1 = R(0.0), sum2 = R(0.0);
-#pragma unroll
-  for (int b = 0; b < N; b++) {
-    if (b == a) continue;
-    const int p = a < b ? Pairs<N>::idx(a, b) : Pairs<N>::idx(b, a);
-    if (!((need >> p) & 1u)) continue; /* partner absent: the pair adds a signed zero */
-    const bool s = a < b;
-    const R Uf = s ? ff.U[p] : -ff.U[p], Ub = s ? fb.U[p] : -fb.U[p];
-    const R Ur = s ? fr.U[p] : -fr.U[p], Ul = s ? fl.U[p] : -fl.U[p];
-    if (m.fn_aniso) {
-      const R Vf = s ? ff.V[p] : -ff.V[p], Vb = s ? fb.V[p] : -fb.V[p];
-      const R Vr = s ? fr.V[p] : -fr.V[p], Vl = s ? fl.V[p] : -fl.V[p];
-      R d1 = divx<V>(m, (Uf * (-sxf[b])) - (Ub * (-sxb[b])));
-      d1 += divy<V>(m, (Ur * (-syr[b])) - (Ul * (-syl[b])));
-      d1 *= m.gamma[a][b];
-      sum1 += d1;
-      R d2 = divx<V>(m, (Vf * (-sxf[b])) - (Vb * (-sxb[b])));
-      d2 += divy<V>(m, (Vr * (-syr[b])) - (Vl * (-syl[b])));
-      d2 *= m.gamma[a][b];
-      sum2 += d2;
-    } else {
-      R d = divx<V>(m, Uf * (-sxf[b]) - (Ub * (-sxb[b])));
-      d += divy<V>(m, Ur * (-syr[b]) - (Ul * (-syl[b])));
-      d *= m.gamma[a][b];
-      sum1 += d;
-    }
-  }
-  if (m.fn_aniso) return (R(2.0) * (sum1 + sum2) + R(0.0));
-  return R(2.0) * sum1;
-}
-
-/* d a / d phi_a at the cell centre with per-pair sharing (function_A_01_dAdphi) */
-template <int N, typename R>
-__device__ __forceinline__ void dadphi_all(const Model<R> &m, const R *phi, const R *cxc, const R *cyc, R *out,
-                                           unsigned need) {
-  if (!m.fn_aniso) {
-#pragma unroll
-    for (int a = 0; a < N; a++) out[a] = dAdphi_A00<N>(m, phi, cxc, cyc, a);
-    return;
-  }
-  constexpr int NP = Pairs<N>::n;
-  R qp[NP > 0 ? NP : 1][2], dq[NP > 0 ? NP : 1][2], ac1[NP > 0 ? NP : 1], q2[NP > 0 ? NP : 1];
-#pragma unroll
-  for (int a = 0; a < N; a++)
-#pragma unroll
-    for (int b = a + 1; b < N; b++) {
-      const int p = Pairs<N>::idx(a, b);
-      if (!((need >> p) & 1u)) continue;
-      R r[2];
-      r[0] = (phi[a] * cxc[b] - phi[b] * cxc[a]);
-      r[1] = (phi[a] * cyc[b] - phi[b] * cyc[a]);
-      rot2(m.rot[a][b], r, qp[p]);
-      aniso_dadq(m.dab[a][b], qp[p], dq[p]);
-      q2[p] = qnorm2(qp[p]);
-      ac1[p] = aniso_ac(m.dab[a][b], qp[p]);
-    }
-#pragma unroll
-  for (int a = 0; a < N; a++) {
-    R sum1 = R(0.0), sum2 = R(0.0);
-#pragma unroll
-    for (int b = 0; b < N; b++) {
-      if (b == a) continue;
-      const bool s = a < b;
-      const int p = s ? Pairs<N>::idx(a, b) : Pairs<N>::idx(b, a);
-      if (!((need >> p) & 1u)) continue;
-      R q[2], d[2], v[2], vr[2];
-      q[0] = s ? qp[p][0] : -qp[p][0];
-      q[1] = s ? qp[p][1] : -qp[p][1];
-      d[0] = s ? dq[p][0] : -dq[p][0];
-      d[1] = s ? dq[p][1] : -dq[p][1];
-      v[0] = cxc[b];
-      v[1] = cyc[b];
-      rot2(m.rot[a][b], v, vr);
-      R s1 = R(0.0), s2 = R(0.0);
-      s1 += q[0] * vr[0];
-      s1 += q[1] * vr[1];
-      s2 += d[0] * vr[0];
-      s2 += d[1] * vr[1];
-      const R ac2 = ac1[p] * ac1[p];
-      s1 *= m.gamma[a][b] * ac2;
-      s2 *= m.gamma[a][b] * ac1[p] * q2[p];
-      sum1 += s1;
-      sum2 += s2;
-    }
-    out[a] = R(2.0) * (sum1 + sum2);
   }
 }
 
@@ -313,73 +156,58 @@ __device__ __forceinline__ void mobility_cell(const Model<R> &m, const R *phi, R
     }
 }
 
-/* delta-phi of the centre from precomputed centre / face data (calculate_divergence_phasefield_2D body) */
-template <int N, int NC, class V, typename R>
-__device__ __forceinline__ void dphi_cell(const Model<R> &m, const R *phi, const R *divphi, const R *psi,
-                                          const R *cxc, const R *cyc, const FaceAn<N, R> &ff,
-                                          const FaceAn<N, R> &fb, const FaceAn<N, R> &fr, const FaceAn<N, R> &fl,
-                                          const R *sxf, const R *sxb, const R *syr, const R *syl, R *out,
-                                          unsigned need) {
-  bool any = false;
-#pragma unroll
-  for (int a = 0; a < N; a++) any = any || (fabs(divphi[a]) > R(0.0));
-#pragma unroll
-  for (int a = 0; a < N; a++) out[a] = R(0.0);
-  if (any) {
-    R dA[N];
-    dadphi_all<N>(m, phi, cxc, cyc, dA, need);
-    R lambda[N], sum_lambda = R(0.0);
-    int active = 0;
-#pragma unroll
-    for (int a = 0; a < N; a++) {
-      lambda[a] = R(0);
-      if (fabs(divphi[a]) > R(0.0)) {
-        const R dD = div_dadgrad<N, V>(m, a, ff, fb, fr, fl, sxf, sxb, syr, syl, need);
-        R dp = R(0.0);
-#pragma unroll
-        for (int b = 0; b < N; b++) dp += psi[b] * dhphi<N>(phi, b, a);
-        lambda[a] = m.eps * (-dA[a] + dD) - m.inv_eps * dwdphi<N>(m, phi, divphi, a, false) - dp;
-        sum_lambda += lambda[a];
-        active++;
-      }
-    }
-    if (active) sum_lambda = qdiv(sum_lambda, R(active));
-    const R tauv = tau_of<N>(m, phi);
-#pragma unroll
-    for (int a = 0; a < N; a++)
-      if (fabs(divphi[a]) > R(0.0)) out[a] = qdiv(m.dt * (lambda[a] - sum_lambda), (tauv * m.eps));
-  }
-  project_simplex<N>(phi, divphi, out);
-}
-
-/* per-lane statistics: max/min of each phi and mu, sums of squared increments, non-finite count */
-template <int N, int NC, int NSA>
-__device__ __forceinline__ void stats_init(double (&st)[NSA]) {
+/* Per-lane statistics in the state's precision R: max/min of each phi and mu (exact in R), the sums of squared
+ * increments over the lane's cells of one tile (at most a few dozen terms) and the non-finite count; the wavefront
+ * and grid reductions are FP64. In FP64 every term is as the reference forms it; in FP32 the per-lane partial sums
+ * carry single-precision rounding (relative 1e-6), which the FP32 mode accepts for these diagnostics, in exchange
+ * for half the registers. */
+template <int N, int NC, int NSA, typename R>
+__device__ __forceinline__ void stats_init(R (&st)[NSA]) {
 #pragma unroll
   for (int a = 0; a < N; a++) {
     st[a] = -INFINITY;
     st[N + a] = INFINITY;
-    st[2 * N + 2 * NC + a] = 0.0;
+    st[2 * N + 2 * NC + a] = R(0.0);
   }
 #pragma unroll
   for (int k = 0; k < NC; k++) {
     st[2 * N + k] = -INFINITY;
     st[2 * N + NC + k] = INFINITY;
-    st[3 * N + 2 * NC + k] = 0.0;
+    st[3 * N + 2 * NC + k] = R(0.0);
   }
-  st[NSA - 1] = 0.0;
+  st[NSA - 1] = R(0.0);
 }
 
-/* wavefront reduction of the per-lane statistics into partials[i * pstride + w] (statistic-major) */
-template <int N, int NC, int NSA>
-__device__ __forceinline__ void stats_partials(double (&st)[NSA], double *__restrict__ partials, int64_t pstride,
+/* accumulate one updated cell: new phi / mu and their increments */
+template <int N, int NC, int NSA, typename R>
+__device__ __forceinline__ void stats_add(R (&st)[NSA], const R *pn, const R *dph, const R *munew, const R *dmu) {
+#pragma unroll
+  for (int a = 0; a < N; a++) {
+    st[a] = fmax(st[a], pn[a]);
+    st[N + a] = fmin(st[N + a], pn[a]);
+    st[2 * N + 2 * NC + a] += dph[a] * dph[a];
+    if (!isfinite(pn[a])) st[NSA - 1] += R(1.0);
+  }
+#pragma unroll
+  for (int k = 0; k < NC; k++) {
+    st[2 * N + k] = fmax(st[2 * N + k], munew[k]);
+    st[2 * N + NC + k] = fmin(st[2 * N + NC + k], munew[k]);
+    st[3 * N + 2 * NC + k] += dmu[k] * dmu[k];
+    if (!isfinite(munew[k])) st[NSA - 1] += R(1.0);
+  }
+}
+
+/* wavefront reduction of the per-lane statistics into partials[i * pstride + w] (statistic-major), in FP64 */
+template <int N, int NC, int NSA, typename R>
+__device__ __forceinline__ void stats_partials(R (&st)[NSA], double *__restrict__ partials, int64_t pstride,
                                                int w, int nwaves, int lane) {
 #pragma unroll
   for (int i = 0; i < NSA; i++) {
     double r;
-    if (i < N || (i >= 2 * N && i < 2 * N + NC)) r = wave_max(st[i]);
-    else if (i < 2 * N + 2 * NC) r = wave_min(st[i]);
-    else r = wave_sum(st[i]);
+    const double v = (double)st[i];
+    if (i < N || (i >= 2 * N && i < 2 * N + NC)) r = wave_max(v);
+    else if (i < 2 * N + 2 * NC) r = wave_min(v);
+    else r = wave_sum(v);
     if (lane == 0 && w < nwaves) partials[(int64_t)i * pstride + w] = r;
   }
 }
@@ -404,15 +232,18 @@ __device__ __forceinline__ int deep_phase(const int *__restrict__ flags, int str
 }
 
 /* STATS: accumulate the step statistics (one partial per wavefront at partials[i * pstride + w]); off for the
- * steps that do not report. flags_in (may be NULL: no tile is deep) and flags_out: tile flags of the previous and
- * of this step; deep tiles are left to k_bulk. */
+ * steps that do not report. dphi_in: delta-phi of the candidate cells (k_dphi). codes_out: phase codes of the new
+ * state (pure_phase of every written cell, halo images included). flags_in (may be NULL: no tile is deep) and
+ * flags_out: tile flags of the previous and of this step; deep tiles are left to k_bulk. */
 template <int N, int NC, typename R, bool P2, bool STATS>
 __global__ __launch_bounds__(256) void k_march(const Model<R> *__restrict__ mp, const R *__restrict__ phi,
-                                               const R *__restrict__ mu, R *__restrict__ phi_o,
-                                               R *__restrict__ mu_o, Maps maps, double *__restrict__ partials,
-                                               int64_t pstride, const int *__restrict__ flags_in,
-                                               int *__restrict__ flags_out, int64_t fs, int64_t ps, int64_t mx,
-                                               int64_t my, int seg_len, int nstrips, int nsegs, int nwaves) {
+                                               const R *__restrict__ mu, const R *__restrict__ dphi_in,
+                                               R *__restrict__ phi_o, R *__restrict__ mu_o,
+                                               int8_t *__restrict__ codes_out, Maps maps,
+                                               double *__restrict__ partials, int64_t pstride,
+                                               const int *__restrict__ flags_in, int *__restrict__ flags_out,
+                                               int64_t fs, int64_t ps, int64_t mx, int64_t my, int seg_len,
+                                               int nstrips, int nsegs, int nwaves) {
   using Vw = View<R, P2>;
   constexpr int NS = 3 * N + 3 * NC + 1;
   const Model<R> &m = *mp;
@@ -421,7 +252,7 @@ __global__ __launch_bounds__(256) void k_march(const Model<R> *__restrict__ mp, 
   const int64_t ry = my + 6;
 
   constexpr int NSA = STATS ? NS : 1;
-  double st[NSA];
+  R st[NSA];
   if (STATS) stats_init<N, NC>(st);
 
   const int strip = w % nstrips, seg = w / nstrips;
@@ -439,12 +270,18 @@ __global__ __launch_bounds__(256) void k_march(const Model<R> *__restrict__ mp, 
     row_halo(A0);
     row_load(A1, phi, mu, fs, (xa - 1) * ps + yl + kYOff);
     row_halo(A1);
-    /* software pipeline: the raw row x+2 is in flight while step x computes */
-    R nph[N], nmu[NC];
+    /* software pipeline: rows x+1 .. x+kMarchDepth are in flight while step x computes (the row loads are the
+     * only dependence of a step on memory, and one step takes less time than an HBM round trip) */
+    constexpr int D = kMarchDepth<R>;
+    R qph[D][N], qmu[D][NC];
 #pragma unroll
-    for (int a = 0; a < N; a++) nph[a] = phi[a * fs + xa * ps + yl + kYOff];
+    for (int d = 0; d < D; d++)
+      if (xa + d <= xb + 2) {
 #pragma unroll
-    for (int k = 0; k < NC; k++) nmu[k] = mu[k * fs + xa * ps + yl + kYOff];
+        for (int a = 0; a < N; a++) qph[d][a] = phi[a * fs + (xa + d) * ps + yl + kYOff];
+#pragma unroll
+        for (int k = 0; k < NC; k++) qmu[d][k] = mu[k * fs + (xa + d) * ps + yl + kYOff];
+      }
 
     R dph_prev[N], cc_prev[N][NC], M_prev[NC][NC], jx_prev[NC], jy_prev[NC], Fx_prev[NC][NC];
 #pragma unroll
@@ -452,8 +289,6 @@ __global__ __launch_bounds__(256) void k_march(const Model<R> *__restrict__ mp, 
 #pragma unroll
     for (int k = 0; k < NC; k++) jx_prev[k] = jy_prev[k] = R(0);
     bool cc_valid = false; /* cc_prev holds c_mu of row x-1 (the pure-phase steps do not form it) */
-    FaceAn<N, R> fb;
-    unsigned back_mask = 0; /* pairs whose entries of fb hold the face between rows x-1 and x */
     int oc = -2; /* purity of this lane's output cells: -2 none yet, b all purely phase b, -1 otherwise */
 
     /* writes the updated row x-1 (and its halo images) and accumulates the statistics and the purity */
@@ -466,43 +301,36 @@ __global__ __launch_bounds__(256) void k_march(const Model<R> *__restrict__ mp, 
       for (int a = 0; a < N; a++) phi_o[a * fs + c] = pn[a];
 #pragma unroll
       for (int k = 0; k < NC; k++) mu_o[k * fs + c] = munew[k];
+      const int8_t code = (int8_t)pc;
+      codes_out[c] = code;
       if (img_y || xo < 7 || xo > mx - 2) {
+        write_images(codes_out, 1, &code, 0, maps, xo, y, 0, ps);
         write_images(phi_o, N, pn, 0, maps, xo, y, fs, ps);
         write_images(mu_o, NC, munew, 1, maps, xo, y, fs, ps);
       }
-      if (STATS) {
-#pragma unroll
-        for (int a = 0; a < N; a++) {
-          const double pv = (double)pn[a], dv = (double)dph_o[a];
-          st[a] = fmax(st[a], pv);
-          st[N + a] = fmin(st[N + a], pv);
-          st[2 * N + 2 * NC + a] += dv * dv;
-          if (!isfinite(pv)) st[NSA - 1] += 1.0;
-        }
-#pragma unroll
-        for (int k = 0; k < NC; k++) {
-          const double mv = (double)munew[k], dv = (double)deltamu[k];
-          st[2 * N + k] = fmax(st[2 * N + k], mv);
-          st[2 * N + NC + k] = fmin(st[2 * N + NC + k], mv);
-          st[3 * N + 2 * NC + k] += dv * dv;
-          if (!isfinite(mv)) st[NSA - 1] += 1.0;
-        }
-      }
+      if (STATS) stats_add<N, NC>(st, pn, dph_o, munew, deltamu);
     };
 
     for (int64_t x = xa - 1; x <= xb + 1; x++) {
 #pragma unroll
-      for (int a = 0; a < N; a++) A2.ph[a] = nph[a];
+      for (int a = 0; a < N; a++) A2.ph[a] = qph[0][a];
 #pragma unroll
-      for (int k = 0; k < NC; k++) A2.mu[k] = nmu[k];
-      row_halo(A2);
-      if (x + 2 <= xb + 2) {
-        const int64_t ni = (x + 2) * ps + yl + kYOff;
+      for (int k = 0; k < NC; k++) A2.mu[k] = qmu[0][k];
 #pragma unroll
-        for (int a = 0; a < N; a++) nph[a] = phi[a * fs + ni];
+      for (int d = 0; d + 1 < D; d++) {
 #pragma unroll
-        for (int k = 0; k < NC; k++) nmu[k] = mu[k * fs + ni];
+        for (int a = 0; a < N; a++) qph[d][a] = qph[d + 1][a];
+#pragma unroll
+        for (int k = 0; k < NC; k++) qmu[d][k] = qmu[d + 1][k];
       }
+      if (x + 1 + D <= xb + 2) {
+        const int64_t ni = (x + 1 + D) * ps + yl + kYOff;
+#pragma unroll
+        for (int a = 0; a < N; a++) qph[D - 1][a] = phi[a * fs + ni];
+#pragma unroll
+        for (int k = 0; k < NC; k++) qmu[D - 1][k] = mu[k * fs + ni];
+      }
+      row_halo(A2);
 
       /* Pure-phase step: every lane of rows x-1, x, x+1 is purely the same phase b, and row x-1 carries no
        * delta-phi and no anti-trapping current. Then delta-phi of row x and every anti-trapping current of the
@@ -515,7 +343,7 @@ __global__ __launch_bounds__(256) void k_march(const Model<R> *__restrict__ mp, 
         for (int a = 0; a < N; a++) lp = lp && (dph_prev[a] == R(0.0));
 #pragma unroll
         for (int k = 0; k < NC; k++) lp = lp && (jx_prev[k] == R(0.0)) && (jy_prev[k] == R(0.0));
-        const int b = __shfl(A1.pure, 0, 64);
+        const int b = __builtin_amdgcn_readfirstlane(A1.pure); /* every lane is active here */
         if (__all(lp && A1.pure == b)) {
           R Fx0[NC][NC];
           if (x >= xa) {
@@ -585,7 +413,6 @@ __global__ __launch_bounds__(256) void k_march(const Model<R> *__restrict__ mp, 
             }
           }
           cc_valid = false;
-          back_mask = 0;
           A0 = A1;
           A1 = A2;
           continue;
@@ -599,66 +426,31 @@ __global__ __launch_bounds__(256) void k_march(const Model<R> *__restrict__ mp, 
       for (int a = 0; a < N; a++)
         if (__any(A0.ph[a] != R(0.0) || A1.ph[a] != R(0.0) || A2.ph[a] != R(0.0))) phmask |= 1u << a;
 
-      /* face gradients / averages and centred gradients (calculate_gradients_phasefield_2D) */
-      R gx0[N], gx1[N], sx0[N], sx1[N], cy0[N], cy1[N], cy2[N], cxc[N], cxd[N], gy1[N], gyu[N], sy1[N], syu[N];
-      R divphi[N];
-      bool lane_busy = false;
+      /* face gradients of row x used by the currents (calculate_gradients_phasefield_2D) */
+      R gx0[N], gx1[N], gy1[N];
 #pragma unroll
       for (int a = 0; a < N; a++) {
         gx0[a] = divx<Vw>(m, A1.ph[a] - A0.ph[a]);
         gx1[a] = divx<Vw>(m, A2.ph[a] - A1.ph[a]);
-        sx0[a] = R(0.5) * (A1.ph[a] + A0.ph[a]);
-        sx1[a] = R(0.5) * (A2.ph[a] + A1.ph[a]);
         gy1[a] = divy<Vw>(m, A1.phd[a] - A1.ph[a]);
-        gyu[a] = divy<Vw>(m, A1.ph[a] - A1.phu[a]);
-        divphi[a] = divx<Vw>(m, (gx1[a] - gx0[a]));
-        divphi[a] += divy<Vw>(m, (gy1[a] - gyu[a]));
-        lane_busy = lane_busy || (fabs(divphi[a]) > R(0.0)) || (A1.ph[a] < R(0.0));
       }
       R cc1[N][NC];
       cmu_all<N, NC>(m, A1.mu, cc1);
+      /* delta-phi of row x: k_dphi wrote it for every candidate cell (a cell whose 5-point neighbourhood is not
+       * purely one phase); every other cell has delta-phi exactly 0. Lanes 1..62 are the ones whose delta-phi
+       * the output lanes read. */
       R dph[N];
 #pragma unroll
       for (int a = 0; a < N; a++) dph[a] = R(0.0);
-      if (__any(lane_busy)) {
+      {
+        const int up = lane_up(A1.pure), dn = lane_dn(A1.pure);
+        const bool cand = lane >= 1 && lane <= 62 && y <= my + 3 &&
+                          (A1.pure < 0 || A0.pure != A1.pure || A2.pure != A1.pure || up != A1.pure || dn != A1.pure);
+        if (cand) {
+          const int64_t c = x * ps + y + kYOff;
 #pragma unroll
-        for (int a = 0; a < N; a++) {
-          cy0[a] = div2y<Vw>(m, A0.phd[a] - A0.phu[a]);
-          cy1[a] = div2y<Vw>(m, A1.phd[a] - A1.phu[a]);
-          cy2[a] = div2y<Vw>(m, A2.phd[a] - A2.phu[a]);
-          cxc[a] = R(0.5) * (gx1[a] + gx0[a]);
-          cxd[a] = R(0.5) * (divx<Vw>(m, A2.phd[a] - A1.phd[a]) + divx<Vw>(m, A1.phd[a] - A0.phd[a]));
-          sy1[a] = R(0.5) * (A1.phd[a] + A1.ph[a]);
-          syu[a] = R(0.5) * (A1.ph[a] + A1.phu[a]);
+          for (int a = 0; a < N; a++) dph[a] = dphi_in[a * fs + c];
         }
-        /* Phase pairs with both phases present in rows x-1..x+1 of the wavefront. For any other pair one
-         * phase is zero on the whole stencil: its q vanishes, and its terms add signed zeros to the sums of the
-         * active phases (which start at +0), so skipping them leaves every result unchanged. */
-        unsigned need = 0;
-#pragma unroll
-        for (int a = 0; a < N; a++)
-#pragma unroll
-          for (int b = a + 1; b < N; b++)
-            if (((phmask >> a) & 1u) && ((phmask >> b) & 1u)) need |= 1u << Pairs<N>::idx(a, b);
-        face_x<N>(m, sx0, gx0, cy0, cy1, fb, need & ~back_mask);
-        FaceAn<N, R> ff, fr, fl;
-        face_x<N>(m, sx1, gx1, cy1, cy2, ff, need);
-        face_y<N>(m, sy1, gy1, cxc, cxd, fr, need);
-        face_shuffle_up<N>(fr, fl, need);
-        R psi[N];
-#pragma unroll
-        for (int b = 0; b < N; b++) {
-          R p = R(0.0);
-#pragma unroll
-          for (int k = 0; k < NC; k++) p -= A1.mu[k] * cc1[b][k];
-          p += free_energy<NC>(m, cc1[b], b);
-          psi[b] = p;
-        }
-        dphi_cell<N, NC, Vw>(m, A1.ph, divphi, psi, cxc, cy1, ff, fb, fr, fl, sx1, sx0, sy1, syu, dph, need);
-        fb = ff;
-        back_mask = need;
-      } else {
-        back_mask = 0;
       }
 
       /* anti-trapping currents: right y-face of row x, x-face between x-1 and x */
@@ -892,7 +684,8 @@ constexpr int kBulkDepth = 4;
 template <int N, int NC, typename R, bool P2, bool STATS>
 __global__ __launch_bounds__(256) void k_bulk(const Model<R> *__restrict__ mp, const R *__restrict__ mu,
                                               R *__restrict__ phi_o, R *__restrict__ mu_o,
-                                              double *__restrict__ partials, int64_t pstride,
+                                              int8_t *__restrict__ codes_out, double *__restrict__ partials,
+                                              int64_t pstride,
                                               const int *__restrict__ flags_in, int *__restrict__ flags_out,
                                               int64_t fs, int64_t ps, int64_t mx, int seg_len, int nstrips, int nsegs,
                                               int nwaves) {
@@ -902,7 +695,7 @@ __global__ __launch_bounds__(256) void k_bulk(const Model<R> *__restrict__ mp, c
   const Model<R> &m = *mp;
   const int lane = threadIdx.x & 63;
   const int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  double st[NSA];
+  R st[NSA];
   if (STATS) stats_init<N, NC>(st);
   const int strip = w % nstrips, seg = w / nstrips;
   const int b = w < nwaves ? deep_phase(flags_in, strip, seg, nstrips, nsegs) : -1;
@@ -1004,20 +797,10 @@ __global__ __launch_bounds__(256) void k_bulk(const Model<R> *__restrict__ mp, c
         for (int a = 0; a < N; a++) phi_o[a * fs + c] = pe[a];
 #pragma unroll
         for (int k = 0; k < NC; k++) mu_o[k * fs + c] = munew[k];
+        codes_out[c] = (int8_t)b;
         if (STATS) {
-#pragma unroll
-          for (int a = 0; a < N; a++) {
-            st[a] = fmax(st[a], (double)pe[a]);
-            st[N + a] = fmin(st[N + a], (double)pe[a]);
-          }
-#pragma unroll
-          for (int k = 0; k < NC; k++) {
-            const double mv = (double)munew[k], dv = (double)deltamu[k];
-            st[2 * N + k] = fmax(st[2 * N + k], mv);
-            st[2 * N + NC + k] = fmin(st[2 * N + NC + k], mv);
-            st[3 * N + 2 * NC + k] += dv * dv;
-            if (!isfinite(mv)) st[NSA - 1] += 1.0;
-          }
+          const R z[N] = {};
+          stats_add<N, NC>(st, pe, z, munew, deltamu);
         }
       }
 #pragma unroll
@@ -1034,4 +817,69 @@ __global__ __launch_bounds__(256) void k_bulk(const Model<R> *__restrict__ mp, c
   stats_partials<N, NC>(st, partials, pstride, w, nwaves, lane);
 }
 
+
+/* Phase codes of a state: codes[cell] = pure_phase of its phi (b when purely phase b, -1 otherwise), over the
+ * whole allocated grid (halo images included); one thread per cell. The step kernels keep the codes of the state
+ * they write; this kernel forms them after an upload or a step of the two-pass path. */
+template <int N, typename R>
+__global__ __launch_bounds__(256) void k_codes(const R *__restrict__ phi, int8_t *__restrict__ codes, int64_t fs,
+                                               int64_t ps, int64_t rx, int64_t ry) {
+  const int64_t y = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
+  const int64_t x = (int64_t)blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x >= rx || y >= ry) return;
+  const int64_t c = x * ps + y + kYOff;
+  R ph[N];
+#pragma unroll
+  for (int a = 0; a < N; a++) ph[a] = phi[a * fs + c];
+  codes[c] = (int8_t)pure_phase<N>(ph);
+}
+
+/* Delta-phi of the candidate cells. A cell whose 5-point neighbourhood is purely one phase has every Laplacian
+ * zero and phi in {0, 1}, so its delta-phi is exactly 0 (no active phase, nothing for the projection to do);
+ * every other cell of x, y in [2, mesh+3] is a candidate. One wavefront per tile of 64 y-cells by kDTileRows
+ * x-rows: it finds the candidates from the phase codes, compacts them in LDS (ballot and prefix count, in row
+ * order), and evaluates phi_delta (the per-cell form of calculate_divergence_phasefield_2D and the simplex
+ * projection) for 64 candidates at a time, so every lane works on an interface cell. */
+constexpr int kDTileRows = 16;
+
+template <int N, int NC, typename R, bool P2>
+__global__ __launch_bounds__(256) void k_dphi(const Model<R> *__restrict__ mp, const R *__restrict__ phi,
+                                              const R *__restrict__ mu, const int8_t *__restrict__ codes,
+                                              R *__restrict__ dphi, int64_t fs, int64_t ps, int64_t mx, int64_t my,
+                                              int nty, int ntiles) {
+  __shared__ uint16_t list[4][kDTileRows * 64];
+  const Model<R> &m = *mp;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int w = blockIdx.x * 4 + wv;
+  int count = 0;
+  const int ty = w % nty, tx = w / nty;
+  const int64_t y = 2 + (int64_t)ty * 64 + lane;
+  const int64_t x0 = 2 + (int64_t)tx * kDTileRows;
+  if (w < ntiles) {
+    const int64_t x1 = (x0 + kDTileRows - 1 < mx + 3) ? x0 + kDTileRows - 1 : mx + 3;
+    const bool ylive = y <= my + 3;
+    for (int64_t x = x0; x <= x1; x++) {
+      bool cand = false;
+      if (ylive) {
+        const int64_t c = x * ps + y + kYOff;
+        const int8_t k0 = codes[c];
+        cand = k0 < 0 || codes[c - ps] != k0 || codes[c + ps] != k0 || codes[c - 1] != k0 || codes[c + 1] != k0;
+      }
+      const uint64_t mask = __ballot(cand);
+      const int pre = __popcll(mask & ((1ull << lane) - 1ull));
+      if (cand) list[wv][count + pre] = (uint16_t)((x - x0) * 64 + lane);
+      count += __popcll(mask);
+    }
+  }
+  __syncthreads();
+  for (int i = lane; i < count; i += 64) {
+    const int r = list[wv][i];
+    const int64_t c = (x0 + r / 64) * ps + (2 + (int64_t)ty * 64 + r % 64) + kYOff;
+    View<R, P2> v{phi, mu, nullptr, fs, ps, c};
+    R d[N];
+    phi_delta<N, NC>(m, v, false, d);
+#pragma unroll
+    for (int a = 0; a < N; a++) dphi[a * fs + c] = d[a];
+  }
+}
 }  // namespace gpk
