@@ -37,20 +37,45 @@ __device__ __forceinline__ void write_images(R *__restrict__ base, int ncomp, co
     }
 }
 
-template <typename T>
-__device__ __forceinline__ T wave_max(T v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-  return v;
+/* Wavefront reductions of a double over the 64 lanes, returned to every lane. DPP steps keep the data in the
+ * VALU (no LDS round trips): quad_perm [1,0,3,2] and [2,3,0,1], row_half_mirror and row_mirror reduce each
+ * 16-lane row; row_bcast:15 (rows 1, 3) and row_bcast:31 (rows 2, 3) combine the rows, leaving the result in
+ * lane 63. Lanes that a masked step does not write receive the operation's identity. */
+namespace dpp {
+constexpr int kQuad1032 = 0xB1, kQuad2301 = 0x4E, kRowHalfMirror = 0x141, kRowMirror = 0x140, kRowBcast15 = 0x142,
+              kRowBcast31 = 0x143;
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double mov(double v, double identity) {
+  const long long b = __double_as_longlong(v), id = __double_as_longlong(identity);
+  const int lo = __builtin_amdgcn_update_dpp((int)(id & 0xffffffffll), (int)(b & 0xffffffffll), CTRL, ROWMASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(id >> 32), (int)(b >> 32), CTRL, ROWMASK, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
-template <typename T>
-__device__ __forceinline__ T wave_min(T v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
-  return v;
+__device__ __forceinline__ double lane63(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), 63), hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
+template <class Op>
+__device__ __forceinline__ double reduce(double v, double identity, Op op) {
+  v = op(v, mov<kQuad1032, 0xf>(v, identity));
+  v = op(v, mov<kQuad2301, 0xf>(v, identity));
+  v = op(v, mov<kRowHalfMirror, 0xf>(v, identity));
+  v = op(v, mov<kRowMirror, 0xf>(v, identity));
+  v = op(v, mov<kRowBcast15, 0xa>(v, identity));
+  v = op(v, mov<kRowBcast31, 0xc>(v, identity));
+  return lane63(v);
+}
+}  // namespace dpp
+
+__device__ __forceinline__ double wave_max(double v) {
+  return dpp::reduce(v, -INFINITY, [](double a, double b) { return fmax(a, b); });
+}
+__device__ __forceinline__ double wave_min(double v) {
+  return dpp::reduce(v, INFINITY, [](double a, double b) { return fmin(a, b); });
+}
+__device__ __forceinline__ double wave_sum(double v) {
+  return dpp::reduce(v, 0.0, [](double a, double b) { return a + b; });
 }
 
 }  // namespace gpk
