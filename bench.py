@@ -98,6 +98,42 @@ def run_mode(case, precision: int, steps: int, warmup: int):
     }
 
 
+def adjoint_ratio(case, steps: int):
+    """Time of the FP64 reverse sweep against the forward, on the bench workload: `steps` plain FP64 steps, then
+    the same steps recorded for the adjoint (checkpoints every about sqrt(steps)) and the reverse sweep for a
+    random linear loss of the final state (segment recompute, transposed step, parameter gradient)."""
+    import numpy as np
+    from microsim_amd import _abi
+    from microsim_amd.solver import GPSolver
+    rng = np.random.default_rng(5)
+    wphi, wmu = rng.standard_normal(case.phi.shape), rng.standard_normal(case.mu.shape)
+    with GPSolver(case.params, _abi.GP_FP64) as s:
+        s.upload(case.phi, case.mu)
+        s.step(2)
+        s.synchronize()
+        s.upload(case.phi, case.mu)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.step(steps)
+        s.synchronize()
+        fwd = time.perf_counter() - t0
+        s.upload(case.phi, case.mu)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.adjoint_forward(steps)
+        s.synchronize()
+        rec = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        s.adjoint_backward(wphi, wmu)
+        bwd = time.perf_counter() - t0
+    return {"steps": steps, "forward_ms_per_step": round(fwd * 1e3 / steps, 4),
+            "recorded_forward_ms_per_step": round(rec * 1e3 / steps, 4),
+            "reverse_ms_per_step": round(bwd * 1e3 / steps, 4),
+            "reverse_over_forward": round(bwd / fwd, 3),
+            "note": "reverse includes the segment recompute and the host copies of the loss gradient in and the "
+                    "state and parameter gradients out"}
+
+
 def copy_bandwidth(nbytes: int = 1 << 31, reps: int = 10):
     """Streaming-copy reference on the same device: y = -x over nbytes (one elementwise kernel reads nbytes and
     writes nbytes), mean over reps. Reported next to the vendor peak."""
@@ -223,6 +259,7 @@ def main():
     ap.add_argument("--seeds", type=int, default=1024)
     ap.add_argument("--modes", default="fp32,fp64")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--adjoint-steps", type=int, default=20, help="steps of the adjoint timing (0: skip)")
     ap.add_argument("--error-steps", type=int, default=10000,
                     help="steps of the FP32-vs-FP64 error study at the bench mesh and at 1024^2 (0: skip)")
     a = ap.parse_args()
@@ -254,6 +291,9 @@ def main():
     }
     if "fp32" in res and "fp64" in res:
         out["fp32_over_fp64"] = res["fp32"]["value"] / res["fp64"]["value"]
+    if a.adjoint_steps > 0:
+        out["adjoint"] = adjoint_ratio(case, a.adjoint_steps)
+        log("adjoint", json.dumps(out["adjoint"]))
     if a.error_steps > 0:
         errs = [precision_errors(case, a.error_steps)]
         if a.mesh != 1024:

@@ -142,6 +142,10 @@ def _declare_hip(lib):
     lib.gp_profile_read.argtypes = [h, _dp, C.POINTER(C.c_int64)]
     lib.gp_kernel_name.argtypes = [C.c_int]
     lib.gp_kernel_name.restype = C.c_char_p
+    lib.gp_theta_count.argtypes = [h]
+    lib.gp_theta_values.argtypes = [h, _dp]
+    lib.gp_adjoint_forward.argtypes = [h, C.c_int64, C.c_int64]
+    lib.gp_adjoint_backward.argtypes = [h, _dp, _dp, _dp, _dp, _dp]
 
 
 _host = None

@@ -30,6 +30,7 @@
 #include "gp_hip.h"
 #include "gp_march.h"
 #include "gp_model.h"
+#include "gp_adjoint.h"
 
 using namespace gpk;
 
@@ -71,6 +72,7 @@ struct gp_handle {
   Maps maps;
   bool pow2;
   bool profiling;
+  struct Adjoint *adj;      /* checkpoints and buffers of the adjoint (gp_adjoint_forward / _backward) */
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof;
   char err[512];
 };
@@ -350,7 +352,7 @@ static int launch_invariants(gp_handle *h) {
 
 /* ------------------------------------------------------------------------------------------------ dispatch */
 
-enum { MODE_SMOOTH = 0, MODE_TWOPASS = 1, MODE_MARCH = 2, MODE_DPHI_PREV = 3, MODE_MARCH_STATS = 4 };
+enum { MODE_SMOOTH = 0, MODE_TWOPASS = 1, MODE_MARCH = 2, MODE_DPHI_PREV = 3, MODE_MARCH_STATS = 4, MODE_DPHI_CUR = 5 };
 
 /* Tile grid of the single-pass step: 60-cell y-strips by x-segments of 32 rows (16 when that gives fewer than
  * 4096 tiles, to fill the device on small grids); GP_SEG_LEN overrides the segment length. Each segment
@@ -380,6 +382,12 @@ static int launch_step_p(gp_handle *h, int mode) {
   dim3 g1((unsigned)((my + 2 + kBX - 1) / kBX), (unsigned)((mx + 2 + kBY - 1) / kBY));
   dim3 g2((unsigned)((my + kBX - 1) / kBX), (unsigned)((mx + kBY - 1) / kBY));
   const bool smooth = mode == MODE_SMOOTH;
+  if (mode == MODE_DPHI_CUR) {
+    /* delta-phi of the current state (the adjoint's recompute) */
+    hipLaunchKernelGGL((k_phi<N, NC, R, false, P2>), g1, blk, 0, h->stream, m, phi, mu, dphi, h->fs, h->ps, mx, my);
+    HIPCHK(h, hipGetLastError());
+    return GP_OK;
+  }
   if (mode == MODE_DPHI_PREV) {
     /* delta-phi of the last step, recomputed from the previous state (still in the other buffer) */
     const R *pp = (const R *)h->phi[1 - h->cur], *mp = (const R *)h->mu[1 - h->cur];
@@ -589,6 +597,139 @@ static int validate(const gp_params *p, char *err) {
   return GP_OK;
 }
 
+/* ------------------------------------------------------------------------------------------------ adjoint */
+
+/* Device buffers of the adjoint: checkpoints every `every` steps, the recomputed states of one segment, the
+ * state adjoints of two consecutive steps, the delta-phi adjoint, the parameter gradient and the dual models of
+ * every direction chunk of stages Q and P. */
+struct Adjoint {
+  int64_t nsteps = 0, every = 0;
+  std::vector<double *> ckpt;
+  std::vector<double *> seg;
+  double *g[2] = {nullptr, nullptr};
+  double *gdphi = nullptr, *gth = nullptr;
+  void *qmodels = nullptr, *pmodels = nullptr;
+  int nq = 0, np = 0;
+  ~Adjoint() {
+    for (double *b : ckpt) (void)hipFree(b);
+    for (double *b : seg) (void)hipFree(b);
+    void *bufs[] = {g[0], g[1], gdphi, gth, qmodels, pmodels};
+    for (void *b : bufs)
+      if (b) (void)hipFree(b);
+  }
+};
+
+static size_t state_elems(const gp_handle *h) { return (size_t)(h->N + h->NC) * h->fs; }
+
+/* the current state into / from a state buffer laid out phi planes then mu planes */
+static int state_save(gp_handle *h, double *dst) {
+  const size_t pb = (size_t)h->N * h->fs * sizeof(double), mb = (size_t)h->NC * h->fs * sizeof(double);
+  HIPCHK(h, hipMemcpyAsync(dst, h->phi[h->cur], pb, hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(dst + (size_t)h->N * h->fs, h->mu[h->cur], mb, hipMemcpyDeviceToDevice, h->stream));
+  return GP_OK;
+}
+static int state_load(gp_handle *h, const double *src) {
+  const size_t pb = (size_t)h->N * h->fs * sizeof(double), mb = (size_t)h->NC * h->fs * sizeof(double);
+  HIPCHK(h, hipMemcpyAsync(h->phi[h->cur], src, pb, hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->mu[h->cur], src + (size_t)h->N * h->fs, mb, hipMemcpyDeviceToDevice, h->stream));
+  h->flags_valid = false;
+  h->codes_valid = false;
+  h->prev_valid = false;
+  return GP_OK;
+}
+
+/* halo-inclusive host planes (rows_x*rows_y, y fastest) <-> device planes, without boundary copies */
+static int planes_to_device(gp_handle *h, double *dst, const double *src, int ncomp) {
+  const int64_t rx = h->rx, ry = h->ry;
+  std::vector<double> stage((size_t)ncomp * h->fs, 0.0);
+  for (int c = 0; c < ncomp; c++)
+    for (int64_t x = 0; x < rx; x++)
+      for (int64_t y = 0; y < ry; y++) stage[(size_t)c * h->fs + x * h->ps + y + kYOff] = src[((size_t)c * rx + x) * ry + y];
+  HIPCHK(h, hipMemcpyAsync(dst, stage.data(), stage.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return GP_OK;
+}
+static int planes_to_host(gp_handle *h, double *dst, const double *src, int ncomp) {
+  const int64_t rx = h->rx, ry = h->ry;
+  std::vector<double> stage((size_t)ncomp * h->fs);
+  HIPCHK(h, hipMemcpyAsync(stage.data(), src, stage.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (int c = 0; c < ncomp; c++)
+    for (int64_t x = 0; x < rx; x++)
+      for (int64_t y = 0; y < ry; y++) dst[((size_t)c * rx + x) * ry + y] = stage[(size_t)c * h->fs + x * h->ps + y + kYOff];
+  return GP_OK;
+}
+
+/* the dual models of every chunk of stage Q (directions 9N + 5NC + 5N + theta) and stage P (9N + NC + theta) */
+static int adj_models(gp_handle *h) {
+  Adjoint *A = h->adj;
+  if (A->qmodels) return GP_OK;
+  const int N = h->N, NC = h->NC, nth = theta_count(N, NC);
+  const int dq = 9 * N + 5 * NC + 5 * N, dp = 9 * N + NC;
+  A->nq = (dq + nth + kAdjK - 1) / kAdjK;
+  A->np = (dp + nth + kAdjK - 1) / kAdjK;
+  using MD = Model<Dual<kAdjK>>;
+  std::vector<MD> q(A->nq), pm(A->np);
+  for (int i = 0; i < A->nq; i++) make_dual_model<kAdjK>(&h->p, dq, i * kAdjK, &q[i]);
+  for (int i = 0; i < A->np; i++) make_dual_model<kAdjK>(&h->p, dp, i * kAdjK, &pm[i]);
+  if (hipMalloc(&A->qmodels, sizeof(MD) * A->nq) != hipSuccess || hipMalloc(&A->pmodels, sizeof(MD) * A->np) != hipSuccess)
+    return fail(h, GP_E_OOM, "adjoint: dual models do not fit in device memory");
+  HIPCHK(h, hipMemcpyAsync(A->qmodels, q.data(), sizeof(MD) * A->nq, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(A->pmodels, pm.data(), sizeof(MD) * A->np, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return GP_OK;
+}
+
+/* one adjoint step: the current state is s_t (delta-phi of s_t in h->dphi); A->g[1] holds the adjoint of
+ * s_{t+1} and becomes, in A->g[0] (then swapped into g[1]), the adjoint of s_t */
+template <int N, int NC, bool P2>
+static int adj_step_t(gp_handle *h) {
+  Adjoint *A = h->adj;
+  const int64_t mx = h->p.mesh_x, my = h->p.mesh_y, fs = h->fs;
+  const double *phi = (const double *)h->phi[h->cur], *mu = (const double *)h->mu[h->cur];
+  const double *dphi = (const double *)h->dphi;
+  double *gn = A->g[1], *gc = A->g[0];
+  const dim3 blk(256);
+  const dim3 gall((unsigned)((h->ry + 63) / 64), (unsigned)((h->rx + 3) / 4));
+  hipLaunchKernelGGL(k_adj_bc, gall, blk, 0, h->stream, gn, N, h->maps, 0, fs, h->ps, h->rx, h->ry);
+  hipLaunchKernelGGL(k_adj_bc, gall, blk, 0, h->stream, gn + (size_t)N * fs, NC, h->maps, 1, fs, h->ps, h->rx, h->ry);
+  HIPCHK(h, hipMemsetAsync(gc, 0, state_elems(h) * sizeof(double), h->stream));
+  HIPCHK(h, hipMemsetAsync(A->gdphi, 0, (size_t)N * fs * sizeof(double), h->stream));
+  using MD = Model<Dual<kAdjK>>;
+  const dim3 gq((unsigned)((my + 63) / 64), (unsigned)((mx + 3) / 4));
+  for (int i = 0; i < A->nq; i++)
+    hipLaunchKernelGGL((k_adj_q<N, NC, P2>), gq, blk, 0, h->stream, (const MD *)A->qmodels + i, phi, mu, dphi, gn,
+                       gn + (size_t)N * fs, gc, gc + (size_t)N * fs, A->gdphi, A->gth, fs, h->ps, mx, my, i * kAdjK);
+  const dim3 gp((unsigned)((my + 2 + 63) / 64), (unsigned)((mx + 2 + 3) / 4));
+  for (int i = 0; i < A->np; i++)
+    hipLaunchKernelGGL((k_adj_p<N, NC, P2>), gp, blk, 0, h->stream, (const MD *)A->pmodels + i, phi, mu, A->gdphi, gc,
+                       gc + (size_t)N * fs, A->gth, fs, h->ps, mx, my, i * kAdjK);
+  HIPCHK(h, hipGetLastError());
+  std::swap(A->g[0], A->g[1]);
+  return GP_OK;
+}
+
+/* fold the halo adjoints of A->g[1] into their source cells */
+static int adj_fold(gp_handle *h) {
+  Adjoint *A = h->adj;
+  const dim3 gall((unsigned)((h->ry + 63) / 64), (unsigned)((h->rx + 3) / 4));
+  hipLaunchKernelGGL(k_adj_bc, gall, dim3(256), 0, h->stream, A->g[1], h->N, h->maps, 0, h->fs, h->ps, h->rx, h->ry);
+  hipLaunchKernelGGL(k_adj_bc, gall, dim3(256), 0, h->stream, A->g[1] + (size_t)h->N * h->fs, h->NC, h->maps, 1, h->fs,
+                     h->ps, h->rx, h->ry);
+  HIPCHK(h, hipGetLastError());
+  return GP_OK;
+}
+
+typedef int (*adj_fn)(gp_handle *);
+
+static adj_fn pick_adj(const gp_handle *h) {
+#define GP_CASE(a, b) \
+  if (h->N == a && h->NC == b) return h->pow2 ? adj_step_t<a, b, true> : adj_step_t<a, b, false>;
+  GP_CASE(2, 1) GP_CASE(3, 2) GP_CASE(3, 1) GP_CASE(2, 2) GP_CASE(4, 2) GP_CASE(4, 1) GP_CASE(3, 3)
+#undef GP_CASE
+  return nullptr;
+}
+
 /* ------------------------------------------------------------------------------------------------ C ABI */
 
 extern "C" {
@@ -702,6 +843,7 @@ void gp_destroy(gp_handle *h) {
     (void)hipEventDestroy(r.second.second);
   }
   h->prof.clear();
+  delete h->adj;
   void *bufs[] = {h->phi[0], h->phi[1], h->mu[0], h->mu[1], h->dphi, h->partials, h->stats_dev, h->model,
                   h->flags[0], h->flags[1], h->inv_dev, h->codes};
   for (void *b : bufs)
@@ -828,6 +970,97 @@ int gp_step(gp_handle *h, int64_t nsteps, gp_stats *stats) {
     if (rc) return rc;
     if (stats->nonfinite) return fail(h, GP_E_NAN, "%lld non-finite values after step %lld", (long long)stats->nonfinite, (long long)h->steps);
   }
+  return GP_OK;
+}
+
+int gp_theta_count(const gp_handle *h) { return h ? theta_count(h->N, h->NC) : GP_E_ARG; }
+
+int gp_theta_values(const gp_handle *h, double *theta) {
+  if (!h || !theta) return GP_E_ARG;
+  const gp_params *p = &h->p;
+  const int N = h->N, NC = h->NC, L = N - 1;
+  int t = 0;
+  for (int a = 0; a < N; a++)
+    for (int b = a + 1; b < N; b++) {
+      theta[t++] = p->dab[a * GP_MAXP + b];
+      theta[t++] = p->tau_ab[a * GP_MAXP + b];
+      theta[t++] = p->gamma[a * GP_MAXP + b];
+    }
+  for (int k = 0; k < NC; k++) theta[t++] = p->diffusivity[(L * GP_MAXNC + k) * GP_MAXNC + k];
+  theta[t++] = p->tau;
+  return GP_OK;
+}
+
+int gp_adjoint_forward(gp_handle *h, int64_t nsteps, int64_t every) {
+  if (!h || nsteps < 0) return h ? fail(h, GP_E_ARG, "bad nsteps") : GP_E_ARG;
+  if (h->precision != GP_FP64) return fail(h, GP_E_ARG, "the adjoint is built for the FP64 mode");
+  if (!pick_adj(h)) return fail(h, GP_E_ARG, "no adjoint instantiated for %d phases / %d components", h->N, h->NC);
+  if (!h->use_march) return fail(h, GP_E_ARG, "the adjoint records single-pass steps (unset GP_PATH)");
+  delete h->adj;
+  h->adj = new Adjoint();
+  Adjoint *A = h->adj;
+  if (every <= 0) every = (int64_t)ceil(sqrt((double)(nsteps > 0 ? nsteps : 1)));
+  A->nsteps = nsteps;
+  A->every = every;
+  const int64_t nck = (nsteps + every - 1) / every;
+  for (int64_t k = 0; k < nck; k++) {
+    double *b = nullptr;
+    if (hipMalloc((void **)&b, state_elems(h) * sizeof(double)) != hipSuccess)
+      return fail(h, GP_E_OOM, "adjoint: checkpoint %lld does not fit in device memory", (long long)k);
+    A->ckpt.push_back(b);
+    int rc = state_save(h, b);
+    if (!rc) rc = gp_step(h, std::min(every, nsteps - k * every), nullptr);
+    if (rc) return rc;
+  }
+  return GP_OK;
+}
+
+int gp_adjoint_backward(gp_handle *h, const double *gphi_T, const double *gmu_T, double *gphi_0, double *gmu_0,
+                        double *gtheta) {
+  if (!h || !gphi_T || !gmu_T || !gphi_0 || !gmu_0 || !gtheta) return h ? fail(h, GP_E_ARG, "null argument") : GP_E_ARG;
+  Adjoint *A = h->adj;
+  if (!A) return fail(h, GP_E_ARG, "gp_adjoint_forward must run first");
+  const int nth = theta_count(h->N, h->NC);
+  const size_t se = state_elems(h);
+  if (!A->g[0]) {
+    bool ok = hipMalloc((void **)&A->g[0], se * sizeof(double)) == hipSuccess &&
+              hipMalloc((void **)&A->g[1], se * sizeof(double)) == hipSuccess &&
+              hipMalloc((void **)&A->gdphi, (size_t)h->N * h->fs * sizeof(double)) == hipSuccess &&
+              hipMalloc((void **)&A->gth, sizeof(double) * nth) == hipSuccess;
+    for (int64_t j = 0; ok && j < std::min(A->every, A->nsteps); j++) {
+      double *b = nullptr;
+      ok = hipMalloc((void **)&b, se * sizeof(double)) == hipSuccess;
+      if (ok) A->seg.push_back(b);
+    }
+    if (!ok) return fail(h, GP_E_OOM, "adjoint: buffers do not fit in device memory");
+  }
+  int rc = adj_models(h);
+  if (rc) return rc;
+  HIPCHK(h, hipMemsetAsync(A->g[1], 0, se * sizeof(double), h->stream));
+  if ((rc = planes_to_device(h, A->g[1], gphi_T, h->N))) return rc;
+  if ((rc = planes_to_device(h, A->g[1] + (size_t)h->N * h->fs, gmu_T, h->NC))) return rc;
+  HIPCHK(h, hipMemsetAsync(A->gth, 0, sizeof(double) * nth, h->stream));
+  step_fn f = pick_any(h);
+  adj_fn fa = pick_adj(h);
+  for (int64_t k = (int64_t)A->ckpt.size() - 1; k >= 0; k--) {
+    const int64_t n = std::min(A->every, A->nsteps - k * A->every);
+    if ((rc = state_load(h, A->ckpt[k]))) return rc;
+    for (int64_t j = 0; j < n; j++) {
+      if ((rc = state_save(h, A->seg[j]))) return rc;
+      if (j + 1 < n && (rc = gp_step(h, 1, nullptr))) return rc;
+    }
+    for (int64_t j = n - 1; j >= 0; j--) {
+      if ((rc = state_load(h, A->seg[j]))) return rc;
+      if ((rc = f(h, MODE_DPHI_CUR))) return rc;
+      if ((rc = fa(h))) return rc;
+    }
+  }
+  if ((rc = adj_fold(h))) return rc;
+  if ((rc = planes_to_host(h, gphi_0, A->g[1], h->N))) return rc;
+  if ((rc = planes_to_host(h, gmu_0, A->g[1] + (size_t)h->N * h->fs, h->NC))) return rc;
+  HIPCHK(h, hipMemcpyAsync(gtheta, A->gth, sizeof(double) * nth, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->steps = 0; /* the handle holds the initial state again */
   return GP_OK;
 }
 

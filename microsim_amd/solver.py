@@ -95,6 +95,33 @@ class GPSolver:
         self._check(self.lib.gp_profile_read(self.h, avg, cnt))
         return {self.lib.gp_kernel_name(k).decode(): (avg[k], cnt[k]) for k in range(_abi.GP_NKERNELS) if cnt[k]}
 
+    # ---------------------------------------------------------------- adjoint (FP64 handles)
+    def theta(self) -> np.ndarray:
+        """Differentiable parameters (gp_hip.h GP_THETA layout): per phase pair a < b dab, tau_ab, Gamma; the
+        liquid's diagonal diffusivities; tau."""
+        n = self.lib.gp_theta_count(self.h)
+        th = np.empty(n)
+        self._check(self.lib.gp_theta_values(self.h, _abi.dptr(th)))
+        return th
+
+    def adjoint_forward(self, nsteps: int, every: int = 0) -> None:
+        """nsteps steps from the current state, keeping a checkpoint every `every` steps (0: about sqrt)."""
+        self._check(self.lib.gp_adjoint_forward(self.h, int(nsteps), int(every)))
+
+    def adjoint_backward(self, gphi: np.ndarray, gmu: np.ndarray) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """Gradients of a scalar loss with respect to the state uploaded before adjoint_forward and to theta,
+        given its gradients with respect to the final state (halo-inclusive arrays). The handle holds the initial
+        state afterwards."""
+        gphi = np.ascontiguousarray(gphi, dtype=np.float64)
+        gmu = np.ascontiguousarray(gmu, dtype=np.float64)
+        assert gphi.shape == (self.N, self.rx, self.ry) and gmu.shape == (self.NC, self.rx, self.ry)
+        g0 = np.empty_like(gphi)
+        m0 = np.empty_like(gmu)
+        th = np.empty(self.lib.gp_theta_count(self.h))
+        self._check(self.lib.gp_adjoint_backward(self.h, _abi.dptr(gphi), _abi.dptr(gmu), _abi.dptr(g0),
+                                                 _abi.dptr(m0), _abi.dptr(th)))
+        return g0, m0, th
+
     def close(self) -> None:
         if getattr(self, "h", None):
             self.lib.gp_destroy(self.h)

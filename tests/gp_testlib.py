@@ -44,6 +44,40 @@ def oracle():
     return _orc
 
 
+_model = None
+
+
+def model_lib():
+    """Host build (g++) of the kernels' per-cell arithmetic, tests/cpu_model/model_step.cpp: model_step (the step
+    in the kernels' decomposition) and model_jvp (its forward-mode directional derivative)."""
+    global _model
+    if _model is None:
+        out = os.path.join(tempfile.mkdtemp(prefix="gp_model_"), "libmodel_step.so")
+        subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+                        "-I" + os.path.join(REPO, "include"), "-I" + os.path.join(REPO, "microsim_amd", "csrc"),
+                        "-o", out, os.path.join(REPO, "tests", "cpu_model", "model_step.cpp")], check=True)
+        lib = C.CDLL(out)
+        lib.model_step.argtypes = [C.POINTER(_abi.GPParams), _abi._dp, _abi._dp, _abi._dp, C.c_int64, C.c_int,
+                                   C.c_int]
+        lib.model_jvp.argtypes = [C.POINTER(_abi.GPParams), _abi._dp, _abi._dp, _abi._dp, _abi._dp, _abi._dp,
+                                  _abi._dp, _abi._dp, C.c_int64, C.c_int]
+        _model = lib
+    return _model
+
+
+def model_jvp(params, phi, mu, vphi, vmu, vtheta, nsteps):
+    """(phi_T, mu_T, tangent of phi_T, tangent of mu_T) along (vphi, vmu, vtheta), host build, FP64."""
+    phi, mu = phi.copy(), mu.copy()
+    tphi, tmu = np.zeros_like(phi), np.zeros_like(mu)
+    vphi = np.ascontiguousarray(vphi, dtype=np.float64)
+    vmu = np.ascontiguousarray(vmu, dtype=np.float64)
+    vtheta = np.ascontiguousarray(vtheta, dtype=np.float64)
+    rc = model_lib().model_jvp(C.byref(params), _abi.dptr(phi), _abi.dptr(mu), _abi.dptr(vphi), _abi.dptr(vmu),
+                               _abi.dptr(vtheta), _abi.dptr(tphi), _abi.dptr(tmu), nsteps, 0)
+    assert rc == 0
+    return phi, mu, tphi, tmu
+
+
 def oracle_step(params, phi, mu, nsteps, dphi=None, stats=None, threads=0):
     rc = oracle().oracle_step(C.byref(params), _abi.dptr(phi), _abi.dptr(mu),
                               _abi.dptr(dphi) if dphi is not None else None, nsteps,

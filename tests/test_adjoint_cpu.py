@@ -1,0 +1,79 @@
+"""Forward-mode derivative of the step arithmetic on the host, the reference the device adjoint is checked against.
+
+model_jvp (tests/cpu_model/model_step.cpp) steps the kernels' per-cell functions in dual numbers. Here:
+  * its values are the FP64 oracle's, bit for bit (the dual arithmetic leaves the value path unchanged);
+  * its tangent matches central finite differences of the oracle over one step on every golden deck, and over
+    twenty steps on the decks whose trajectories are smooth at the finite-difference scale.
+Over many steps a trajectory that starts from a sharp interface is not smooth at that scale: cells of the
+spreading tail cross the anisotropy's |q|^2 > 1e-15 cut-off and the simplex projection's clamp, so central
+differences of a deck such as bin_well_rot wander by 1e-4 relative as the step size changes. Those decks are
+checked through the dot-product test of the device adjoint against this tangent (tests/test_adjoint_gpu.py),
+which holds whatever the branches.
+"""
+import numpy as np
+import pytest
+
+from gp_testlib import golden_cases, load_golden, model_jvp, oracle_step
+from microsim_amd import adjoint
+
+
+def directions(case, seed, phi_cells="interface"):
+    """A tangent (vphi, vmu, vtheta): mu on every interior cell, phi on the interface cells (every phase strictly
+    inside (0, 1)) or on every interior cell, theta relative to its values."""
+    rng = np.random.default_rng(seed)
+    inner = np.zeros(case.phi.shape[1:], dtype=bool)
+    inner[3:-3, 3:-3] = True
+    cells = inner
+    if phi_cells == "interface":
+        cells = inner & np.all((case.phi > 1e-3) & (case.phi < 1 - 1e-3), axis=0)
+    vphi = rng.standard_normal(case.phi.shape) * cells
+    vmu = rng.standard_normal(case.mu.shape) * inner
+    th = adjoint.theta_from_params(case.params)
+    vth = rng.standard_normal(th.shape) * np.abs(th)
+    return vphi, vmu, vth
+
+
+def _oracle_run(case, params, phi, mu, n):
+    phi, mu = phi.copy(), mu.copy()
+    oracle_step(params, phi, mu, n)
+    return phi, mu
+
+
+def _fd(case, vphi, vmu, vth, n, h):
+    th = adjoint.theta_from_params(case.params)
+    out = []
+    for s in (h, -h):
+        p = adjoint.params_with_theta(case.params, th + s * vth)
+        out.append(_oracle_run(case, p, case.phi + s * vphi, case.mu + s * vmu, n))
+    return [(a - b) / (2 * h) for a, b in zip(out[0], out[1])]
+
+
+@pytest.mark.parametrize("name", golden_cases())
+def test_jvp_values_are_the_oracle(name):
+    case, _ = load_golden(name)
+    vphi, vmu, vth = directions(case, 1, phi_cells="all")
+    phi, mu, _, _ = model_jvp(case.params, case.phi, case.mu, vphi, vmu, vth, 12)
+    rphi, rmu = _oracle_run(case, case.params, case.phi, case.mu, 12)
+    assert np.array_equal(phi, rphi) and np.array_equal(mu, rmu)
+
+
+def _check_tangent(case, n, seed, h, rtol):
+    vphi, vmu, vth = directions(case, seed)
+    _, _, tphi, tmu = model_jvp(case.params, case.phi, case.mu, vphi, vmu, vth, n)
+    fphi, fmu = _fd(case, vphi, vmu, vth, n, h)
+    for t, f in ((tphi, fphi), (tmu, fmu)):
+        scale = np.abs(f).max()
+        assert scale > 0
+        assert np.abs(t - f).max() <= rtol * scale, (np.abs(t - f).max(), scale)
+
+
+@pytest.mark.parametrize("name", golden_cases())
+def test_jvp_one_step_matches_central_differences(name):
+    case, _ = load_golden(name)
+    _check_tangent(case, 1, 2, 1e-7, 1e-6)
+
+
+@pytest.mark.parametrize("name", [c for c in golden_cases() if c != "bin_well_rot"])
+def test_jvp_trajectory_matches_central_differences(name):
+    case, _ = load_golden(name)
+    _check_tangent(case, 20, 3, 1e-6, 1e-6)

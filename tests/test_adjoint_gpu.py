@@ -1,0 +1,205 @@
+"""Adjoint of the FP64 solver loop against forward-mode derivatives and central finite differences.
+
+The loss is a fixed random linear functional of the final state, L = <w_phi, phi_T> + <w_mu, mu_T> over every
+halo-inclusive entry (so the transposed boundary conditions at the final state are exercised too). The checks:
+  * dot-product test on every golden deck: <adjoint gradient, v> against <w, J v>, where J v is the host
+    forward-mode tangent of the same trajectory (tests/cpu_model/model_step.cpp, itself checked against finite
+    differences of the oracle in tests/test_adjoint_cpu.py), for a random v over every interior cell of phi and
+    mu and every parameter; and dL/dtheta component by component against the tangent along each parameter.
+    Both sides take the branches the values take, so these hold to rounding on any deck;
+  * dL/dtheta for every parameter, against central differences of the GPU forward with a relative step of 1e-6,
+    and the state gradient along a random direction (all interior mu cells and the interface cells of phi)
+    against the central difference of L along it, on the decks whose trajectories are smooth at that scale
+    (bin_well_rot is not: cells of its spreading tail cross the anisotropy cut-off and the projection's clamp);
+  * independence of the checkpoint interval (the reverse sweep recomputes segments; the gradients must agree to
+    rounding, which the atomic accumulation makes order-dependent);
+  * the forward of the differentiable trajectory equals plain stepping bit for bit.
+The decks are the golden ones (binary, ternary and four-phase; isotropic and anisotropic, rotated; obstacle and
+well potentials; periodic, Neumann and mixed sides; dilute and LU solves; undercooling).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from gp_testlib import golden_cases, load_golden, model_jvp
+from microsim_amd import _abi
+
+CASES = ["bin_tanh_restart", "bin_aniso_neu", "tern_aniso_per", "tern_c1_mini", "quad_k3"]
+NSTEPS = 20
+EVERY = 6
+RTOL = 1e-4
+
+
+def _say(msg):
+    print("[gpu] adjoint " + msg, flush=True)
+    sys.stdout.flush()
+
+
+def _weights(case, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal(case.phi.shape), rng.standard_normal(case.mu.shape)
+
+
+def _loss(params, phi0, mu0, wphi, wmu, nsteps=NSTEPS):
+    from microsim_amd.solver import GPSolver
+    with GPSolver(params, _abi.GP_FP64) as s:
+        s.upload(phi0, mu0)
+        s.step(nsteps)
+        phi, mu = s.download()
+    return float(np.sum(wphi * phi) + np.sum(wmu * mu))
+
+
+def _adjoint(params, phi0, mu0, wphi, wmu, nsteps=NSTEPS, every=EVERY):
+    from microsim_amd.solver import GPSolver
+    with GPSolver(params, _abi.GP_FP64) as s:
+        s.upload(phi0, mu0)
+        s.adjoint_forward(nsteps, every)
+        phiT, muT = s.download()
+        g0, m0, th = s.adjoint_backward(wphi, wmu)
+    return g0, m0, th, phiT, muT
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", golden_cases())
+def test_adjoint_dot_product_with_forward_mode(name):
+    from microsim_amd import adjoint
+    _say("dot product " + name)
+    case, _ = load_golden(name)
+    wphi, wmu = _weights(case, 21)
+    g0, m0, gth, phiT, muT = _adjoint(case.params, case.phi, case.mu, wphi, wmu)
+    rng = np.random.default_rng(22)
+    inner = np.zeros(case.phi.shape[1:], dtype=bool)
+    inner[3:-3, 3:-3] = True
+    vphi = rng.standard_normal(case.phi.shape) * inner
+    vmu = rng.standard_normal(case.mu.shape) * inner
+    th0 = adjoint.theta_from_params(case.params)
+    vth = rng.standard_normal(th0.shape) * np.abs(th0)
+    hphi, hmu, tphi, tmu = model_jvp(case.params, case.phi, case.mu, vphi, vmu, vth, NSTEPS)
+    assert np.array_equal(hphi, phiT) and np.array_equal(hmu, muT)
+    terms_adj = np.concatenate([(g0 * vphi).ravel(), (m0 * vmu).ravel(), gth * vth])
+    terms_fwd = np.concatenate([(wphi * tphi).ravel(), (wmu * tmu).ravel()])
+    adj, fwd = terms_adj.sum(), terms_fwd.sum()
+    scale = max(np.abs(terms_adj).sum(), np.abs(terms_fwd).sum())
+    print("  <g, v> % .15e  <w, Jv> % .15e  (scale %.3e)" % (adj, fwd, scale))
+    assert abs(adj - fwd) <= 1e-10 * scale
+    # every parameter on its own: the tangent along theta_i alone is dL/dtheta_i
+    names = adjoint.theta_names(case.N, case.NC + 1)
+    zphi, zmu = np.zeros_like(vphi), np.zeros_like(vmu)
+    for i in range(th0.size):
+        e = np.zeros_like(th0)
+        e[i] = 1.0
+        _, _, tphi, tmu = model_jvp(case.params, case.phi, case.mu, zphi, zmu, e, NSTEPS)
+        ref = np.sum(wphi * tphi) + np.sum(wmu * tmu)
+        s = np.abs(wphi * tphi).sum() + np.abs(wmu * tmu).sum()
+        assert abs(gth[i] - ref) <= 1e-10 * max(s, 1e-300), (names[i], gth[i], ref)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", CASES)
+def test_theta_gradient_matches_central_differences(name):
+    from microsim_amd import adjoint
+    _say("theta fd " + name)
+    case, _ = load_golden(name)
+    wphi, wmu = _weights(case, 11)
+    _, _, gth, _, _ = _adjoint(case.params, case.phi, case.mu, wphi, wmu)
+    th0 = adjoint.theta_from_params(case.params)
+    names = adjoint.theta_names(case.N, case.NC + 1)
+    assert gth.shape == th0.shape
+    fd = np.empty_like(th0)
+    for i in range(th0.size):
+        h = 1e-6 * abs(th0[i])
+        assert h > 0, names[i]
+        tp, tm = th0.copy(), th0.copy()
+        tp[i] += h
+        tm[i] -= h
+        lp = _loss(adjoint.params_with_theta(case.params, tp), case.phi, case.mu, wphi, wmu)
+        lm = _loss(adjoint.params_with_theta(case.params, tm), case.phi, case.mu, wphi, wmu)
+        fd[i] = (lp - lm) / ((th0[i] + h) - (th0[i] - h))
+    scale = np.abs(fd) * np.abs(th0)          # sensitivity of L to a relative change of each parameter
+    err = np.abs(gth - fd) * np.abs(th0)
+    for i in range(th0.size):
+        print("  %-14s adj % .10e  fd % .10e" % (names[i], gth[i], fd[i]))
+    bad = err > RTOL * np.maximum(scale, 1e-3 * scale.max())
+    assert not bad.any(), [(names[i], gth[i], fd[i]) for i in np.nonzero(bad)[0]]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", CASES)
+def test_state_gradient_matches_directional_difference(name):
+    _say("state fd " + name)
+    case, _ = load_golden(name)
+    wphi, wmu = _weights(case, 12)
+    g0, m0, _, _, _ = _adjoint(case.params, case.phi, case.mu, wphi, wmu)
+    rng = np.random.default_rng(13)
+    inner = np.zeros(case.phi.shape[1:], dtype=bool)
+    inner[3:-3, 3:-3] = True
+    # phi directions only where every phase is strictly inside (0, 1): the per-cell arithmetic branches on
+    # exactly pure values, where a perturbation changes the branch
+    iface = inner & np.all((case.phi > 1e-3) & (case.phi < 1 - 1e-3), axis=0)
+    vphi = rng.standard_normal(case.phi.shape) * iface
+    vmu = rng.standard_normal(case.mu.shape) * inner
+    adj = float(np.sum(g0 * vphi) + np.sum(m0 * vmu))
+    h = 1e-6
+    lp = _loss(case.params, case.phi + h * vphi, case.mu + h * vmu, wphi, wmu)
+    lm = _loss(case.params, case.phi - h * vphi, case.mu - h * vmu, wphi, wmu)
+    fd = (lp - lm) / (2 * h)
+    print("  cells phi %d mu %d: adj % .10e fd % .10e" % (iface.sum(), inner.sum(), adj, fd))
+    assert abs(adj - fd) <= RTOL * max(abs(fd), 1e-8)
+    # the halo entries of the initial state are overwritten from the interior by the upload: zero gradient
+    halo = ~inner
+    assert np.all(g0[:, halo] == 0) and np.all(m0[:, halo] == 0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["tern_aniso_per", "bin_aniso_neu"])
+def test_checkpoint_interval_does_not_change_gradients(name):
+    _say("checkpoint intervals " + name)
+    case, _ = load_golden(name)
+    wphi, wmu = _weights(case, 14)
+    ref = _adjoint(case.params, case.phi, case.mu, wphi, wmu, every=NSTEPS)
+    for every in (1, 0, 7):
+        got = _adjoint(case.params, case.phi, case.mu, wphi, wmu, every=every)
+        for a, b in zip(got[:3], ref[:3]):
+            tol = 1e-12 * np.abs(b).max()
+            assert np.abs(a - b).max() <= tol, every
+        assert np.array_equal(got[3], ref[3]) and np.array_equal(got[4], ref[4])
+
+
+@pytest.mark.gpu
+def test_trajectory_function_forward_and_backward():
+    import torch
+    from microsim_amd import adjoint
+    from microsim_amd.solver import GPSolver
+    _say("torch trajectory tern_c1_mini")
+    case, _ = load_golden("tern_c1_mini")
+    th = torch.tensor(adjoint.theta_from_params(case.params), requires_grad=True)
+    phi0 = torch.tensor(case.phi, requires_grad=True)
+    mu0 = torch.tensor(case.mu, requires_grad=True)
+    phi, mu = adjoint.trajectory(case.params, th, phi0, mu0, NSTEPS, every=5)
+    with GPSolver(case.params, _abi.GP_FP64) as s:
+        s.upload(case.phi, case.mu)
+        s.step(NSTEPS)
+        phi_ref, mu_ref = s.download()
+    assert np.array_equal(phi.detach().numpy(), phi_ref) and np.array_equal(mu.detach().numpy(), mu_ref)
+    wphi, wmu = _weights(case, 15)
+    loss = (phi * torch.from_numpy(wphi)).sum() + (mu * torch.from_numpy(wmu)).sum()
+    loss.backward()
+    g0, m0, gth, _, _ = _adjoint(case.params, case.phi, case.mu, wphi, wmu, every=5)
+    # a second reverse sweep accumulates in another atomic order: equal to rounding
+    for a, b in ((th.grad.numpy(), gth), (phi0.grad.numpy(), g0), (mu0.grad.numpy(), m0)):
+        assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
+
+
+def test_theta_round_trip_cpu():
+    from microsim_amd import adjoint
+    for name in CASES:
+        case, _ = load_golden(name)
+        th = adjoint.theta_from_params(case.params)
+        assert len(th) == len(adjoint.theta_names(case.N, case.NC + 1))
+        p2 = adjoint.params_with_theta(case.params, th)
+        assert bytes(p2) == bytes(case.params), name
+        p3 = adjoint.params_with_theta(case.params, th * 1.5)
+        assert np.array_equal(adjoint.theta_from_params(p3), th * 1.5)
+    with pytest.raises(ValueError):
+        adjoint.params_with_theta(case.params, th[:-1])

@@ -2,31 +2,12 @@
 the kernels' decomposition, against the FP64 oracle. Catches arithmetic mistakes in the kernel source on the
 CPU, before any GPU run; the GPU tests then only have to show the device reproduces its own host build."""
 import ctypes as C
-import os
-import subprocess
-import tempfile
 
 import numpy as np
 import pytest
 
-from gp_testlib import REPO, golden_cases, load_golden, oracle_smooth, oracle_step
+from gp_testlib import golden_cases, load_golden, model_lib, oracle_smooth, oracle_step
 from microsim_amd import _abi
-
-_lib = None
-
-
-def model_lib():
-    global _lib
-    if _lib is None:
-        out = os.path.join(tempfile.mkdtemp(prefix="gp_model_"), "libmodel_step.so")
-        subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
-                        "-I" + os.path.join(REPO, "include"), "-I" + os.path.join(REPO, "microsim_amd", "csrc"),
-                        "-o", out, os.path.join(REPO, "tests", "cpu_model", "model_step.cpp")], check=True)
-        lib = C.CDLL(out)
-        lib.model_step.argtypes = [C.POINTER(_abi.GPParams), _abi._dp, _abi._dp, _abi._dp, C.c_int64, C.c_int, C.c_int]
-        _lib = lib
-    return _lib
-
 
 @pytest.mark.parametrize("pow2", [0, 1])
 @pytest.mark.parametrize("name", golden_cases())
