@@ -125,27 +125,206 @@ __device__ __forceinline__ void theta_flush(const double *th, int nth, double *g
 template <int N, int NC>
 constexpr int kTheta = 3 * (N * (N - 1) / 2) + NC + 1;
 
-/* Stage Q transposed for directions [chunk0, chunk0 + K): one thread per interior cell (blocks of 64 x 4). */
+/* ---------------------------------------------------------------- stage Q of bulk cells, in closed form
+ *
+ * A "bulk" cell c of stage Q has phi_b(c) exactly 1 for some phase (mu_update takes its bulk branch, with the
+ * first such b) and no solid-phase delta-phi on itself or its four neighbours (each anti-trapping current of
+ * its faces returns +0 before any arithmetic, so carries no derivative). Its stage Q is then
+ *   phi_new = phi + dphi,   mu_new = Mu_b(c_b(mu) + dt divflux),
+ *   divflux_k = sum over faces f (neighbour e, spacing d_f) of sum_l M_f[k][l] (mu_l(e) - mu_l(c)) / d_f^2,
+ *   M_f = (M(c) + M(e)) / 2,   M(x)[k][l] = sum_a sum_q DC[a][k][l][q] phi_a(x),
+ * and its transpose is formed here in closed form (exact derivatives, rounded differently from the dual
+ * evaluation). With cbar_k(c) = sum_j gmu'_j dMu_j/dc_k and Fbar(c) = dt cbar(c) (zero for other cells), the
+ * adjoint of a cell n gathers, over its faces f with neighbour e:
+ *   gmu_l(n)  += sum_k (Fbar_k(e) - Fbar_k(n)) M_f[k][l] / d_f^2
+ *   gphi_a(n) += sum_k (Fbar_k(n) - Fbar_k(e)) / 2 sum_l DCs[a][k][l] (mu_l(e) - mu_l(n)) / d_f^2
+ * plus, when n itself is bulk, gmu_l(n) += sum_k cbar_k(n) cmu_b[k][l], gphi(n) += gphi'(n), gdphi(n) = gphi'(n),
+ * and the liquid diffusivities' gradient sum_k Fbar_k(n) sum_f (phi_L(n) + phi_L(e)) / 2 sum_l cmu_L[k][l]
+ * (mu_l(e) - mu_l(n)) / d_f^2 (DCs[a][k][l] = sum_q DC[a][k][l][q] = dM/dphi_a). */
+/* Append the cells of this wavefront with `flag` set to list (count: entries so far): one atomic per wavefront,
+ * issued by the first flagged lane. */
+__device__ __forceinline__ void list_append(bool flag, int32_t c, int32_t *list, int *count) {
+  const unsigned long long mask = __ballot(flag);
+  if (!mask) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((long long)mask) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(count, __popcll(mask));
+  base = __shfl(base, leader);
+  if (flag) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = c;
+}
+
+template <int N, int NC>
+__device__ __forceinline__ int bulk_q(const double *phi, const double *dphi, int64_t c, int64_t fs, int64_t ps) {
+  int b = -1;
+  for (int a = 0; a < N; a++)
+    if (b < 0 && phi[a * fs + c] == 1.0) b = a;
+  if (b < 0) return -1;
+  for (int a = 0; a < N - 1; a++) {
+    const double *d = dphi + a * fs + c;
+    if (d[0] != 0.0 || d[ps] != 0.0 || d[-ps] != 0.0 || d[1] != 0.0 || d[-1] != 0.0) return -1;
+  }
+  return b;
+}
+
+/* cbar of every cell of the allocated grid (zero outside the interior and for cells that are not bulk), and the
+ * list of the interior cells that are not bulk and have a nonzero incoming adjoint (stage Q by dual numbers) */
+template <int N, int NC>
+__global__ __launch_bounds__(256) void k_adj_cbar(const Model<double> *__restrict__ mp, const double *__restrict__ phi,
+                                                  const double *__restrict__ dphi, const double *__restrict__ gphi_n,
+                                                  const double *__restrict__ gmu_n, double *__restrict__ cbar,
+                                                  int32_t *__restrict__ qlist, int *qcount, int64_t fs, int64_t ps,
+                                                  int64_t mx, int64_t my) {
+  const int64_t y = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
+  const int64_t x = (int64_t)blockIdx.y * 4 + (threadIdx.x >> 6);
+  const bool valid = x < mx + 6 && y < my + 6;
+  const int64_t c = x * ps + y + kYOff;
+  const bool inner = valid && x >= 3 && x <= mx + 2 && y >= 3 && y <= my + 2;
+  const int b = inner ? bulk_q<N, NC>(phi, dphi, c, fs, ps) : -1;
+  bool dual = false;
+  if (valid) {
+    const Model<double> &m = *mp;
+    for (int k = 0; k < NC; k++) {
+      double s = 0.0;
+      if (b >= 0)
+        for (int j = 0; j < NC; j++) s += gmu_n[j * fs + c] * (j == k ? 2.0 * m.A[b][j][j] : m.A[b][j][k]);
+      cbar[k * fs + c] = s;
+    }
+    if (inner && b < 0) {
+      for (int a = 0; a < N; a++) dual = dual || gphi_n[a * fs + c] != 0.0;
+      for (int k = 0; k < NC; k++) dual = dual || gmu_n[k * fs + c] != 0.0;
+    }
+  }
+  list_append(dual, (int32_t)c, qlist, qcount);
+}
+
+/* the gather above for every cell n of [2, mesh+3]^2: plain stores into gphi / gmu (cleared before) and, for bulk
+ * n, into gdphi; the dual kernels then add the other cells' contributions */
+template <int N, int NC>
+__global__ __launch_bounds__(256) void k_adj_bulk(const Model<double> *__restrict__ mp, const double *__restrict__ phi,
+                                                  const double *__restrict__ mu, const double *__restrict__ dphi,
+                                                  const double *__restrict__ cbar, const double *__restrict__ gphi_n,
+                                                  double *__restrict__ gphi, double *__restrict__ gmu,
+                                                  double *__restrict__ gdphi, double *gtheta, int th0, int64_t fs,
+                                                  int64_t ps, int64_t mx, int64_t my) {
+  const Model<double> &m = *mp;
+  const int64_t y = 2 + (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
+  const int64_t x = 2 + (int64_t)blockIdx.y * 4 + (threadIdx.x >> 6);
+  constexpr int L = N - 1;
+  double th[NC];
+  for (int k = 0; k < NC; k++) th[k] = 0.0;
+  if (x <= mx + 3 && y <= my + 3) {
+    const int64_t c = x * ps + y + kYOff;
+    const bool inner = x >= 3 && x <= mx + 2 && y >= 3 && y <= my + 2;
+    const int b = inner ? bulk_q<N, NC>(phi, dphi, c, fs, ps) : -1;
+    double gph[N], gm[NC], cb[NC], mun[NC], phn[N], Mn[NC][NC];
+    for (int k = 0; k < NC; k++) {
+      cb[k] = cbar[k * fs + c];
+      mun[k] = mu[k * fs + c];
+      gm[k] = 0.0;
+    }
+    for (int a = 0; a < N; a++) {
+      phn[a] = phi[a * fs + c];
+      gph[a] = 0.0;
+    }
+    if (b >= 0) {
+      for (int a = 0; a < N; a++) {
+        gph[a] = gphi_n[a * fs + c];
+        gdphi[a * fs + c] = gph[a];
+      }
+      for (int l = 0; l < NC; l++)
+        for (int k = 0; k < NC; k++) gm[l] += cb[k] * m.cmu[b][k][l];
+    }
+    for (int k = 0; k < NC; k++)
+      for (int l = 0; l < NC; l++) {
+        double s = 0.0;
+        for (int a = 0; a < N; a++)
+          for (int q = 0; q < NC; q++) s += m.DC[a][k][l][q] * phn[a];
+        Mn[k][l] = s;
+      }
+    const double wx = m.inv_dx * m.inv_dx, wy = m.inv_dy * m.inv_dy;
+    const int64_t offs[4] = {ps, -ps, 1, -1};
+    for (int f = 0; f < 4; f++) {
+      const int64_t e = c + offs[f];
+      const double w = f < 2 ? wx : wy;
+      double dF[NC], due[NC];
+      bool nz = false;
+      for (int k = 0; k < NC; k++) {
+        dF[k] = m.dt * (cbar[k * fs + e] - cb[k]);
+        nz = nz || dF[k] != 0.0;
+      }
+      if (!nz) continue;
+      double phe[N];
+      for (int a = 0; a < N; a++) phe[a] = phi[a * fs + e];
+      for (int l = 0; l < NC; l++) due[l] = mu[l * fs + e] - mun[l];
+      for (int l = 0; l < NC; l++) {
+        double s = 0.0;
+        for (int k = 0; k < NC; k++) {
+          double Me = 0.0;
+          for (int a = 0; a < N; a++)
+            for (int q = 0; q < NC; q++) Me += m.DC[a][k][l][q] * phe[a];
+          s += dF[k] * 0.5 * (Mn[k][l] + Me);
+        }
+        gm[l] += w * s;
+      }
+      for (int a = 0; a < N; a++) {
+        double s = 0.0;
+        for (int k = 0; k < NC; k++) {
+          double t = 0.0;
+          for (int l = 0; l < NC; l++) {
+            double dcs = 0.0;
+            for (int q = 0; q < NC; q++) dcs += m.DC[a][k][l][q];
+            t += dcs * due[l];
+          }
+          s -= dF[k] * t;
+        }
+        gph[a] += 0.5 * w * s;
+      }
+      if (b >= 0)
+        for (int k = 0; k < NC; k++) {
+          double t = 0.0;
+          for (int l = 0; l < NC; l++) t += m.cmu[L][k][l] * due[l];
+          th[k] += m.dt * cb[k] * 0.5 * w * (phn[L] + phe[L]) * t;
+        }
+    }
+    for (int a = 0; a < N; a++) gphi[a * fs + c] = gph[a];
+    for (int k = 0; k < NC; k++) gmu[k * fs + c] = gm[k];
+  }
+  __shared__ double red[4][NC];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int k = 0; k < NC; k++) {
+    const double r = wave_sum(th[k]);
+    if (lane == 0) red[wv][k] = r;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int k = 0; k < NC; k++) {
+      const double s = red[0][k] + red[1][k] + red[2][k] + red[3][k];
+      if (s != 0.0) atomicAdd(&gtheta[th0 + k], s);
+    }
+}
+
+/* Stage Q transposed for directions [chunk0, chunk0 + K), for the listed cells (interior, not bulk, nonzero
+ * incoming adjoint; k_adj_bulk covers the bulk cells): a grid-stride loop over the list. */
 template <int N, int NC, bool P2>
 __global__ __launch_bounds__(256) void k_adj_q(const Model<Dual<kAdjK>> *__restrict__ mp, const double *__restrict__ phi,
                                                const double *__restrict__ mu, const double *__restrict__ dphi,
                                                const double *__restrict__ gphi_n, const double *__restrict__ gmu_n,
-                                               double *gphi, double *gmu, double *gdphi, double *gtheta, int64_t fs,
-                                               int64_t ps, int64_t mx, int64_t my, int chunk0) {
+                                               double *gphi, double *gmu, double *gdphi, double *gtheta,
+                                               const int32_t *__restrict__ list, const int *__restrict__ count,
+                                               int64_t fs, int64_t ps, int chunk0) {
   constexpr int K = kAdjK;
-  const int64_t y = 3 + (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
-  const int64_t x = 3 + (int64_t)blockIdx.y * 4 + (threadIdx.x >> 6);
   constexpr int NTH = kTheta<N, NC>;
   double th[NTH];
   for (int t = 0; t < NTH; t++) th[t] = 0.0;
-  if (x <= mx + 2 && y <= my + 2) {
-    const int64_t c = x * ps + y + kYOff;
+  const int n = *count;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int64_t c = list[i];
+    const int64_t y = (c - kYOff) % ps;
     double go[N + NC];
-    bool any = false;
     for (int a = 0; a < N; a++) go[a] = gphi_n[a * fs + c];
     for (int k = 0; k < NC; k++) go[N + k] = gmu_n[k * fs + c];
-    for (int o = 0; o < N + NC; o++) any = any || go[o] != 0.0;
-    if (any) {
+    {
       const Model<Dual<K>> &m = *mp;
       SeedView<N, NC, K, P2, 1> v{phi, mu, dphi, fs, ps, c, 0, 0, chunk0};
       Dual<K> jx0[NC], jxb[NC], jy0[NC], jyl[NC], d[N], pn[N], mn[NC], dm[NC];
@@ -166,25 +345,59 @@ __global__ __launch_bounds__(256) void k_adj_q(const Model<Dual<kAdjK>> *__restr
   theta_flush<NTH>(th, NTH, gtheta);
 }
 
-/* Stage P transposed: one thread per cell of [2, mesh+3]^2 with a nonzero delta-phi adjoint. */
+/* Stage P transposed runs on the cells of [2, mesh+3]^2 with a nonzero delta-phi adjoint whose 5-point
+ * neighbourhood is not purely one phase. (Where it is, every Laplacian of phi is exactly 0 and no phi is
+ * negative: phi_delta returns zeros untouched by the projection, with zero derivative.) */
+template <int N>
+__device__ __forceinline__ bool p_candidate(const double *phi, int64_t c, int64_t fs, int64_t ps) {
+  const int64_t o[5] = {0, ps, -ps, 1, -1};
+  int b0 = -2;
+  for (int i = 0; i < 5; i++) {
+    int code = -1, nz = 0;
+    for (int a = 0; a < N; a++) {
+      const double v = phi[a * fs + c + o[i]];
+      if (v == 1.0) code = a;
+      nz += (v != 0.0);
+    }
+    if (nz != 1 || code < 0) return true;
+    if (b0 == -2) b0 = code;
+    else if (code != b0) return true;
+  }
+  return false;
+}
+
+/* the cells of [2, mesh+3]^2 with a nonzero delta-phi adjoint that are P candidates */
+template <int N>
+__global__ __launch_bounds__(256) void k_adj_plist(const double *__restrict__ phi, const double *__restrict__ gdphi,
+                                                   int32_t *__restrict__ plist, int *pcount, int64_t fs, int64_t ps,
+                                                   int64_t mx, int64_t my) {
+  const int64_t y = 2 + (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
+  const int64_t x = 2 + (int64_t)blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int64_t c = x * ps + y + kYOff;
+  bool flag = false;
+  if (x <= mx + 3 && y <= my + 3) {
+    for (int a = 0; a < N; a++) flag = flag || gdphi[a * fs + c] != 0.0;
+    flag = flag && p_candidate<N>(phi, c, fs, ps);
+  }
+  list_append(flag, (int32_t)c, plist, pcount);
+}
+
 template <int N, int NC, bool P2>
 __global__ __launch_bounds__(256) void k_adj_p(const Model<Dual<kAdjK>> *__restrict__ mp, const double *__restrict__ phi,
                                                const double *__restrict__ mu, const double *__restrict__ gdphi,
-                                               double *gphi, double *gmu, double *gtheta, int64_t fs, int64_t ps,
-                                               int64_t mx, int64_t my, int chunk0) {
+                                               double *gphi, double *gmu, double *gtheta,
+                                               const int32_t *__restrict__ list, const int *__restrict__ count,
+                                               int64_t fs, int64_t ps, int chunk0) {
   constexpr int K = kAdjK;
-  const int64_t y = 2 + (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
-  const int64_t x = 2 + (int64_t)blockIdx.y * 4 + (threadIdx.x >> 6);
   constexpr int NTH = kTheta<N, NC>;
   double th[NTH];
   for (int t = 0; t < NTH; t++) th[t] = 0.0;
-  if (x <= mx + 3 && y <= my + 3) {
-    const int64_t c = x * ps + y + kYOff;
+  const int n = *count;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int64_t c = list[i];
     double go[N];
-    bool any = false;
     for (int a = 0; a < N; a++) go[a] = gdphi[a * fs + c];
-    for (int a = 0; a < N; a++) any = any || go[a] != 0.0;
-    if (any) {
+    {
       const Model<Dual<K>> &m = *mp;
       SeedView<N, NC, K, P2, 0> v{phi, mu, nullptr, fs, ps, c, 0, 0, chunk0};
       Dual<K> d[N];

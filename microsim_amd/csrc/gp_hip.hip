@@ -607,13 +607,15 @@ struct Adjoint {
   std::vector<double *> ckpt;
   std::vector<double *> seg;
   double *g[2] = {nullptr, nullptr};
-  double *gdphi = nullptr, *gth = nullptr;
+  double *gdphi = nullptr, *gth = nullptr, *cbar = nullptr;
+  int32_t *qlist = nullptr, *plist = nullptr; /* cells of the dual-number stages Q and P */
+  int *counts = nullptr;                      /* their lengths */
   void *qmodels = nullptr, *pmodels = nullptr;
   int nq = 0, np = 0;
   ~Adjoint() {
     for (double *b : ckpt) (void)hipFree(b);
     for (double *b : seg) (void)hipFree(b);
-    void *bufs[] = {g[0], g[1], gdphi, gth, qmodels, pmodels};
+    void *bufs[] = {g[0], g[1], gdphi, gth, cbar, qlist, plist, counts, qmodels, pmodels};
     for (void *b : bufs)
       if (b) (void)hipFree(b);
   }
@@ -695,15 +697,27 @@ static int adj_step_t(gp_handle *h) {
   hipLaunchKernelGGL(k_adj_bc, gall, blk, 0, h->stream, gn + (size_t)N * fs, NC, h->maps, 1, fs, h->ps, h->rx, h->ry);
   HIPCHK(h, hipMemsetAsync(gc, 0, state_elems(h) * sizeof(double), h->stream));
   HIPCHK(h, hipMemsetAsync(A->gdphi, 0, (size_t)N * fs * sizeof(double), h->stream));
+  /* bulk cells of stage Q in closed form (plain stores), then the dual kernels add the rest */
+  HIPCHK(h, hipMemsetAsync(A->counts, 0, 2 * sizeof(int), h->stream));
+  const Model<double> *md = (const Model<double> *)h->model;
+  hipLaunchKernelGGL((k_adj_cbar<N, NC>), gall, blk, 0, h->stream, md, phi, dphi, gn, gn + (size_t)N * fs, A->cbar,
+                     A->qlist, A->counts, fs, h->ps, mx, my);
+  const dim3 gb((unsigned)((my + 2 + 63) / 64), (unsigned)((mx + 2 + 3) / 4));
+  hipLaunchKernelGGL((k_adj_bulk<N, NC>), gb, blk, 0, h->stream, md, phi, mu, dphi, (const double *)A->cbar, gn, gc,
+                     gc + (size_t)N * fs, A->gdphi, A->gth, 3 * (N * (N - 1) / 2), fs, h->ps, mx, my);
+  /* the other cells by dual numbers, over compacted lists (a grid-stride loop of one resident round) */
   using MD = Model<Dual<kAdjK>>;
-  const dim3 gq((unsigned)((my + 63) / 64), (unsigned)((mx + 3) / 4));
+  const dim3 gl(512);
   for (int i = 0; i < A->nq; i++)
-    hipLaunchKernelGGL((k_adj_q<N, NC, P2>), gq, blk, 0, h->stream, (const MD *)A->qmodels + i, phi, mu, dphi, gn,
-                       gn + (size_t)N * fs, gc, gc + (size_t)N * fs, A->gdphi, A->gth, fs, h->ps, mx, my, i * kAdjK);
-  const dim3 gp((unsigned)((my + 2 + 63) / 64), (unsigned)((mx + 2 + 3) / 4));
+    hipLaunchKernelGGL((k_adj_q<N, NC, P2>), gl, blk, 0, h->stream, (const MD *)A->qmodels + i, phi, mu, dphi, gn,
+                       gn + (size_t)N * fs, gc, gc + (size_t)N * fs, A->gdphi, A->gth, (const int32_t *)A->qlist,
+                       (const int *)A->counts, fs, h->ps, i * kAdjK);
+  hipLaunchKernelGGL((k_adj_plist<N>), gb, blk, 0, h->stream, phi, (const double *)A->gdphi, A->plist, A->counts + 1, fs,
+                     h->ps, mx, my);
   for (int i = 0; i < A->np; i++)
-    hipLaunchKernelGGL((k_adj_p<N, NC, P2>), gp, blk, 0, h->stream, (const MD *)A->pmodels + i, phi, mu, A->gdphi, gc,
-                       gc + (size_t)N * fs, A->gth, fs, h->ps, mx, my, i * kAdjK);
+    hipLaunchKernelGGL((k_adj_p<N, NC, P2>), gl, blk, 0, h->stream, (const MD *)A->pmodels + i, phi, mu, A->gdphi, gc,
+                       gc + (size_t)N * fs, A->gth, (const int32_t *)A->plist, (const int *)A->counts + 1, fs, h->ps,
+                       i * kAdjK);
   HIPCHK(h, hipGetLastError());
   std::swap(A->g[0], A->g[1]);
   return GP_OK;
@@ -996,6 +1010,7 @@ int gp_adjoint_forward(gp_handle *h, int64_t nsteps, int64_t every) {
   if (h->precision != GP_FP64) return fail(h, GP_E_ARG, "the adjoint is built for the FP64 mode");
   if (!pick_adj(h)) return fail(h, GP_E_ARG, "no adjoint instantiated for %d phases / %d components", h->N, h->NC);
   if (!h->use_march) return fail(h, GP_E_ARG, "the adjoint records single-pass steps (unset GP_PATH)");
+  if (h->fs > 0x7fffffff) return fail(h, GP_E_ARG, "the adjoint indexes cells with 32 bits: grid too large");
   delete h->adj;
   h->adj = new Adjoint();
   Adjoint *A = h->adj;
@@ -1026,7 +1041,11 @@ int gp_adjoint_backward(gp_handle *h, const double *gphi_T, const double *gmu_T,
     bool ok = hipMalloc((void **)&A->g[0], se * sizeof(double)) == hipSuccess &&
               hipMalloc((void **)&A->g[1], se * sizeof(double)) == hipSuccess &&
               hipMalloc((void **)&A->gdphi, (size_t)h->N * h->fs * sizeof(double)) == hipSuccess &&
-              hipMalloc((void **)&A->gth, sizeof(double) * nth) == hipSuccess;
+              hipMalloc((void **)&A->gth, sizeof(double) * nth) == hipSuccess &&
+              hipMalloc((void **)&A->cbar, (size_t)h->NC * h->fs * sizeof(double)) == hipSuccess &&
+              hipMalloc((void **)&A->qlist, (size_t)h->fs * sizeof(int32_t)) == hipSuccess &&
+              hipMalloc((void **)&A->plist, (size_t)h->fs * sizeof(int32_t)) == hipSuccess &&
+              hipMalloc((void **)&A->counts, 2 * sizeof(int)) == hipSuccess;
     for (int64_t j = 0; ok && j < std::min(A->every, A->nsteps); j++) {
       double *b = nullptr;
       ok = hipMalloc((void **)&b, se * sizeof(double)) == hipSuccess;
