@@ -99,9 +99,12 @@ def run_mode(case, precision: int, steps: int, warmup: int):
 
 
 def adjoint_ratio(case, steps: int):
-    """Time of the FP64 reverse sweep against the forward, on the bench workload: `steps` plain FP64 steps, then
-    the same steps recorded for the adjoint (checkpoints every about sqrt(steps)) and the reverse sweep for a
-    random linear loss of the final state (segment recompute, transposed step, parameter gradient)."""
+    """Cost of the FP64 adjoint against the forward on the bench workload. `steps` plain FP64 steps are timed,
+    then the same steps recorded for the adjoint (checkpoints every about sqrt(steps)) and the reverse sweep for a
+    random linear loss of the final state. From HIP events on the solver's stream: the forward step (its kernels),
+    the transposed step (reload of the recomputed state, delta-phi, the adjoint kernels) and the segment
+    recompute per step; the adjoint step is the transposed step plus the recompute share. Wall-clock times
+    include the host copies of the loss gradient in and of the state and parameter gradients out."""
     import numpy as np
     from microsim_amd import _abi
     from microsim_amd.solver import GPSolver
@@ -110,28 +113,37 @@ def adjoint_ratio(case, steps: int):
     with GPSolver(case.params, _abi.GP_FP64) as s:
         s.upload(case.phi, case.mu)
         s.step(2)
-        s.synchronize()
         s.upload(case.phi, case.mu)
         s.synchronize()
+        s.profile_enable(True)
+        s.profile_read()
         t0 = time.perf_counter()
         s.step(steps)
         s.synchronize()
         fwd = time.perf_counter() - t0
+        pf = s.profile_read()
         s.upload(case.phi, case.mu)
         s.synchronize()
         t0 = time.perf_counter()
         s.adjoint_forward(steps)
         s.synchronize()
         rec = time.perf_counter() - t0
+        s.profile_read()
         t0 = time.perf_counter()
         s.adjoint_backward(wphi, wmu)
         bwd = time.perf_counter() - t0
-    return {"steps": steps, "forward_ms_per_step": round(fwd * 1e3 / steps, 4),
-            "recorded_forward_ms_per_step": round(rec * 1e3 / steps, 4),
-            "reverse_ms_per_step": round(bwd * 1e3 / steps, 4),
-            "reverse_over_forward": round(bwd / fwd, 3),
-            "note": "reverse includes the segment recompute and the host copies of the loss gradient in and the "
-                    "state and parameter gradients out"}
+        pb = s.profile_read()
+        s.profile_enable(False)
+    fwd_step = sum(ms * n for ms, n in pf.values()) / steps
+    adj_ms, adj_n = pb.pop("adj_step")
+    recompute = sum(ms * n for ms, n in pb.values()) / adj_n
+    adj_step = adj_ms + recompute
+    return {"steps": steps, "forward_step_ms": round(fwd_step, 4), "transposed_step_ms": round(adj_ms, 4),
+            "recompute_ms_per_step": round(recompute, 4), "adjoint_step_ms": round(adj_step, 4),
+            "adjoint_over_forward": round(adj_step / fwd_step, 3),
+            "wall": {"forward_ms_per_step": round(fwd * 1e3 / steps, 4),
+                     "recorded_forward_ms_per_step": round(rec * 1e3 / steps, 4),
+                     "reverse_ms_per_step": round(bwd * 1e3 / steps, 4)}}
 
 
 def copy_bandwidth(nbytes: int = 1 << 31, reps: int = 10):

@@ -182,7 +182,8 @@ const char *gp_version(void);
 #define GP_KERNEL_FUSED_STATS 4 /* single-pass step kernel with the statistics reduction */
 #define GP_KERNEL_BULK 5     /* single-pass step kernel of the tiles inside one pure phase */
 #define GP_KERNEL_DPHI 6     /* delta-phi of the interface candidate cells */
-#define GP_NKERNELS 7
+#define GP_KERNEL_ADJ 7      /* one transposed step of gp_adjoint_backward (its segment recompute not included) */
+#define GP_NKERNELS 8
 int gp_profile_enable(gp_handle *h, int enable);
 int gp_profile_read(gp_handle *h, double *avg_ms, int64_t *count);
 const char *gp_kernel_name(int kind);

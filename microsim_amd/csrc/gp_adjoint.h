@@ -27,7 +27,26 @@
 
 namespace gpk {
 
-constexpr int kAdjK = 8; /* directions per dual pass */
+/* Directions per dual pass. Measured on the 4096^2 ternary workload (reverse step, FP64): K = 8 113 ms, 4 92 ms,
+ * 2 64 ms, 1 43 ms -- the per-cell functions in Dual<K> are register-bound, and one direction per pass keeps them
+ * in registers at the price of recomputing the values every pass. */
+#ifndef GP_ADJ_K
+#define GP_ADJ_K 1
+#endif
+constexpr int kAdjK = GP_ADJ_K;
+
+/* The parameters each stage depends on, seeded as directions after the stage's state inputs: stage P
+ * (phi_delta) reads the pair parameters and tau but no diffusivity; stage Q (currents and mu update) reads only
+ * the liquid's diffusivities. Direction j of a stage's parameter block is theta entry stage_theta(stage, j). */
+enum { kStageP = 0, kStageQ = 1 };
+__host__ __device__ inline int stage_theta_count(int stage, int N, int NC) {
+  return stage == kStageP ? 3 * (N * (N - 1) / 2) + 1 : NC;
+}
+__host__ __device__ inline int stage_theta(int stage, int j, int N, int NC) {
+  const int np3 = 3 * (N * (N - 1) / 2);
+  if (stage == kStageP) return j < np3 ? j : np3 + NC;
+  return np3 + j;
+}
 
 /* Stencil accessor that seeds the inputs of one stage. MODE 0 (stage P): phi on the 3x3 stencil, mu at the
  * centre. MODE 1 (stage Q): phi on the 3x3 stencil, mu and delta-phi on the 5-point stencil. Directions:
@@ -102,7 +121,8 @@ __device__ __forceinline__ void scatter_dir(int dir, double g, int64_t c, int64_
     }
     dir -= 5 * N;
   }
-  if (dir < nth) th[dir] += g;
+  if (dir < stage_theta_count(MODE == 0 ? kStageP : kStageQ, N, NC))
+    th[stage_theta(MODE == 0 ? kStageP : kStageQ, dir, N, NC)] += g;
 }
 
 /* block sum of the per-thread parameter adjoints, one atomic per block and parameter */

@@ -640,25 +640,18 @@ static int state_load(gp_handle *h, const double *src) {
   return GP_OK;
 }
 
-/* halo-inclusive host planes (rows_x*rows_y, y fastest) <-> device planes, without boundary copies */
+/* halo-inclusive host planes (rows_x*rows_y, y fastest) <-> device planes, without boundary copies: one strided
+ * copy (the planes are rows_x rows apart on both sides) */
 static int planes_to_device(gp_handle *h, double *dst, const double *src, int ncomp) {
-  const int64_t rx = h->rx, ry = h->ry;
-  std::vector<double> stage((size_t)ncomp * h->fs, 0.0);
-  for (int c = 0; c < ncomp; c++)
-    for (int64_t x = 0; x < rx; x++)
-      for (int64_t y = 0; y < ry; y++) stage[(size_t)c * h->fs + x * h->ps + y + kYOff] = src[((size_t)c * rx + x) * ry + y];
-  HIPCHK(h, hipMemcpyAsync(dst, stage.data(), stage.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpy2DAsync(dst + kYOff, h->ps * sizeof(double), src, h->ry * sizeof(double), h->ry * sizeof(double),
+                             (size_t)ncomp * h->rx, hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return GP_OK;
 }
 static int planes_to_host(gp_handle *h, double *dst, const double *src, int ncomp) {
-  const int64_t rx = h->rx, ry = h->ry;
-  std::vector<double> stage((size_t)ncomp * h->fs);
-  HIPCHK(h, hipMemcpyAsync(stage.data(), src, stage.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipMemcpy2DAsync(dst, h->ry * sizeof(double), src + kYOff, h->ps * sizeof(double), h->ry * sizeof(double),
+                             (size_t)ncomp * h->rx, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  for (int c = 0; c < ncomp; c++)
-    for (int64_t x = 0; x < rx; x++)
-      for (int64_t y = 0; y < ry; y++) dst[((size_t)c * rx + x) * ry + y] = stage[(size_t)c * h->fs + x * h->ps + y + kYOff];
   return GP_OK;
 }
 
@@ -666,14 +659,23 @@ static int planes_to_host(gp_handle *h, double *dst, const double *src, int ncom
 static int adj_models(gp_handle *h) {
   Adjoint *A = h->adj;
   if (A->qmodels) return GP_OK;
-  const int N = h->N, NC = h->NC, nth = theta_count(N, NC);
+  const int N = h->N, NC = h->NC;
   const int dq = 9 * N + 5 * NC + 5 * N, dp = 9 * N + NC;
-  A->nq = (dq + nth + kAdjK - 1) / kAdjK;
-  A->np = (dp + nth + kAdjK - 1) / kAdjK;
+  const int tq = stage_theta_count(kStageQ, N, NC), tp = stage_theta_count(kStageP, N, NC);
+  A->nq = (dq + tq + kAdjK - 1) / kAdjK;
+  A->np = (dp + tp + kAdjK - 1) / kAdjK;
   using MD = Model<Dual<kAdjK>>;
   std::vector<MD> q(A->nq), pm(A->np);
-  for (int i = 0; i < A->nq; i++) make_dual_model<kAdjK>(&h->p, dq, i * kAdjK, &q[i]);
-  for (int i = 0; i < A->np; i++) make_dual_model<kAdjK>(&h->p, dp, i * kAdjK, &pm[i]);
+  /* theta entry t carries direction base + j of the chunk when stage_theta(stage, j) == t */
+  auto seeds = [&](int stage, int base, int nt, int chunk0) {
+    return [=](int t, int k) {
+      for (int j = 0; j < nt; j++)
+        if (stage_theta(stage, j, N, NC) == t) return base + j - chunk0 == k ? 1.0 : 0.0;
+      return 0.0;
+    };
+  };
+  for (int i = 0; i < A->nq; i++) make_dual_model_w<kAdjK>(&h->p, seeds(kStageQ, dq, tq, i * kAdjK), &q[i]);
+  for (int i = 0; i < A->np; i++) make_dual_model_w<kAdjK>(&h->p, seeds(kStageP, dp, tp, i * kAdjK), &pm[i]);
   if (hipMalloc(&A->qmodels, sizeof(MD) * A->nq) != hipSuccess || hipMalloc(&A->pmodels, sizeof(MD) * A->np) != hipSuccess)
     return fail(h, GP_E_OOM, "adjoint: dual models do not fit in device memory");
   HIPCHK(h, hipMemcpyAsync(A->qmodels, q.data(), sizeof(MD) * A->nq, hipMemcpyHostToDevice, h->stream));
@@ -1069,6 +1071,7 @@ int gp_adjoint_backward(gp_handle *h, const double *gphi_T, const double *gmu_T,
       if (j + 1 < n && (rc = gp_step(h, 1, nullptr))) return rc;
     }
     for (int64_t j = n - 1; j >= 0; j--) {
+      ProfScope ps(h, GP_KERNEL_ADJ);
       if ((rc = state_load(h, A->seg[j]))) return rc;
       if ((rc = f(h, MODE_DPHI_CUR))) return rc;
       if ((rc = fa(h))) return rc;
@@ -1091,7 +1094,7 @@ int gp_profile_enable(gp_handle *h, int enable) {
 
 const char *gp_kernel_name(int kind) {
   static const char *names[GP_NKERNELS] = {"k_phi", "k_mu", "k_stats", "k_march", "k_march_stats", "k_bulk",
-                                           "k_dphi"};
+                                           "k_dphi", "adj_step"};
   return (kind >= 0 && kind < GP_NKERNELS) ? names[kind] : "?";
 }
 

@@ -54,10 +54,4 @@ inline void make_dual_model_w(const gp_params *p, W w, Model<Dual<K>> *m) {
       }
 }
 
-/* unit seeds: theta t is direction dtheta0 + t; the model carries directions [chunk0, chunk0 + K) */
-template <int K>
-inline void make_dual_model(const gp_params *p, int dtheta0, int chunk0, Model<Dual<K>> *m) {
-  make_dual_model_w<K>(p, [=](int t, int k) { return dtheta0 + t - chunk0 == k ? 1.0 : 0.0; }, m);
-}
-
 }  // namespace gpk
