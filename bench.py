@@ -53,6 +53,17 @@ def make_case(mesh: int, seeds: int, seed: int = 1234):
     return case, d
 
 
+def binary_seed_case(mesh: int, steps: int, radius: float = 60.0):
+    """Binary anisotropic seed material (configs.config_c0: c_s 0.2, c_l 0.4, liquid at 0.34) at mesh^2 with a tanh
+    seed of the given radius (radius 10 is below the critical radius of this material and melts)."""
+    from microsim_amd import configs, host
+    inp, _ = configs.config_c0(mesh, steps, steps)
+    d = tempfile.mkdtemp(prefix="gp_bench_c0_")
+    configs.write_case(d, inp, [])
+    phi_i, mu_i = configs.tanh_seed_state(mesh, mesh, radius, 2.0, [2.0 * 1.0 * 0.34])
+    return host.state_case(os.path.join(d, "Input.in"), phi_i, mu_i)
+
+
 def run_mode(case, precision: int, steps: int, warmup: int):
     """Time `steps` solver steps with the reference loop's per-step statistics (cumulative max/min, sums of
     squared increments) folded on the device every step, as the driver runs them."""
@@ -98,7 +109,7 @@ def run_mode(case, precision: int, steps: int, warmup: int):
     }
 
 
-def adjoint_ratio(case, steps: int):
+def adjoint_ratio(params, phi0, mu0, steps: int, every: int = 0):
     """Cost of the FP64 adjoint against the forward on the bench workload. `steps` plain FP64 steps are timed,
     then the same steps recorded for the adjoint (checkpoints every about sqrt(steps)) and the reverse sweep for a
     random linear loss of the final state. From HIP events on the solver's stream: the forward step (its kernels),
@@ -109,11 +120,11 @@ def adjoint_ratio(case, steps: int):
     from microsim_amd import _abi
     from microsim_amd.solver import GPSolver
     rng = np.random.default_rng(5)
-    wphi, wmu = rng.standard_normal(case.phi.shape), rng.standard_normal(case.mu.shape)
-    with GPSolver(case.params, _abi.GP_FP64) as s:
-        s.upload(case.phi, case.mu)
+    wphi, wmu = rng.standard_normal(phi0.shape), rng.standard_normal(mu0.shape)
+    with GPSolver(params, _abi.GP_FP64) as s:
+        s.upload(phi0, mu0)
         s.step(2)
-        s.upload(case.phi, case.mu)
+        s.upload(phi0, mu0)
         s.synchronize()
         s.profile_enable(True)
         s.profile_read()
@@ -122,10 +133,10 @@ def adjoint_ratio(case, steps: int):
         s.synchronize()
         fwd = time.perf_counter() - t0
         pf = s.profile_read()
-        s.upload(case.phi, case.mu)
+        s.upload(phi0, mu0)
         s.synchronize()
         t0 = time.perf_counter()
-        s.adjoint_forward(steps)
+        s.adjoint_forward(steps, every)
         s.synchronize()
         rec = time.perf_counter() - t0
         s.profile_read()
@@ -138,7 +149,7 @@ def adjoint_ratio(case, steps: int):
     adj_ms, adj_n = pb.pop("adj_step")
     recompute = sum(ms * n for ms, n in pb.values()) / adj_n
     adj_step = adj_ms + recompute
-    return {"steps": steps, "forward_step_ms": round(fwd_step, 4), "transposed_step_ms": round(adj_ms, 4),
+    return {"steps": steps, "checkpoint_every": every or int(np.ceil(np.sqrt(steps))), "forward_step_ms": round(fwd_step, 4), "transposed_step_ms": round(adj_ms, 4),
             "recompute_ms_per_step": round(recompute, 4), "adjoint_step_ms": round(adj_step, 4),
             "adjoint_over_forward": round(adj_step / fwd_step, 3),
             "wall": {"forward_ms_per_step": round(fwd * 1e3 / steps, 4),
@@ -304,8 +315,14 @@ def main():
     if "fp32" in res and "fp64" in res:
         out["fp32_over_fp64"] = res["fp32"]["value"] / res["fp64"]["value"]
     if a.adjoint_steps > 0:
-        out["adjoint"] = adjoint_ratio(case, a.adjoint_steps)
+        out["adjoint"] = adjoint_ratio(case.params, case.phi, case.mu, a.adjoint_steps)
+        out["adjoint"]["workload"] = "this workload, FP64, random linear loss of the final state"
         log("adjoint", json.dumps(out["adjoint"]))
+        bcase = binary_seed_case(256, 2000)
+        out["adjoint_binary256"] = adjoint_ratio(bcase.params, bcase.phi, bcase.mu, 2000, 100)
+        out["adjoint_binary256"]["workload"] = ("binary anisotropic seed (radius 60) 256x256, 2000 steps, "
+                                                "checkpoints every 100, FP64")
+        log("adjoint binary 256", json.dumps(out["adjoint_binary256"]))
     if a.error_steps > 0:
         errs = [precision_errors(case, a.error_steps)]
         if a.mesh != 1024:

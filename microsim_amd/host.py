@@ -84,6 +84,16 @@ def load_case(input_path: str, filling_path: str, restart_vtk: str = None, quiet
     return Case(inp, phi, mu)
 
 
+def state_case(input_path: str, phi_interior: np.ndarray, mu_interior: np.ndarray) -> Case:
+    """Parameters from Input.in with a given interior state (what a restart snapshot supplies), halos applied."""
+    inp = read_input(input_path)
+    phi, mu = empty_state(inp)
+    phi[:, 3:-3, 3:-3] = phi_interior
+    mu[:, 3:-3, 3:-3] = mu_interior
+    _abi.host_lib().gp_apply_bc_host(C.byref(inp.p), _abi.dptr(phi), _abi.dptr(mu))
+    return Case(inp, phi, mu)
+
+
 def write_vtk(case_inp: _abi.GPInput, path: str, phi: np.ndarray, mu: np.ndarray) -> None:
     lib = _abi.host_lib()
     phi = np.ascontiguousarray(phi, dtype=np.float64)

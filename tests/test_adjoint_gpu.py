@@ -203,3 +203,84 @@ def test_theta_round_trip_cpu():
         assert np.array_equal(adjoint.theta_from_params(p3), th * 1.5)
     with pytest.raises(ValueError):
         adjoint.params_with_theta(case.params, th[:-1])
+
+
+def _c0_case(mesh, steps, radius=60.0):
+    """The binary anisotropic seed material (configs.config_c0) at mesh^2: tanh seed, liquid mu. The workload's
+    own radius-10 seed lies below the critical radius (interface energy 1 over a driving force of 0.024: about 42
+    cells) and melts within 500 steps, after which every gradient of a loss on the final phi is zero; a seed of
+    radius 60 keeps a moving interface through the run."""
+    import os
+    import tempfile
+    from microsim_amd import configs, host
+    inp, _ = configs.config_c0(mesh, steps, steps)
+    d = tempfile.mkdtemp(prefix="gp_c0_")
+    configs.write_case(d, inp, [])
+    phi_i, mu_i = configs.tanh_seed_state(mesh, mesh, radius, 2.0, [2.0 * 1.0 * 0.34])
+    case = host.state_case(os.path.join(d, "Input.in"), phi_i, mu_i)
+    return case.params, case.phi, case.mu
+
+
+@pytest.mark.gpu
+def test_calibration_gradient_binary_256():
+    """Binary anisotropic seed (radius 60) at 256^2 over 2000 steps, loss = mean squared difference between the final phi and
+    a target made by a forward run with dab 0.03, tau 0.25 and liquid diffusivity 1.2; gradients with respect to
+    dab, tau_ab, Gamma, the liquid diffusivity, tau and the initial phi (checkpoints every 100 steps), against
+    central differences with a relative step of 1e-6."""
+    import torch
+    from microsim_amd import adjoint
+    from microsim_amd.solver import GPSolver
+    _say("calibration gradient binary 256^2 2000 steps")
+    steps = 2000
+    params, phi0, mu0 = _c0_case(256, steps)
+    names = adjoint.theta_names(2, 2)
+    th0 = adjoint.theta_from_params(params)
+    tt = th0.copy()
+    tt[names.index("dab[0][1]")] = 0.03
+    tt[names.index("tau_ab[0][1]")] = 0.25
+    tt[names.index("tau")] = 0.25
+    tt[names.index("D[1][0][0]")] = 1.2
+
+    def final_phi(th, phi):
+        with GPSolver(adjoint.params_with_theta(params, th), _abi.GP_FP64) as s:
+            s.upload(phi, mu0)
+            s.step(steps)
+            return s.download()[0]
+
+    target = final_phi(tt, phi0)[:, 3:-3, 3:-3]
+
+    def loss_np(th, phi):
+        return float(np.mean((final_phi(th, phi)[:, 3:-3, 3:-3] - target) ** 2))
+
+    th = torch.tensor(th0, requires_grad=True)
+    p0 = torch.tensor(phi0, requires_grad=True)
+    phiT, _ = adjoint.trajectory(params, th, p0, torch.tensor(mu0), steps, every=100)
+    loss = torch.mean((phiT[:, 3:-3, 3:-3] - torch.from_numpy(target)) ** 2)
+    loss.backward()
+    lval = float(loss.detach())
+    assert lval > 0 and abs(lval - loss_np(th0, phi0)) <= 1e-12 * lval
+    gth = th.grad.numpy()
+    fd = np.empty_like(th0)
+    for i in range(th0.size):
+        h = 1e-6 * abs(th0[i])
+        tp, tm = th0.copy(), th0.copy()
+        tp[i] += h
+        tm[i] -= h
+        fd[i] = (loss_np(tp, phi0) - loss_np(tm, phi0)) / (tp[i] - tm[i])
+        print("  %-14s adj % .10e  fd % .10e" % (names[i], gth[i], fd[i]))
+    scale = np.abs(fd * th0)
+    err = np.abs(gth - fd) * np.abs(th0)
+    assert np.all(err <= 1e-4 * np.maximum(scale, 1e-3 * scale.max())), list(zip(names, gth, fd))
+    # the initial phi: directional difference over the interface cells. The loss is strongly curved along this
+    # direction (central differences of the oracle's trajectory move from -2.0e-4 at h = 1e-4 to -3.0104e-4 at
+    # 1e-6, -3.01129e-4 at 1e-7 and -3.011304e-4 at 1e-8), so the step here is 1e-8.
+    rng = np.random.default_rng(31)
+    iface = np.zeros(phi0.shape[1:], dtype=bool)
+    iface[3:-3, 3:-3] = True
+    iface &= np.all((phi0 > 1e-3) & (phi0 < 1 - 1e-3), axis=0)
+    v = rng.standard_normal(phi0.shape) * iface
+    adj = float(np.sum(p0.grad.numpy() * v))
+    hh = 1e-8
+    fdv = (loss_np(th0, phi0 + hh * v) - loss_np(th0, phi0 - hh * v)) / (2 * hh)
+    print("  initial phi along v (%d cells): adj % .10e fd % .10e" % (iface.sum(), adj, fdv))
+    assert abs(adj - fdv) <= 1e-4 * abs(fdv)
