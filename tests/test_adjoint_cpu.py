@@ -1,9 +1,13 @@
-"""Forward-mode derivative of the step arithmetic on the host, the reference the device adjoint is checked against.
+"""Derivative references for the device adjoint, on the host.
 
-model_jvp (tests/cpu_model/model_step.cpp) steps the kernels' per-cell functions in dual numbers. Here:
-  * its values are the FP64 oracle's, bit for bit (the dual arithmetic leaves the value path unchanged);
-  * its tangent matches central finite differences of the oracle over one step on every golden deck, and over
-    twenty steps on the decks whose trajectories are smooth at the finite-difference scale.
+model_jvp (tests/cpu_model/model_step.cpp) steps the kernels' per-cell functions in dual numbers; torch_step.py is
+an independent whole-grid torch restatement of the step, differentiated by torch.autograd. Here:
+  * model_jvp's values are the FP64 oracle's, bit for bit (the dual arithmetic leaves the value path unchanged);
+  * torch_step's values agree with the oracle to rounding on every golden deck;
+  * the two derivatives agree: <grad L, v> from torch.autograd equals <w, J v> from model_jvp to rounding, for a
+    tangent v over every interior cell of phi and mu and every parameter, on every golden deck;
+  * model_jvp's tangent matches central finite differences of the oracle over one step on every golden deck,
+    and over twenty steps on the decks whose trajectories are smooth at the finite-difference scale.
 Over many steps a trajectory that starts from a sharp interface is not smooth at that scale: cells of the
 spreading tail cross the anisotropy's |q|^2 > 1e-15 cut-off and the simplex projection's clamp, so central
 differences of a deck such as bin_well_rot wander by 1e-4 relative as the step size changes. Those decks are
@@ -13,6 +17,7 @@ which holds whatever the branches.
 import numpy as np
 import pytest
 
+import torch_step
 from gp_testlib import golden_cases, load_golden, model_jvp, oracle_step
 from microsim_amd import adjoint
 
@@ -77,3 +82,44 @@ def test_jvp_one_step_matches_central_differences(name):
 def test_jvp_trajectory_matches_central_differences(name):
     case, _ = load_golden(name)
     _check_tangent(case, 20, 3, 1e-6, 1e-6)
+
+
+def torch_trajectory(case, n, theta=None, phi0=None, mu0=None):
+    """n torch steps from (phi0, mu0) (the case's state by default), boundary copies applied first as an upload
+    does; returns (phi, mu)."""
+    import torch
+    m = torch_step.TorchModel(case.params, theta)
+    phi = phi0 if phi0 is not None else torch.tensor(case.phi)
+    mu = mu0 if mu0 is not None else torch.tensor(case.mu)
+    phi = torch.stack(torch_step.apply_bc(m, list(phi), 0))
+    mu = torch.stack(torch_step.apply_bc(m, list(mu), 1))
+    for _ in range(n):
+        phi, mu = torch_step.step(m, phi, mu)
+    return phi, mu
+
+
+@pytest.mark.parametrize("name", golden_cases())
+def test_torch_step_matches_oracle(name):
+    case, _ = load_golden(name)
+    phi, mu = torch_trajectory(case, 5)
+    rphi, rmu = _oracle_run(case, case.params, case.phi, case.mu, 5)
+    assert np.abs(phi.numpy() - rphi).max() <= 1e-14 * max(np.abs(rphi).max(), 1.0)
+    assert np.abs(mu.numpy() - rmu).max() <= 1e-14 * max(np.abs(rmu).max(), 1.0)
+
+
+@pytest.mark.parametrize("name", golden_cases())
+def test_torch_autograd_matches_forward_mode(name):
+    import torch
+    case, _ = load_golden(name)
+    rng = np.random.default_rng(5)
+    wphi, wmu = rng.standard_normal(case.phi.shape), rng.standard_normal(case.mu.shape)
+    vphi, vmu, vth = directions(case, 6, phi_cells="all")
+    th = torch.tensor(adjoint.theta_from_params(case.params), requires_grad=True)
+    phi0 = torch.tensor(case.phi, requires_grad=True)
+    mu0 = torch.tensor(case.mu, requires_grad=True)
+    phi, mu = torch_trajectory(case, 5, th, phi0, mu0)
+    ((phi * torch.from_numpy(wphi)).sum() + (mu * torch.from_numpy(wmu)).sum()).backward()
+    lhs = np.concatenate([(phi0.grad.numpy() * vphi).ravel(), (mu0.grad.numpy() * vmu).ravel(), th.grad.numpy() * vth])
+    _, _, tphi, tmu = model_jvp(case.params, case.phi, case.mu, vphi, vmu, vth, 5)
+    rhs = np.concatenate([(wphi * tphi).ravel(), (wmu * tmu).ravel()])
+    assert abs(lhs.sum() - rhs.sum()) <= 1e-13 * np.abs(rhs).sum()

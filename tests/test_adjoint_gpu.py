@@ -13,7 +13,9 @@ halo-inclusive entry (so the transposed boundary conditions at the final state a
     (bin_well_rot is not: cells of its spreading tail cross the anisotropy cut-off and the projection's clamp);
   * independence of the checkpoint interval (the reverse sweep recomputes segments; the gradients must agree to
     rounding, which the atomic accumulation makes order-dependent);
-  * the forward of the differentiable trajectory equals plain stepping bit for bit.
+  * the forward of the differentiable trajectory equals plain stepping bit for bit;
+  * the full gradients (initial phi, mu and theta) equal those of torch.autograd through torch_step.py, an
+    independent FP64 torch restatement of the step, to rounding.
 The decks are the golden ones (binary, ternary and four-phase; isotropic and anisotropic, rotated; obstacle and
 well potentials; periodic, Neumann and mixed sides; dilute and LU solves; undercooling).
 """
@@ -284,3 +286,26 @@ def test_calibration_gradient_binary_256():
     fdv = (loss_np(th0, phi0 + hh * v) - loss_np(th0, phi0 - hh * v)) / (2 * hh)
     print("  initial phi along v (%d cells): adj % .10e fd % .10e" % (iface.sum(), adj, fdv))
     assert abs(adj - fdv) <= 1e-4 * abs(fdv)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,rtol", [("bin_tanh_restart", 1e-10), ("tern_c1_mini", 1e-10), ("quad_k3", 1e-10),
+                                       ("bin_well_rot", 1e-6)])
+def test_gradients_match_torch_autograd(name, rtol):
+    """Element-wise agreement, relative to the largest entry. bin_well_rot's initial-phi gradients reach 2e7 (the
+    1/|q| factors of the anisotropy at the cells of its sharp initial interface), and the rounding differences of
+    two evaluation orders grow with them: measured 2.8e-7 there."""
+    import torch
+    from microsim_amd import adjoint
+    from test_adjoint_cpu import torch_trajectory
+    _say("torch autograd reference " + name)
+    case, _ = load_golden(name)
+    wphi, wmu = _weights(case, 41)
+    g0, m0, gth, _, _ = _adjoint(case.params, case.phi, case.mu, wphi, wmu)
+    th = torch.tensor(adjoint.theta_from_params(case.params), requires_grad=True)
+    phi0 = torch.tensor(case.phi, requires_grad=True)
+    mu0 = torch.tensor(case.mu, requires_grad=True)
+    phi, mu = torch_trajectory(case, NSTEPS, th, phi0, mu0)
+    ((phi * torch.from_numpy(wphi)).sum() + (mu * torch.from_numpy(wmu)).sum()).backward()
+    for got, ref in ((g0, phi0.grad.numpy()), (m0, mu0.grad.numpy()), (gth, th.grad.numpy())):
+        assert np.abs(got - ref).max() <= rtol * np.abs(ref).max()
