@@ -29,7 +29,7 @@ GP_SOLVE_BINARY = 1
 GP_SOLVE_TERNARY = 2
 GP_SOLVE_DILUTE = 3
 
-GP_NKERNELS = 8
+GP_NKERNELS = 10
 
 P, NCM = GP_MAXP, GP_MAXNC
 

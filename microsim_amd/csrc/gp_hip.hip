@@ -28,6 +28,7 @@
 
 #include "gp_common.h"
 #include "gp_hip.h"
+#include "gp_cells.h"
 #include "gp_march.h"
 #include "gp_model.h"
 #include "gp_adjoint.h"
@@ -62,12 +63,16 @@ struct gp_handle {
   int64_t nparts_max;       /* rows of the partials buffer */
   int march_seg, march_nstrips, march_nsegs; /* tile grid of the single-pass step (see gp_march.h) */
   int64_t march_nwaves;     /* tiles = wavefronts of k_march and of k_bulk */
+  int cell_seg, cell_nsegs;  /* tile grid of k_cbulk / k_ciface (same strips, shorter segments) */
+  int64_t cell_nwaves;
   int *flags[2];            /* per-tile purity flags of the last two steps */
-  int8_t *codes;            /* phase code of every cell of the current state (pure phase b, or -1) */
+  int8_t *codes[2];         /* phase codes (pure phase b, or -1) of every cell: codes[ccur] of the current state */
+  int ccur;
   bool codes_valid;         /* codes match the current state */
   int fcur;                 /* flags[fcur] describes the current state */
   bool flags_valid;         /* flags[fcur] is valid for the current state (else no tile is taken as deep) */
   bool use_march;           /* single-pass steps (default) or the two-pass form */
+  bool use_cells;           /* single-pass update by cell class (k_cbulk + k_ciface, default) or by march tiles */
   bool prev_valid;          /* the other buffer holds the state before the last step */
   Maps maps;
   bool pow2;
@@ -369,6 +374,16 @@ static void march_plan(gp_handle *h) {
   h->march_nstrips = (int)nstrips;
   h->march_nsegs = (int)((mx + seg - 1) / seg);
   h->march_nwaves = nstrips * h->march_nsegs;
+  /* the cell-classified kernels have almost no per-tile warm-up: 16-row segments (more, smaller tiles) even out
+   * the interface work between wavefronts; GP_CELL_SEG overrides (at most kCellSegMax) */
+  int64_t cseg = 16;
+  const char *ce = getenv("GP_CELL_SEG");
+  if (ce && atoi(ce) > 0) cseg = atoi(ce);
+  if (cseg > kCellSegMax) cseg = kCellSegMax;
+  if (cseg > mx) cseg = mx;
+  h->cell_seg = (int)cseg;
+  h->cell_nsegs = (int)((mx + cseg - 1) / cseg);
+  h->cell_nwaves = nstrips * h->cell_nsegs;
 }
 
 template <int N, int NC, typename R, bool P2>
@@ -396,48 +411,90 @@ static int launch_step_p(gp_handle *h, int mode) {
     return GP_OK;
   }
   if (mode == MODE_MARCH || mode == MODE_MARCH_STATS) {
-    /* k_march updates the tiles near an interface, k_bulk the deep tiles (none when the flags are not valid);
-     * both read the current state, write disjoint tiles of the other buffer and the tile flags of the new state */
+    /* delta-phi of the candidate cells, then the update: by cell class (k_cbulk + k_ciface) or by march tiles
+     * (k_march near an interface, k_bulk on deep tiles). The update kernels read the current state and its phase
+     * codes and write disjoint cells of the other buffers (state and codes). */
     const int nt = (int)h->march_nwaves;
     const unsigned nblk = (unsigned)((nt + 3) / 4);
     const int *fin = h->flags_valid ? h->flags[h->fcur] : nullptr;
     int *fout = h->flags[1 - h->fcur];
     const int64_t pstride = 2 * (int64_t)nt;
+    const int8_t *codes = h->codes[h->ccur];
+    int8_t *codes_o = h->codes[1 - h->ccur];
+    const bool stats = mode == MODE_MARCH_STATS;
     if (!h->codes_valid) {
       hipLaunchKernelGGL((k_codes<N, R>), dim3((unsigned)((h->ry + 63) / 64), (unsigned)((h->rx + 3) / 4)), dim3(256),
-                         0, h->stream, phi, h->codes, h->fs, h->ps, h->rx, h->ry);
+                         0, h->stream, phi, h->codes[h->ccur], h->fs, h->ps, h->rx, h->ry);
     }
     {
-      /* delta-phi of the candidate cells, tiles of 64 y-cells by kDTileRows rows over x, y in [2, mesh+3] */
+      /* tiles of 64 y-cells by kDTileRows rows over x, y in [2, mesh+3] */
       const int nty = (int)((my + 2 + 63) / 64), ntx = (int)((mx + 2 + kDTileRows - 1) / kDTileRows);
       ProfScope ps(h, GP_KERNEL_DPHI);
       hipLaunchKernelGGL((k_dphi<N, NC, R, P2>), dim3((unsigned)((nty * ntx + 3) / 4)), dim3(256), 0, h->stream, m,
-                         phi, mu, h->codes, dphi, h->fs, h->ps, mx, my, nty, nty * ntx);
+                         phi, mu, codes, dphi, h->fs, h->ps, mx, my, nty, nty * ntx);
+    }
+    if (h->use_cells) {
+      const int ct = (int)h->cell_nwaves;
+      const unsigned cblk = (unsigned)((ct + 3) / 4);
+      const int64_t cstride = 2 * (int64_t)ct;
+      {
+        ProfScope ps(h, GP_KERNEL_CBULK);
+        if (stats)
+          hipLaunchKernelGGL((k_cbulk<N, NC, R, P2, true>), dim3(cblk), dim3(256), 0, h->stream, m, mu, codes, phi_o,
+                             mu_o, codes_o, h->maps, h->partials, cstride, h->fs, h->ps, mx, my, h->cell_seg,
+                             h->march_nstrips, ct, (int)h->prev_valid);
+        else
+          hipLaunchKernelGGL((k_cbulk<N, NC, R, P2, false>), dim3(cblk), dim3(256), 0, h->stream, m, mu, codes,
+                             phi_o, mu_o, codes_o, h->maps, h->partials, cstride, h->fs, h->ps, mx, my, h->cell_seg,
+                             h->march_nstrips, ct, (int)h->prev_valid);
+      }
+      {
+        ProfScope ps(h, GP_KERNEL_CIFACE);
+        if (stats)
+          hipLaunchKernelGGL((k_ciface<N, NC, R, P2, true>), dim3(cblk), dim3(256), 0, h->stream, m, phi, mu, dphi,
+                             codes, phi_o, mu_o, codes_o, h->maps, h->partials + ct, cstride, h->fs, h->ps, mx, my,
+                             h->cell_seg, h->march_nstrips, ct);
+        else
+          hipLaunchKernelGGL((k_ciface<N, NC, R, P2, false>), dim3(cblk), dim3(256), 0, h->stream, m, phi, mu, dphi,
+                             codes, phi_o, mu_o, codes_o, h->maps, h->partials + ct, cstride, h->fs, h->ps, mx, my,
+                             h->cell_seg, h->march_nstrips, ct);
+      }
+      if (stats) {
+        ProfScope ps(h, GP_KERNEL_STATS);
+        hipLaunchKernelGGL((k_stats<N, NC>), dim3(3 * N + 3 * NC + 1), dim3(kStatThreads), 0, h->stream,
+                           h->partials, cstride, cstride, h->stats_dev);
+      }
+      HIPCHK(h, hipGetLastError());
+      h->cur = 1 - h->cur;
+      h->ccur = 1 - h->ccur;
+      h->flags_valid = false; /* this path forms no tile flags */
+      h->codes_valid = true;
+      return GP_OK;
     }
     if (mode == MODE_MARCH) {
       {
         ProfScope ps(h, GP_KERNEL_FUSED);
         hipLaunchKernelGGL((k_march<N, NC, R, P2, false>), dim3(nblk), dim3(256), 0, h->stream, m, phi, mu, dphi,
-                           phi_o, mu_o, h->codes, h->maps, h->partials, pstride, fin, fout, h->fs, h->ps, mx, my,
+                           phi_o, mu_o, codes_o, h->maps, h->partials, pstride, fin, fout, h->fs, h->ps, mx, my,
                            h->march_seg, h->march_nstrips, h->march_nsegs, nt);
       }
       if (fin) {
         ProfScope ps(h, GP_KERNEL_BULK);
         hipLaunchKernelGGL((k_bulk<N, NC, R, P2, false>), dim3(nblk), dim3(256), 0, h->stream, m, mu, phi_o, mu_o,
-                           h->codes, h->partials + nt, pstride, fin, fout, h->fs, h->ps, mx, h->march_seg,
+                           codes_o, h->partials + nt, pstride, fin, fout, h->fs, h->ps, mx, h->march_seg,
                            h->march_nstrips, h->march_nsegs, nt);
       }
     } else {
       {
         ProfScope ps(h, GP_KERNEL_FUSED_STATS);
         hipLaunchKernelGGL((k_march<N, NC, R, P2, true>), dim3(nblk), dim3(256), 0, h->stream, m, phi, mu, dphi,
-                           phi_o, mu_o, h->codes, h->maps, h->partials, pstride, fin, fout, h->fs, h->ps, mx, my,
+                           phi_o, mu_o, codes_o, h->maps, h->partials, pstride, fin, fout, h->fs, h->ps, mx, my,
                            h->march_seg, h->march_nstrips, h->march_nsegs, nt);
       }
       if (fin) {
         ProfScope ps(h, GP_KERNEL_BULK);
         hipLaunchKernelGGL((k_bulk<N, NC, R, P2, true>), dim3(nblk), dim3(256), 0, h->stream, m, mu, phi_o, mu_o,
-                           h->codes, h->partials + nt, pstride, fin, fout, h->fs, h->ps, mx, h->march_seg,
+                           codes_o, h->partials + nt, pstride, fin, fout, h->fs, h->ps, mx, h->march_seg,
                            h->march_nstrips, h->march_nsegs, nt);
       }
       ProfScope ps(h, GP_KERNEL_STATS);
@@ -446,6 +503,7 @@ static int launch_step_p(gp_handle *h, int mode) {
     }
     HIPCHK(h, hipGetLastError());
     h->cur = 1 - h->cur;
+    h->ccur = 1 - h->ccur;
     h->fcur = 1 - h->fcur;
     h->flags_valid = true;
     h->codes_valid = true;
@@ -794,9 +852,10 @@ int gp_create(const gp_params *params, int precision, gp_handle **out) {
   }
   h->nblocks_mu = ((params->mesh_y + kBX - 1) / kBX) * ((params->mesh_x + kBY - 1) / kBY);
   march_plan(h);
-  h->nparts_max = std::max(h->nblocks_mu, 2 * h->march_nwaves); /* k_march and k_bulk partials */
+  h->nparts_max = std::max(h->nblocks_mu, 2 * std::max(h->march_nwaves, h->cell_nwaves)); /* two kernels' partials */
   const char *path = getenv("GP_PATH");
   h->use_march = !(path && strcmp(path, "twopass") == 0);
+  h->use_cells = h->use_march && !(path && strcmp(path, "march") == 0);
   h->prev_valid = false;
   h->fcur = 0;
   h->flags_valid = false;
@@ -813,7 +872,9 @@ int gp_create(const gp_params *params, int precision, gp_handle **out) {
   ok = ok && hipMalloc((void **)&h->inv_dev, sizeof(double) * (GP_MAXNC + 1)) == hipSuccess;
   ok = ok && hipMalloc((void **)&h->flags[0], sizeof(int) * h->march_nwaves) == hipSuccess;
   ok = ok && hipMalloc((void **)&h->flags[1], sizeof(int) * h->march_nwaves) == hipSuccess;
-  ok = ok && hipMalloc((void **)&h->codes, (size_t)h->rx * h->ps) == hipSuccess;
+  ok = ok && hipMalloc((void **)&h->codes[0], (size_t)h->rx * h->ps) == hipSuccess &&
+       hipMalloc((void **)&h->codes[1], (size_t)h->rx * h->ps) == hipSuccess;
+  h->ccur = 0;
   const size_t msize = precision == GP_FP64 ? sizeof(Model<double>) : sizeof(Model<float>);
   ok = ok && hipMalloc(&h->model, msize) == hipSuccess;
   if (!ok) {
@@ -861,7 +922,7 @@ void gp_destroy(gp_handle *h) {
   h->prof.clear();
   delete h->adj;
   void *bufs[] = {h->phi[0], h->phi[1], h->mu[0], h->mu[1], h->dphi, h->partials, h->stats_dev, h->model,
-                  h->flags[0], h->flags[1], h->inv_dev, h->codes};
+                  h->flags[0], h->flags[1], h->inv_dev, h->codes[0], h->codes[1]};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -1094,7 +1155,7 @@ int gp_profile_enable(gp_handle *h, int enable) {
 
 const char *gp_kernel_name(int kind) {
   static const char *names[GP_NKERNELS] = {"k_phi", "k_mu", "k_stats", "k_march", "k_march_stats", "k_bulk",
-                                           "k_dphi", "adj_step"};
+                                           "k_dphi", "adj_step", "k_cbulk", "k_ciface"};
   return (kind >= 0 && kind < GP_NKERNELS) ? names[kind] : "?";
 }
 

@@ -848,22 +848,39 @@ __global__ __launch_bounds__(256) void k_dphi(const Model<R> *__restrict__ mp, c
                                               R *__restrict__ dphi, int64_t fs, int64_t ps, int64_t mx, int64_t my,
                                               int nty, int ntiles) {
   __shared__ uint16_t list[4][kDTileRows * 64];
+  /* the tile's codes, rows x0-1 .. x1+1, columns y-1 .. y+64 (slot j+1 holds y0+j), staged with one batch of
+   * loads so that the candidate scan does not wait on memory row by row */
+  __shared__ int8_t cs_all[4][(kDTileRows + 2) * 66];
   const Model<R> &m = *mp;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int w = blockIdx.x * 4 + wv;
+  int8_t *cs = cs_all[wv];
   int count = 0;
   const int ty = w % nty, tx = w / nty;
   const int64_t y = 2 + (int64_t)ty * 64 + lane;
   const int64_t x0 = 2 + (int64_t)tx * kDTileRows;
+  const int64_t x1 = (x0 + kDTileRows - 1 < mx + 3) ? x0 + kDTileRows - 1 : mx + 3;
   if (w < ntiles) {
-    const int64_t x1 = (x0 + kDTileRows - 1 < mx + 3) ? x0 + kDTileRows - 1 : mx + 3;
+    const int64_t ry = my + 6;
+    const int64_t yc = y < ry - 1 ? y : ry - 1;
+    const int nr = (int)(x1 - x0) + 3;
+    for (int r = 0; r < nr; r++) {
+      const int64_t row = (x0 - 1 + r) * ps + kYOff;
+      cs[r * 66 + lane + 1] = codes[row + yc];
+      if (lane == 0) cs[r * 66] = codes[row + y - 1];
+      if (lane == 63) cs[r * 66 + 65] = codes[row + (y + 1 < ry - 1 ? y + 1 : ry - 1)];
+    }
+  }
+  __syncthreads();
+  if (w < ntiles) {
     const bool ylive = y <= my + 3;
     for (int64_t x = x0; x <= x1; x++) {
       bool cand = false;
       if (ylive) {
-        const int64_t c = x * ps + y + kYOff;
-        const int8_t k0 = codes[c];
-        cand = k0 < 0 || codes[c - ps] != k0 || codes[c + ps] != k0 || codes[c - 1] != k0 || codes[c + 1] != k0;
+        const int r = (int)(x - x0) + 1, j = lane + 1;
+        const int8_t k0 = cs[r * 66 + j];
+        cand = k0 < 0 || cs[(r - 1) * 66 + j] != k0 || cs[(r + 1) * 66 + j] != k0 || cs[r * 66 + j - 1] != k0 ||
+               cs[r * 66 + j + 1] != k0;
       }
       const uint64_t mask = __ballot(cand);
       const int pre = __popcll(mask & ((1ull << lane) - 1ull));
@@ -875,7 +892,17 @@ __global__ __launch_bounds__(256) void k_dphi(const Model<R> *__restrict__ mp, c
   for (int i = lane; i < count; i += 64) {
     const int r = list[wv][i];
     const int64_t c = (x0 + r / 64) * ps + (2 + (int64_t)ty * 64 + r % 64) + kYOff;
-    View<R, P2> v{phi, mu, nullptr, fs, ps, c};
+    /* phi on the 3x3 stencil and mu at the centre, gathered once into registers */
+    RegStore<N, NC, R> rs;
+#pragma unroll
+    for (int a = 0; a < N; a++)
+#pragma unroll
+      for (int di = 0; di < 3; di++)
+#pragma unroll
+        for (int dj = 0; dj < 3; dj++) rs.p[a][di][dj] = phi[a * fs + c + (di - 1) * ps + (dj - 1)];
+#pragma unroll
+    for (int k = 0; k < NC; k++) rs.u[k][0] = mu[k * fs + c];
+    const RegView<N, NC, R, P2> v{&rs, 0, 0};
     R d[N];
     phi_delta<N, NC>(m, v, false, d);
 #pragma unroll

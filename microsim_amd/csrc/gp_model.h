@@ -81,6 +81,31 @@ struct View {
   }
 };
 
+/* Register-resident stencil of one cell: phi on the 3x3 neighbourhood, mu and delta-phi on the 5-point one,
+ * gathered once (independent loads) so that the per-cell functions, which read the same values many times, work
+ * from registers. A RegView is a position (si, sj) on the store; after inlining every index is a constant. The
+ * per-cell functions read phi within one cell of the centre and mu and delta-phi on the 5-point stencil only
+ * (offsets (0,0), (1,0), (-1,0), (0,1), (0,-1): slots 0..4). */
+GP_HD int five_slot(int i, int j) {
+  return (i == 0 && j == 0) ? 0 : (i == 1 && j == 0) ? 1 : (i == -1 && j == 0) ? 2 : (i == 0 && j == 1) ? 3 : 4;
+}
+template <int N, int NC, typename R>
+struct RegStore {
+  R p[N][3][3];
+  R u[NC > 0 ? NC : 1][5];
+  R d[N][5];
+};
+template <int N, int NC, typename R, bool POW2>
+struct RegView {
+  static constexpr bool kPow2 = POW2;
+  const RegStore<N, NC, R> *s;
+  int si, sj;
+  GP_HD R ph(int a, int i, int j) const { return s->p[a][si + i + 1][sj + j + 1]; }
+  GP_HD R m(int k, int i, int j) const { return s->u[k][five_slot(si + i, sj + j)]; }
+  GP_HD R dp(int a, int i, int j) const { return s->d[a][five_slot(si + i, sj + j)]; }
+  GP_HD RegView shift(int i, int j) const { return RegView{s, si + i, sj + j}; }
+};
+
 /* Quotient and square root of the step arithmetic (the one place that chooses their form). FP64: IEEE,
  * correctly rounded, as the reference computes them, so the FP64 results are bit-identical to it. FP32 (the
  * production mode) takes the device build's fast single-precision forms (hipcc
