@@ -88,6 +88,111 @@ struct CodesView {
   }
 };
 
+/* the bulk update of one cell in phase b (UNIFORM: b is the same on every lane, read as a scalar) */
+template <int N, int NC, typename R, bool P2, bool STATS, bool UNIFORM, int NSA>
+__device__ __forceinline__ void cbulk_cell(const Model<R> &m, int b, int64_t x, const CellTile &t, const R *m0,
+                                           const R *m1, const R *m2, const R *mud, const R *muu, R divjat0,
+                                           R *__restrict__ phi_o, R *__restrict__ mu_o,
+                                           int8_t *__restrict__ codes_out, const Maps &maps, int64_t fs, int64_t ps,
+                                           int64_t mx, int keep, R (&st)[NSA]) {
+  using Vw = View<R, P2>;
+  R Mb[NC][NC], pe[N];
+#pragma unroll
+  for (int k = 0; k < NC; k++)
+#pragma unroll
+    for (int l = 0; l < NC; l++) {
+      if (UNIFORM) {
+        Mb[k][l] = m.Mpure[b][k][l];
+      } else {
+        Mb[k][l] = m.Mpure[0][k][l];
+#pragma unroll
+        for (int a = 1; a < N; a++)
+          if (b == a) Mb[k][l] = m.Mpure[a][k][l];
+      }
+    }
+#pragma unroll
+  for (int a = 0; a < N; a++) pe[a] = a == b ? R(1.0) : R(0.0);
+  R divflux[NC], deltamu[NC], munew[NC];
+#pragma unroll
+  for (int k = 0; k < NC; k++) {
+    divflux[k] = R(0.0);
+#pragma unroll
+    for (int l = 0; l < NC; l++) {
+      const R Fb = Mb[k][l] * divx<Vw>(m, (m1[l] - m0[l]));
+      const R Ff = Mb[k][l] * divx<Vw>(m, (m2[l] - m1[l]));
+      const R Fy = Mb[k][l] * divy<Vw>(m, (mud[l] - m1[l]));
+      const R Fyl = Mb[k][l] * divy<Vw>(m, (m1[l] - muu[l])); /* = Fy of the lane below, bit for bit */
+      divflux[k] += divx<Vw>(m, (Ff - Fb));
+      divflux[k] += divy<Vw>(m, (Fy - Fyl));
+    }
+  }
+  if (kDirectBulkMu<R>) {
+    R dc[NC];
+#pragma unroll
+    for (int k = 0; k < NC; k++) dc[k] = m.dt * (divflux[k]) - divjat0;
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+      R s = R(0);
+      if (UNIFORM) {
+        s = Mu_lin<NC>(m, dc, b, k);
+      } else {
+#pragma unroll
+        for (int a = 0; a < N; a++)
+          if (a == b) s = Mu_lin<NC>(m, dc, a, k);
+      }
+      deltamu[k] = s;
+      munew[k] = m1[k] + deltamu[k];
+    }
+  } else {
+    R c_new[NC];
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+      R s = R(0);
+      if (UNIFORM) {
+        s = c_mu<NC>(m, m1, b, k);
+      } else {
+#pragma unroll
+        for (int a = 0; a < N; a++)
+          if (a == b) s = c_mu<NC>(m, m1, a, k);
+      }
+      c_new[k] = s + m.dt * (divflux[k]) - divjat0;
+    }
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+      R s = R(0);
+      if (UNIFORM) {
+        s = Mu<NC>(m, c_new, b, k);
+      } else {
+#pragma unroll
+        for (int a = 0; a < N; a++)
+          if (a == b) s = Mu<NC>(m, c_new, a, k);
+      }
+      munew[k] = s;
+      deltamu[k] = munew[k] - m1[k];
+    }
+  }
+  const int64_t c = x * ps + t.y + kYOff;
+  const int8_t code = (int8_t)b;
+  /* keep: the output buffers hold the state before the current one and its codes; a cell that was purely
+   * phase b there already holds phi = e_b, so only mu is written */
+  if (!(keep && codes_out[c] == code)) {
+#pragma unroll
+    for (int a = 0; a < N; a++) phi_o[a * fs + c] = pe[a];
+    codes_out[c] = code;
+  }
+#pragma unroll
+  for (int k = 0; k < NC; k++) mu_o[k * fs + c] = munew[k];
+  if (t.img_y || x < 7 || x > mx - 2) {
+    write_images(codes_out, 1, &code, 0, maps, x, t.y, 0, ps);
+    write_images(phi_o, N, pe, 0, maps, x, t.y, fs, ps);
+    write_images(mu_o, NC, munew, 1, maps, x, t.y, fs, ps);
+  }
+  if (STATS) {
+    const R z[N] = {};
+    stats_add<N, NC>(st, pe, z, munew, deltamu);
+  }
+}
+
 constexpr int kCellBulkDepth = 4; /* mu rows in flight ahead of the current one */
 
 template <int N, int NC, typename R, bool P2, bool STATS>
@@ -145,88 +250,16 @@ __global__ __launch_bounds__(256) void k_cbulk(const Model<R> *__restrict__ mp, 
         mud[k] = lane_dn(m1[k]);
         muu[k] = lane_up(m1[k]);
       }
-      const int b = t.out_lane ? bulk_of(cs, (int)(x - t.xa) + 2, lane) : -1;
-      if (b >= 0) {
-        R Mb[NC][NC], pe[N];
-#pragma unroll
-        for (int k = 0; k < NC; k++)
-#pragma unroll
-          for (int l = 0; l < NC; l++) {
-            Mb[k][l] = m.Mpure[0][k][l];
-#pragma unroll
-            for (int a = 1; a < N; a++)
-              if (b == a) Mb[k][l] = m.Mpure[a][k][l];
-          }
-#pragma unroll
-        for (int a = 0; a < N; a++) pe[a] = a == b ? R(1.0) : R(0.0);
-        R divflux[NC], deltamu[NC], munew[NC];
-#pragma unroll
-        for (int k = 0; k < NC; k++) {
-          divflux[k] = R(0.0);
-#pragma unroll
-          for (int l = 0; l < NC; l++) {
-            const R Fb = Mb[k][l] * divx<Vw>(m, (m1[l] - m0[l]));
-            const R Ff = Mb[k][l] * divx<Vw>(m, (m2[l] - m1[l]));
-            const R Fy = Mb[k][l] * divy<Vw>(m, (mud[l] - m1[l]));
-            const R Fyl = Mb[k][l] * divy<Vw>(m, (m1[l] - muu[l])); /* = Fy of the lane below, bit for bit */
-            divflux[k] += divx<Vw>(m, (Ff - Fb));
-            divflux[k] += divy<Vw>(m, (Fy - Fyl));
-          }
-        }
-        if (kDirectBulkMu<R>) {
-          R dc[NC];
-#pragma unroll
-          for (int k = 0; k < NC; k++) dc[k] = m.dt * (divflux[k]) - divjat0;
-#pragma unroll
-          for (int k = 0; k < NC; k++) {
-            R s = R(0);
-#pragma unroll
-            for (int a = 0; a < N; a++)
-              if (a == b) s = Mu_lin<NC>(m, dc, a, k);
-            deltamu[k] = s;
-            munew[k] = m1[k] + deltamu[k];
-          }
-        } else {
-          R c_new[NC];
-#pragma unroll
-          for (int k = 0; k < NC; k++) {
-            R s = R(0);
-#pragma unroll
-            for (int a = 0; a < N; a++)
-              if (a == b) s = c_mu<NC>(m, m1, a, k);
-            c_new[k] = s + m.dt * (divflux[k]) - divjat0;
-          }
-#pragma unroll
-          for (int k = 0; k < NC; k++) {
-            R s = R(0);
-#pragma unroll
-            for (int a = 0; a < N; a++)
-              if (a == b) s = Mu<NC>(m, c_new, a, k);
-            munew[k] = s;
-            deltamu[k] = munew[k] - m1[k];
-          }
-        }
-        const int64_t c = x * ps + t.y + kYOff;
-        const int8_t code = (int8_t)b;
-        /* keep: the output buffers hold the state before the current one and its codes; a cell that was purely
-         * phase b there already holds phi = e_b, so only mu is written */
-        if (!(keep && codes_out[c] == code)) {
-#pragma unroll
-          for (int a = 0; a < N; a++) phi_o[a * fs + c] = pe[a];
-          codes_out[c] = code;
-        }
-#pragma unroll
-        for (int k = 0; k < NC; k++) mu_o[k * fs + c] = munew[k];
-        if (t.img_y || x < 7 || x > mx - 2) {
-          write_images(codes_out, 1, &code, 0, maps, x, t.y, 0, ps);
-          write_images(phi_o, N, pe, 0, maps, x, t.y, fs, ps);
-          write_images(mu_o, NC, munew, 1, maps, x, t.y, fs, ps);
-        }
-        if (STATS) {
-          const R z[N] = {};
-          stats_add<N, NC>(st, pe, z, munew, deltamu);
-        }
-      }
+      const int bl = t.out_lane ? bulk_of(cs, (int)(x - t.xa) + 2, lane) : -1;
+      /* a row whose output lanes are all bulk in one phase (the common case) takes the phase as a scalar */
+      const int b0 = __builtin_amdgcn_readlane(bl, 2);
+      if (b0 >= 0 && __all(!t.out_lane || bl == b0)) {
+        if (t.out_lane)
+          cbulk_cell<N, NC, R, P2, STATS, true>(m, b0, x, t, m0, m1, m2, mud, muu, divjat0, phi_o, mu_o, codes_out,
+                                                maps, fs, ps, mx, keep, st);
+      } else if (bl >= 0)
+        cbulk_cell<N, NC, R, P2, STATS, false>(m, bl, x, t, m0, m1, m2, mud, muu, divjat0, phi_o, mu_o, codes_out,
+                                               maps, fs, ps, mx, keep, st);
 #pragma unroll
       for (int k = 0; k < NC; k++) {
         m0[k] = m1[k];
