@@ -12,8 +12,11 @@ the whole grid; one cell-update is one interior cell advanced by one step.
 Prints one JSON line. ``value`` is cell-updates/s of the first mode in --modes with the state resident in HBM,
 every step including the reference loop's per-step statistics (``no_stats`` gives the same steps without
 them); the other modes appear under ``modes``. ``roofline`` is for the dominant kernel: the algorithmic bytes
-2 (N + NC) sizeof(real) per cell-update (phi and mu read once and written once) times the cells of one launch,
-over that kernel's mean duration, timed with HIP events on the solver's stream inside the timed region.
+2 (N + NC) sizeof(real) per cell-update (phi and mu read once and written once) times the cells that one launch
+advances (k_cbulk: the bulk cells, k_ciface: the interface cells, counted on the state before and after the timed
+steps), over that kernel's mean duration, timed with HIP events on the solver's stream inside the timed region.
+``roofline_step`` is the same for the whole step: every interior cell over the summed mean durations of the
+step's kernels (delta-phi, both updates, statistics fold); ``roofline_kernels`` lists each update kernel.
 A streaming-copy kernel is timed on the same device as a measured bandwidth reference. ``cpu_baseline`` times
 the serial reference binary (oracle/_ref, built from the reference sources) on a bounded sample: the same
 ternary model on 1024 x 1024 with 64 seeds, per-step wall time from two runs that differ only in their step
@@ -64,6 +67,26 @@ def binary_seed_case(mesh: int, steps: int, radius: float = 60.0):
     return host.state_case(os.path.join(d, "Input.in"), phi_i, mu_i)
 
 
+def cell_classes(phi: "np.ndarray"):
+    """(bulk cells, interface cells) of a halo-inclusive state, as the cell-classified update splits them: a bulk
+    cell's 13-cell diamond (Manhattan radius 2) of phase codes is purely one phase."""
+    import numpy as np
+    N = phi.shape[0]
+    code = np.full(phi.shape[1:], -1, dtype=np.int8)
+    nz = np.zeros(phi.shape[1:], dtype=np.int8)
+    for a in range(N):
+        code[phi[a] == 1.0] = a
+        nz += (phi[a] != 0.0)
+    code[nz != 1] = -1
+    c = code[1:-1, 1:-1]  # cells 1 .. rows-2, so the diamond of every interior cell is inside
+    ok = c[2:-2, 2:-2] >= 0
+    centre = c[2:-2, 2:-2]
+    for dx, dy in ((1, 0), (-1, 0), (2, 0), (-2, 0), (0, 1), (0, -1), (0, 2), (0, -2), (1, 1), (1, -1), (-1, 1), (-1, -1)):
+        ok &= c[2 + dx:c.shape[0] - 2 + dx, 2 + dy:c.shape[1] - 2 + dy] == centre
+    nb = int(ok.sum())
+    return nb, int(ok.size) - nb
+
+
 def run_mode(case, precision: int, steps: int, warmup: int):
     """Time `steps` solver steps with the reference loop's per-step statistics (cumulative max/min, sums of
     squared increments) folded on the device every step, as the driver runs them."""
@@ -77,11 +100,13 @@ def run_mode(case, precision: int, steps: int, warmup: int):
         s.synchronize()
         s.profile_enable(True)
         s.profile_read()
+        phi_a, _ = s.download()
         t0 = time.perf_counter()
         st = s.step(steps, stats=True)
         s.synchronize()
         el = time.perf_counter() - t0
         prof = s.profile_read()
+        phi_b, _ = s.download()
         # the same steps without the statistics work (a caller that does not ask for diagnostics)
         t0 = time.perf_counter()
         s.step(steps)
@@ -90,21 +115,37 @@ def run_mode(case, precision: int, steps: int, warmup: int):
         prof_ns = s.profile_read()
         s.profile_enable(False)
     step_ms = sum(ms for ms, cnt in prof.values())
-    dom = max(prof, key=lambda k: prof[k][0])
     b_alg = 2 * (N + NC) * es
-    # algorithmic bytes of one launch of the dominant kernel (one step over every interior cell) / its mean time
-    achieved = cells * b_alg / (prof[dom][0] * 1e-3) / 1e9 if prof[dom][0] > 0 else 0.0
+    # cells each update kernel advances per launch: the mean of the bulk / interface split before and after the
+    # timed steps
+    cls = [cell_classes(p) for p in (phi_a, phi_b)]
+    nbulk = 0.5 * (cls[0][0] + cls[1][0])
+    kcells = {"k_cbulk": nbulk, "k_ciface": cells - nbulk, "k_march": cells, "k_march_stats": cells}
+    per_kernel = {}
+    for k, (ms, cnt) in prof.items():
+        if k in kcells and ms > 0:
+            gbs = kcells[k] * b_alg / (ms * 1e-3) / 1e9
+            per_kernel[k] = {"ms": round(ms, 4), "cells": int(kcells[k]), "GBps": round(gbs, 1),
+                             "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic_per_launch(k, precision)}
+    dom = max(per_kernel, key=lambda k: per_kernel[k]["ms"])
+    d = per_kernel[dom]
+    step_gbs = cells * b_alg / (step_ms * 1e-3) / 1e9
     return {
         "value": cells * steps / el,
         "ms_per_step": el * 1e3 / steps,
         "kernels_ms": {k: round(v[0], 4) for k, v in prof.items()},
         "step_kernels_ms": round(step_ms, 4),
         "dominant": dom,
+        "bulk_cell_fraction": round(nbulk / cells, 4),
         "no_stats": {"value": cells * steps / el_ns, "ms_per_step": el_ns * 1e3 / steps,
                      "kernels_ms": {k: round(v[0], 4) for k, v in prof_ns.items()}},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_per_launch(dom, precision),
-                     "kernel": dom, "alg_bytes_per_cell_update": b_alg, "cells_per_launch": cells},
+        "roofline": {"bound": "hbm", "achieved": d["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": d["frac"], "traffic": d["traffic"], "kernel": dom,
+                     "alg_bytes_per_cell_update": b_alg, "cells_per_launch": d["cells"]},
+        "roofline_step": {"achieved": round(step_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": round(step_gbs / HBM_PEAK_GBS, 4),
+                          "kernels": sorted(prof), "cells_per_step": cells, "alg_bytes_per_cell_update": b_alg},
+        "roofline_kernels": per_kernel,
         "nonfinite": st["nonfinite"],
     }
 
@@ -309,6 +350,7 @@ def main():
                    "mesh": [a.mesh, a.mesh], "phases": 3, "components": 3, "seeds": a.seeds,
                    "parallelism": "single-gpu"},
         "roofline": dict(r0["roofline"], measured_copy_GBps=copy["GBps"]),
+        "roofline_step": r0["roofline_step"],
         "copy_kernel": copy,
         "modes": {m: {k: v for k, v in r.items()} for m, r in res.items()},
     }
