@@ -13,7 +13,7 @@ Prints one JSON line. ``value`` is cell-updates/s of the first mode in --modes w
 every step including the reference loop's per-step statistics (``no_stats`` gives the same steps without
 them); the other modes appear under ``modes``. ``roofline`` is for the dominant kernel: the algorithmic bytes
 2 (N + NC) sizeof(real) per cell-update (phi and mu read once and written once) times the cells that one launch
-advances (k_cbulk: the bulk cells, k_ciface: the interface cells, counted on the state before and after the timed
+advances (k_cbulk: the bulk cells, k_ilist: the interface cells, counted on the state before and after the timed
 steps), over that kernel's mean duration, timed with HIP events on the solver's stream inside the timed region.
 ``roofline_step`` is the same for the whole step: every interior cell over the summed mean durations of the
 step's kernels (delta-phi, both updates, statistics fold); ``roofline_kernels`` lists each update kernel.
@@ -120,7 +120,7 @@ def run_mode(case, precision: int, steps: int, warmup: int):
     # timed steps
     cls = [cell_classes(p) for p in (phi_a, phi_b)]
     nbulk = 0.5 * (cls[0][0] + cls[1][0])
-    kcells = {"k_cbulk": nbulk, "k_ciface": cells - nbulk, "k_march": cells, "k_march_stats": cells}
+    kcells = {"k_cbulk": nbulk, "k_ilist": cells - nbulk, "k_march": cells, "k_march_stats": cells}
     per_kernel = {}
     for k, (ms, cnt) in prof.items():
         if k in kcells and ms > 0:

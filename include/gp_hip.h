@@ -184,7 +184,7 @@ const char *gp_version(void);
 #define GP_KERNEL_DPHI 6     /* delta-phi of the interface candidate cells */
 #define GP_KERNEL_ADJ 7      /* one transposed step of gp_adjoint_backward (its segment recompute not included) */
 #define GP_KERNEL_CBULK 8    /* update of the bulk cells (cell-classified step) */
-#define GP_KERNEL_CIFACE 9   /* update of the interface cells (cell-classified step) */
+#define GP_KERNEL_CIFACE 9   /* update of the interface cells (cell-classified step, k_ilist) */
 #define GP_NKERNELS 10
 int gp_profile_enable(gp_handle *h, int enable);
 int gp_profile_read(gp_handle *h, double *avg_ms, int64_t *count);

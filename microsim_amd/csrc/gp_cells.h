@@ -11,12 +11,12 @@
  * it streams the mu rows with a deep prefetch and forms, per lane, the same operations in the same order as the
  * pure-phase step and k_bulk, so the results are bit-identical. phi of a bulk cell is e_b; when the output buffer
  * already holds e_b there (the cell was purely b two states back) it is not rewritten.
- * k_ciface collects the interface cells of a tile into an LDS list and runs the general per-cell update
- * (gp_model.h: jat_x, jat_y, mu_update -- bit-identical to the reference by construction) on 64 of them at a
- * time, so that every lane works on an interface cell, not one in five as in a march row.
- * Both kernels stage the tile's phase codes (rows xa-2 .. xb+2, the strip's 64 lanes) in LDS and apply the same
- * predicate, so each interior cell is written by exactly one of them. They read the codes of the current state
- * and write those of the new one (a second buffer).
+ * k_cbulk also lists the interface cells of its tile (LDS, then one atomic per wavefront into a global list);
+ * k_ilist runs the general per-cell update (gp_model.h: jat_x, jat_y, mu_update -- bit-identical to the
+ * reference by construction) on the listed cells in a grid-stride loop, so that every lane of every wavefront
+ * works on an interface cell (a march row has about one in five) and no wavefront holds more than its share.
+ * Every interior cell is written by exactly one of the two. They read the phase codes of the current state and
+ * write those of the new one (a second buffer).
  */
 #pragma once
 
@@ -60,33 +60,6 @@ __device__ __forceinline__ void stage_codes(int8_t *cs, const int8_t *__restrict
   const int nr = (int)(t.xb - t.xa) + 5;
   for (int r = 0; r < nr; r++) cs[r * 64 + lane] = codes[(t.xa - 2 + r) * ps + t.yl + kYOff];
 }
-
-/* View of the general per-cell functions whose delta-phi reads k_dphi's values on candidate cells (codes of the
- * 5-point neighbourhood not uniform) and exactly 0 elsewhere, as the step defines them */
-template <typename R, bool POW2>
-struct CodesView {
-  static constexpr bool kPow2 = POW2;
-  const R *phi, *mu, *dphi;
-  int64_t fs, ps, c;
-  const int8_t *cs;
-  int r, l;
-  __device__ R ph(int a, int i, int j) const { return phi[a * fs + c + i * ps + j]; }
-  __device__ R m(int k, int i, int j) const { return mu[k * fs + c + i * ps + j]; }
-  __device__ R dp(int a, int i, int j) const {
-    const int rr = r + i, ll = l + j;
-    const int8_t k0 = cs[rr * 64 + ll];
-    const bool cand = k0 < 0 || cs[(rr - 1) * 64 + ll] != k0 || cs[(rr + 1) * 64 + ll] != k0 ||
-                      cs[rr * 64 + ll - 1] != k0 || cs[rr * 64 + ll + 1] != k0;
-    return cand ? dphi[a * fs + c + i * ps + j] : R(0.0);
-  }
-  __device__ CodesView shift(int i, int j) const {
-    CodesView v = *this;
-    v.c = c + i * ps + j;
-    v.r = r + i;
-    v.l = l + j;
-    return v;
-  }
-};
 
 /* the bulk update of one cell in phase b (UNIFORM: b is the same on every lane, read as a scalar) */
 template <int N, int NC, typename R, bool P2, bool STATS, bool UNIFORM, int NSA>
@@ -201,11 +174,14 @@ __global__ __launch_bounds__(256) void k_cbulk(const Model<R> *__restrict__ mp, 
                                                R *__restrict__ mu_o, int8_t *__restrict__ codes_out, Maps maps,
                                                double *__restrict__ partials, int64_t pstride, int64_t fs, int64_t ps,
                                                int64_t mx, int64_t my, int seg_len, int nstrips, int nwaves,
-                                               int keep) {
+                                               int keep, int32_t *__restrict__ ilist, int *icount) {
   using Vw = View<R, P2>;
   constexpr int NS = 3 * N + 3 * NC + 1;
   constexpr int NSA = STATS ? NS : 1;
   __shared__ int8_t cs_all[4][kCellRows * 64];
+  __shared__ uint16_t il_all[4][kCellSegMax * 64];
+  uint16_t *il = il_all[threadIdx.x >> 6];
+  int icnt = 0;
   const Model<R> &m = *mp;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int w = blockIdx.x * 4 + wv;
@@ -251,6 +227,13 @@ __global__ __launch_bounds__(256) void k_cbulk(const Model<R> *__restrict__ mp, 
         muu[k] = lane_up(m1[k]);
       }
       const int bl = t.out_lane ? bulk_of(cs, (int)(x - t.xa) + 2, lane) : -1;
+      {
+        /* the row's interface cells, for k_ilist */
+        const bool iface = t.out_lane && bl < 0;
+        const uint64_t mask = __ballot(iface);
+        if (iface) il[icnt + __popcll(mask & ((1ull << lane) - 1ull))] = (uint16_t)((x - t.xa) * 64 + lane);
+        icnt += __popcll(mask);
+      }
       /* a row whose output lanes are all bulk in one phase (the common case) takes the phase as a scalar */
       const int b0 = __builtin_amdgcn_readlane(bl, 2);
       if (b0 >= 0 && __all(!t.out_lane || bl == b0)) {
@@ -266,66 +249,73 @@ __global__ __launch_bounds__(256) void k_cbulk(const Model<R> *__restrict__ mp, 
         m1[k] = m2[k];
       }
     }
+    /* one atomic per wavefront: reserve the tile's interface cells in the global list */
+    if (icnt > 0) {
+      int base = 0;
+      if (lane == 0) base = atomicAdd(icount, icnt);
+      base = __shfl(base, 0);
+      for (int i = lane; i < icnt; i += 64) {
+        const int e = il[i];
+        ilist[base + i] = (int32_t)((t.xa + (e >> 6)) * ps + (t.y - lane + (e & 63)) + kYOff);
+      }
+    }
   }
   if (!STATS) return;
   stats_partials<N, NC>(st, partials, pstride, w, nwaves, lane);
 }
 
+/* The general update of the interface cells listed by k_cbulk (any order): a grid-stride loop of full
+ * wavefronts. Each cell gathers its stencil from memory in one batch: phi on the 3x3 neighbourhood, mu and
+ * delta-phi on the 5-point one, and the phase codes of its radius-2 diamond, which decide which of the five
+ * delta-phi values are k_dphi's (candidates) and which are exactly 0. Statistics: one partial per wavefront of the
+ * grid (the grid is fixed, the order of the cells within it is not). */
 template <int N, int NC, typename R, bool P2, bool STATS>
-__global__ __launch_bounds__(256) void k_ciface(const Model<R> *__restrict__ mp, const R *__restrict__ phi,
-                                                const R *__restrict__ mu, const R *__restrict__ dphi,
-                                                const int8_t *__restrict__ codes_in, R *__restrict__ phi_o,
-                                                R *__restrict__ mu_o, int8_t *__restrict__ codes_out, Maps maps,
-                                                double *__restrict__ partials, int64_t pstride, int64_t fs,
-                                                int64_t ps, int64_t mx, int64_t my, int seg_len, int nstrips,
-                                                int nwaves) {
+__global__ __launch_bounds__(256) void k_ilist(const Model<R> *__restrict__ mp, const R *__restrict__ phi,
+                                               const R *__restrict__ mu, const R *__restrict__ dphi,
+                                               const int8_t *__restrict__ codes_in, R *__restrict__ phi_o,
+                                               R *__restrict__ mu_o, int8_t *__restrict__ codes_out, Maps maps,
+                                               double *__restrict__ partials, int64_t pstride, int64_t fs,
+                                               int64_t ps, int64_t mx, int64_t my,
+                                               const int32_t *__restrict__ ilist, const int *__restrict__ icount) {
   constexpr int NS = 3 * N + 3 * NC + 1;
   constexpr int NSA = STATS ? NS : 1;
-  __shared__ int8_t cs_all[4][kCellRows * 64];
-  __shared__ uint16_t list_all[4][kCellSegMax * 64];
   const Model<R> &m = *mp;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int w = blockIdx.x * 4 + wv;
-  int8_t *cs = cs_all[wv];
-  uint16_t *list = list_all[wv];
+  const int lane = threadIdx.x & 63;
+  const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int nw = gridDim.x * 4;
   R st[NSA];
   if (STATS) stats_init<N, NC>(st);
-  const CellTile t(w, lane, mx, my, seg_len, nstrips);
-  if (w < nwaves) stage_codes(cs, codes_in, t, ps, lane);
-  __syncthreads();
-  int count = 0;
-  if (w < nwaves) {
-    for (int64_t x = t.xa; x <= t.xb; x++) {
-      const bool iface = t.out_lane && bulk_of(cs, (int)(x - t.xa) + 2, lane) < 0;
-      const uint64_t mask = __ballot(iface);
-      const int pre = __popcll(mask & ((1ull << lane) - 1ull));
-      if (iface) list[count + pre] = (uint16_t)((x - t.xa) * 64 + lane);
-      count += __popcll(mask);
-    }
-  }
-  __syncthreads();
-  for (int i = lane; i < count; i += 64) {
-    const int e = list[i];
-    const int xr = e >> 6, l = e & 63;
-    const int64_t x = t.xa + xr, y = t.y - lane + l;
-    const int64_t c = x * ps + y + kYOff;
-    /* gather the cell's stencil (delta-phi gated by the phase codes), then compute from registers */
+  const int n = *icount;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int64_t c = ilist[i];
+    const int64_t x = (c - kYOff) / ps, y = (c - kYOff) % ps;
+    /* codes of the diamond: slot (di, dj) for |di| + |dj| <= 2 */
+    int8_t cd[5][5];
+#pragma unroll
+    for (int di = -2; di <= 2; di++)
+#pragma unroll
+      for (int dj = -2; dj <= 2; dj++)
+        if ((di < 0 ? -di : di) + (dj < 0 ? -dj : dj) <= 2) cd[di + 2][dj + 2] = codes_in[c + di * ps + dj];
     RegStore<N, NC, R> rs;
-    {
-      const CodesView<R, P2> g{phi, mu, dphi, fs, ps, c, cs, xr + 2, l};
-      const int oi[5] = {0, 1, -1, 0, 0}, oj[5] = {0, 0, 0, 1, -1};
+    const int oi[5] = {0, 1, -1, 0, 0}, oj[5] = {0, 0, 0, 1, -1};
 #pragma unroll
-      for (int a = 0; a < N; a++)
+    for (int a = 0; a < N; a++)
 #pragma unroll
-        for (int i = 0; i < 3; i++)
+      for (int di = 0; di < 3; di++)
 #pragma unroll
-          for (int j = 0; j < 3; j++) rs.p[a][i][j] = g.ph(a, i - 1, j - 1);
+        for (int dj = 0; dj < 3; dj++) rs.p[a][di][dj] = phi[a * fs + c + (di - 1) * ps + (dj - 1)];
 #pragma unroll
-      for (int o = 0; o < 5; o++) {
+    for (int o = 0; o < 5; o++) {
+      const int64_t co = c + oi[o] * ps + oj[o];
 #pragma unroll
-        for (int k = 0; k < NC; k++) rs.u[k][o] = g.m(k, oi[o], oj[o]);
+      for (int k = 0; k < NC; k++) rs.u[k][o] = mu[k * fs + co];
+      const int r = oi[o] + 2, q = oj[o] + 2;
+      const int8_t k0 = cd[r][q];
+      const bool cand = k0 < 0 || cd[r - 1][q] != k0 || cd[r + 1][q] != k0 || cd[r][q - 1] != k0 || cd[r][q + 1] != k0;
 #pragma unroll
-        for (int a = 0; a < N; a++) rs.d[a][o] = g.dp(a, oi[o], oj[o]);
+      for (int a = 0; a < N; a++) {
+        const R v = dphi[a * fs + co];
+        rs.d[a][o] = cand ? v : R(0.0);
       }
     }
     const RegView<N, NC, R, P2> v{&rs, 0, 0};
@@ -351,7 +341,7 @@ __global__ __launch_bounds__(256) void k_ciface(const Model<R> *__restrict__ mp,
     if (STATS) stats_add<N, NC>(st, pn, d, mn, dm);
   }
   if (!STATS) return;
-  stats_partials<N, NC>(st, partials, pstride, w, nwaves, lane);
+  stats_partials<N, NC>(st, partials, pstride, w, nw, lane);
 }
 
 }  // namespace gpk

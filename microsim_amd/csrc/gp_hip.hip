@@ -40,6 +40,7 @@ namespace {
 constexpr int kBX = 64;          /* threads along y (one wavefront) */
 constexpr int kBY = 4;           /* rows per block */
 constexpr int kNStatMax = 3 * GP_MAXP + 3 * GP_MAXNC + 1;
+constexpr int kListBlocks = 2048; /* grid of the list-driven kernels (grid-stride loops), 4 wavefronts each */
 
 char g_create_error[512];
 
@@ -63,8 +64,10 @@ struct gp_handle {
   int64_t nparts_max;       /* rows of the partials buffer */
   int march_seg, march_nstrips, march_nsegs; /* tile grid of the single-pass step (see gp_march.h) */
   int64_t march_nwaves;     /* tiles = wavefronts of k_march and of k_bulk */
-  int cell_seg, cell_nsegs;  /* tile grid of k_cbulk / k_ciface (same strips, shorter segments) */
+  int cell_seg, cell_nsegs;  /* tile grid of k_cbulk (same strips as the march, shorter segments) */
   int64_t cell_nwaves;
+  int32_t *clist, *ilist;    /* delta-phi candidates and interface cells of the current step (cell offsets) */
+  int *lcount;               /* their lengths */
   int *flags[2];            /* per-tile purity flags of the last two steps */
   int8_t *codes[2];         /* phase codes (pure phase b, or -1) of every cell: codes[ccur] of the current state */
   int ccur;
@@ -72,7 +75,7 @@ struct gp_handle {
   int fcur;                 /* flags[fcur] describes the current state */
   bool flags_valid;         /* flags[fcur] is valid for the current state (else no tile is taken as deep) */
   bool use_march;           /* single-pass steps (default) or the two-pass form */
-  bool use_cells;           /* single-pass update by cell class (k_cbulk + k_ciface, default) or by march tiles */
+  bool use_cells;           /* single-pass update by cell class (k_cbulk + k_ilist, default) or by march tiles */
   bool prev_valid;          /* the other buffer holds the state before the last step */
   Maps maps;
   bool pow2;
@@ -411,7 +414,7 @@ static int launch_step_p(gp_handle *h, int mode) {
     return GP_OK;
   }
   if (mode == MODE_MARCH || mode == MODE_MARCH_STATS) {
-    /* delta-phi of the candidate cells, then the update: by cell class (k_cbulk + k_ciface) or by march tiles
+    /* delta-phi of the candidate cells, then the update: by cell class (k_cbulk + k_ilist) or by march tiles
      * (k_march near an interface, k_bulk on deep tiles). The update kernels read the current state and its phase
      * codes and write disjoint cells of the other buffers (state and codes). */
     const int nt = (int)h->march_nwaves;
@@ -427,37 +430,48 @@ static int launch_step_p(gp_handle *h, int mode) {
                          0, h->stream, phi, h->codes[h->ccur], h->fs, h->ps, h->rx, h->ry);
     }
     {
-      /* tiles of 64 y-cells by kDTileRows rows over x, y in [2, mesh+3] */
+      /* tiles of 64 y-cells by kDTileRows rows over x, y in [2, mesh+3]; the cell-classified path lists the
+       * candidates and evaluates them with full wavefronts (k_dlist) */
       const int nty = (int)((my + 2 + 63) / 64), ntx = (int)((mx + 2 + kDTileRows - 1) / kDTileRows);
       ProfScope ps(h, GP_KERNEL_DPHI);
-      hipLaunchKernelGGL((k_dphi<N, NC, R, P2>), dim3((unsigned)((nty * ntx + 3) / 4)), dim3(256), 0, h->stream, m,
-                         phi, mu, codes, dphi, h->fs, h->ps, mx, my, nty, nty * ntx);
+      if (h->use_cells) {
+        HIPCHK(h, hipMemsetAsync(h->lcount, 0, 2 * sizeof(int), h->stream));
+        hipLaunchKernelGGL((k_dphi<N, NC, R, P2>), dim3((unsigned)((nty * ntx + 3) / 4)), dim3(256), 0, h->stream, m,
+                           phi, mu, codes, dphi, h->fs, h->ps, mx, my, nty, nty * ntx, h->clist, h->lcount);
+        hipLaunchKernelGGL((k_dlist<N, NC, R, P2>), dim3(kListBlocks), dim3(256), 0, h->stream, m, phi, mu, dphi,
+                           h->fs, h->ps, (const int32_t *)h->clist, (const int *)h->lcount);
+      } else {
+        hipLaunchKernelGGL((k_dphi<N, NC, R, P2>), dim3((unsigned)((nty * ntx + 3) / 4)), dim3(256), 0, h->stream, m,
+                           phi, mu, codes, dphi, h->fs, h->ps, mx, my, nty, nty * ntx, (int32_t *)nullptr,
+                           (int *)nullptr);
+      }
     }
     if (h->use_cells) {
+      /* statistics partials: k_cbulk's wavefronts, then k_ilist's */
       const int ct = (int)h->cell_nwaves;
       const unsigned cblk = (unsigned)((ct + 3) / 4);
-      const int64_t cstride = 2 * (int64_t)ct;
+      const int64_t cstride = (int64_t)ct + 4 * kListBlocks;
       {
         ProfScope ps(h, GP_KERNEL_CBULK);
         if (stats)
           hipLaunchKernelGGL((k_cbulk<N, NC, R, P2, true>), dim3(cblk), dim3(256), 0, h->stream, m, mu, codes, phi_o,
                              mu_o, codes_o, h->maps, h->partials, cstride, h->fs, h->ps, mx, my, h->cell_seg,
-                             h->march_nstrips, ct, (int)h->prev_valid);
+                             h->march_nstrips, ct, (int)h->prev_valid, h->ilist, h->lcount + 1);
         else
           hipLaunchKernelGGL((k_cbulk<N, NC, R, P2, false>), dim3(cblk), dim3(256), 0, h->stream, m, mu, codes,
                              phi_o, mu_o, codes_o, h->maps, h->partials, cstride, h->fs, h->ps, mx, my, h->cell_seg,
-                             h->march_nstrips, ct, (int)h->prev_valid);
+                             h->march_nstrips, ct, (int)h->prev_valid, h->ilist, h->lcount + 1);
       }
       {
         ProfScope ps(h, GP_KERNEL_CIFACE);
         if (stats)
-          hipLaunchKernelGGL((k_ciface<N, NC, R, P2, true>), dim3(cblk), dim3(256), 0, h->stream, m, phi, mu, dphi,
-                             codes, phi_o, mu_o, codes_o, h->maps, h->partials + ct, cstride, h->fs, h->ps, mx, my,
-                             h->cell_seg, h->march_nstrips, ct);
+          hipLaunchKernelGGL((k_ilist<N, NC, R, P2, true>), dim3(kListBlocks), dim3(256), 0, h->stream, m, phi, mu,
+                             dphi, codes, phi_o, mu_o, codes_o, h->maps, h->partials + ct, cstride, h->fs, h->ps, mx,
+                             my, (const int32_t *)h->ilist, (const int *)h->lcount + 1);
         else
-          hipLaunchKernelGGL((k_ciface<N, NC, R, P2, false>), dim3(cblk), dim3(256), 0, h->stream, m, phi, mu, dphi,
-                             codes, phi_o, mu_o, codes_o, h->maps, h->partials + ct, cstride, h->fs, h->ps, mx, my,
-                             h->cell_seg, h->march_nstrips, ct);
+          hipLaunchKernelGGL((k_ilist<N, NC, R, P2, false>), dim3(kListBlocks), dim3(256), 0, h->stream, m, phi, mu,
+                             dphi, codes, phi_o, mu_o, codes_o, h->maps, h->partials + ct, cstride, h->fs, h->ps, mx,
+                             my, (const int32_t *)h->ilist, (const int *)h->lcount + 1);
       }
       if (stats) {
         ProfScope ps(h, GP_KERNEL_STATS);
@@ -852,7 +866,7 @@ int gp_create(const gp_params *params, int precision, gp_handle **out) {
   }
   h->nblocks_mu = ((params->mesh_y + kBX - 1) / kBX) * ((params->mesh_x + kBY - 1) / kBY);
   march_plan(h);
-  h->nparts_max = std::max(h->nblocks_mu, 2 * std::max(h->march_nwaves, h->cell_nwaves)); /* two kernels' partials */
+  h->nparts_max = std::max(h->nblocks_mu, std::max(2 * h->march_nwaves, h->cell_nwaves + 4 * (int64_t)kListBlocks));
   const char *path = getenv("GP_PATH");
   h->use_march = !(path && strcmp(path, "twopass") == 0);
   h->use_cells = h->use_march && !(path && strcmp(path, "march") == 0);
@@ -872,6 +886,9 @@ int gp_create(const gp_params *params, int precision, gp_handle **out) {
   ok = ok && hipMalloc((void **)&h->inv_dev, sizeof(double) * (GP_MAXNC + 1)) == hipSuccess;
   ok = ok && hipMalloc((void **)&h->flags[0], sizeof(int) * h->march_nwaves) == hipSuccess;
   ok = ok && hipMalloc((void **)&h->flags[1], sizeof(int) * h->march_nwaves) == hipSuccess;
+  ok = ok && hipMalloc((void **)&h->clist, (size_t)h->fs * sizeof(int32_t)) == hipSuccess &&
+       hipMalloc((void **)&h->ilist, (size_t)h->fs * sizeof(int32_t)) == hipSuccess &&
+       hipMalloc((void **)&h->lcount, 2 * sizeof(int)) == hipSuccess;
   ok = ok && hipMalloc((void **)&h->codes[0], (size_t)h->rx * h->ps) == hipSuccess &&
        hipMalloc((void **)&h->codes[1], (size_t)h->rx * h->ps) == hipSuccess;
   h->ccur = 0;
@@ -922,7 +939,7 @@ void gp_destroy(gp_handle *h) {
   h->prof.clear();
   delete h->adj;
   void *bufs[] = {h->phi[0], h->phi[1], h->mu[0], h->mu[1], h->dphi, h->partials, h->stats_dev, h->model,
-                  h->flags[0], h->flags[1], h->inv_dev, h->codes[0], h->codes[1]};
+                  h->flags[0], h->flags[1], h->inv_dev, h->codes[0], h->codes[1], h->clist, h->ilist, h->lcount};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -1155,7 +1172,7 @@ int gp_profile_enable(gp_handle *h, int enable) {
 
 const char *gp_kernel_name(int kind) {
   static const char *names[GP_NKERNELS] = {"k_phi", "k_mu", "k_stats", "k_march", "k_march_stats", "k_bulk",
-                                           "k_dphi", "adj_step", "k_cbulk", "k_ciface"};
+                                           "k_dphi", "adj_step", "k_cbulk", "k_ilist"};
   return (kind >= 0 && kind < GP_NKERNELS) ? names[kind] : "?";
 }
 

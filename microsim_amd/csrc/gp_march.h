@@ -842,11 +842,31 @@ __global__ __launch_bounds__(256) void k_codes(const R *__restrict__ phi, int8_t
  * projection) for 64 candidates at a time, so every lane works on an interface cell. */
 constexpr int kDTileRows = 16;
 
+/* phi on the 3x3 stencil and mu at the centre gathered once into registers, then phi_delta */
+template <int N, int NC, typename R, bool P2>
+__device__ __forceinline__ void dphi_cell(const Model<R> &m, const R *__restrict__ phi, const R *__restrict__ mu,
+                                          R *__restrict__ dphi, int64_t fs, int64_t ps, int64_t c) {
+  RegStore<N, NC, R> rs;
+#pragma unroll
+  for (int a = 0; a < N; a++)
+#pragma unroll
+    for (int di = 0; di < 3; di++)
+#pragma unroll
+      for (int dj = 0; dj < 3; dj++) rs.p[a][di][dj] = phi[a * fs + c + (di - 1) * ps + (dj - 1)];
+#pragma unroll
+  for (int k = 0; k < NC; k++) rs.u[k][0] = mu[k * fs + c];
+  const RegView<N, NC, R, P2> v{&rs, 0, 0};
+  R d[N];
+  phi_delta<N, NC>(m, v, false, d);
+#pragma unroll
+  for (int a = 0; a < N; a++) dphi[a * fs + c] = d[a];
+}
+
 template <int N, int NC, typename R, bool P2>
 __global__ __launch_bounds__(256) void k_dphi(const Model<R> *__restrict__ mp, const R *__restrict__ phi,
                                               const R *__restrict__ mu, const int8_t *__restrict__ codes,
                                               R *__restrict__ dphi, int64_t fs, int64_t ps, int64_t mx, int64_t my,
-                                              int nty, int ntiles) {
+                                              int nty, int ntiles, int32_t *__restrict__ clist, int *ccount) {
   __shared__ uint16_t list[4][kDTileRows * 64];
   /* the tile's codes, rows x0-1 .. x1+1, columns y-1 .. y+64 (slot j+1 holds y0+j), staged with one batch of
    * loads so that the candidate scan does not wait on memory row by row */
@@ -889,24 +909,34 @@ __global__ __launch_bounds__(256) void k_dphi(const Model<R> *__restrict__ mp, c
     }
   }
   __syncthreads();
+  if (clist) {
+    /* list mode: reserve the tile's candidates in the global list (one atomic per wavefront); k_dlist evaluates
+     * them with full wavefronts */
+    if (count > 0) {
+      int base = 0;
+      if (lane == 0) base = atomicAdd(ccount, count);
+      base = __shfl(base, 0);
+      for (int i = lane; i < count; i += 64) {
+        const int r = list[wv][i];
+        clist[base + i] = (int32_t)((x0 + r / 64) * ps + (2 + (int64_t)ty * 64 + r % 64) + kYOff);
+      }
+    }
+    return;
+  }
   for (int i = lane; i < count; i += 64) {
     const int r = list[wv][i];
     const int64_t c = (x0 + r / 64) * ps + (2 + (int64_t)ty * 64 + r % 64) + kYOff;
-    /* phi on the 3x3 stencil and mu at the centre, gathered once into registers */
-    RegStore<N, NC, R> rs;
-#pragma unroll
-    for (int a = 0; a < N; a++)
-#pragma unroll
-      for (int di = 0; di < 3; di++)
-#pragma unroll
-        for (int dj = 0; dj < 3; dj++) rs.p[a][di][dj] = phi[a * fs + c + (di - 1) * ps + (dj - 1)];
-#pragma unroll
-    for (int k = 0; k < NC; k++) rs.u[k][0] = mu[k * fs + c];
-    const RegView<N, NC, R, P2> v{&rs, 0, 0};
-    R d[N];
-    phi_delta<N, NC>(m, v, false, d);
-#pragma unroll
-    for (int a = 0; a < N; a++) dphi[a * fs + c] = d[a];
+    dphi_cell<N, NC, R, P2>(m, phi, mu, dphi, fs, ps, c);
   }
+}
+
+/* delta-phi of the listed candidate cells: a grid-stride loop of full wavefronts */
+template <int N, int NC, typename R, bool P2>
+__global__ __launch_bounds__(256) void k_dlist(const Model<R> *__restrict__ mp, const R *__restrict__ phi,
+                                               const R *__restrict__ mu, R *__restrict__ dphi, int64_t fs, int64_t ps,
+                                               const int32_t *__restrict__ clist, const int *__restrict__ ccount) {
+  const int n = *ccount;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    dphi_cell<N, NC, R, P2>(*mp, phi, mu, dphi, fs, ps, clist[i]);
 }
 }  // namespace gpk
