@@ -161,8 +161,20 @@ __device__ __forceinline__ void cbulk_cell(const Model<R> &m, int b, int64_t x, 
     write_images(mu_o, NC, munew, 1, maps, x, t.y, fs, ps);
   }
   if (STATS) {
-    const R z[N] = {};
-    stats_add<N, NC>(st, pe, z, munew, deltamu);
+    /* stats_add with delta-phi = 0 and phi = e_b: the squared increments of phi add +0 to sums that start at +0
+     * and e_b is finite, so only the extrema of phi and the mu terms remain */
+#pragma unroll
+    for (int a = 0; a < N; a++) {
+      st[a] = fmax(st[a], pe[a]);
+      st[N + a] = fmin(st[N + a], pe[a]);
+    }
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+      st[2 * N + k] = fmax(st[2 * N + k], munew[k]);
+      st[2 * N + NC + k] = fmin(st[2 * N + NC + k], munew[k]);
+      st[3 * N + 2 * NC + k] += deltamu[k] * deltamu[k];
+      if (!isfinite(munew[k])) st[NSA - 1] += R(1.0);
+    }
   }
 }
 
@@ -174,14 +186,16 @@ __global__ __launch_bounds__(256) void k_cbulk(const Model<R> *__restrict__ mp, 
                                                R *__restrict__ mu_o, int8_t *__restrict__ codes_out, Maps maps,
                                                double *__restrict__ partials, int64_t pstride, int64_t fs, int64_t ps,
                                                int64_t mx, int64_t my, int seg_len, int nstrips, int nwaves,
-                                               int keep, int32_t *__restrict__ ilist, int *icount) {
+                                               int keep, int32_t *__restrict__ ilist, int32_t *__restrict__ clist,
+                                               int *counts) {
   using Vw = View<R, P2>;
   constexpr int NS = 3 * N + 3 * NC + 1;
   constexpr int NSA = STATS ? NS : 1;
   __shared__ int8_t cs_all[4][kCellRows * 64];
-  __shared__ uint16_t il_all[4][kCellSegMax * 64];
-  uint16_t *il = il_all[threadIdx.x >> 6];
-  int icnt = 0;
+  __shared__ uint16_t il_all[4][kCellSegMax * 64], cl_all[4][kCellSegMax * 64];
+  __shared__ int wcnt[2][4], wbase[2][4];
+  uint16_t *il = il_all[threadIdx.x >> 6], *cl = cl_all[threadIdx.x >> 6];
+  int icnt = 0, ccnt = 0;
   const Model<R> &m = *mp;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int w = blockIdx.x * 4 + wv;
@@ -192,6 +206,16 @@ __global__ __launch_bounds__(256) void k_cbulk(const Model<R> *__restrict__ mp, 
   if (w < nwaves) stage_codes(cs, codes_in, t, ps, lane);
   __syncthreads();
   if (w < nwaves) {
+    /* a tile whose staged codes are all one phase b has only bulk cells of phase b: no per-cell test */
+    int tb = -1;
+    {
+      const int u = cs[lane];
+      bool same = u >= 0;
+      const int nr = (int)(t.xb - t.xa) + 5;
+      for (int r = 1; r < nr; r++) same = same && cs[r * 64 + lane] == u;
+      const int u0 = __builtin_amdgcn_readfirstlane(u);
+      if (__all(same && u == u0)) tb = u0;
+    }
     const R zj = R(0.0) * m.jat_fac; /* (j - j_prev) * jat_fac with both currents +0 */
     R divjat0 = divx<Vw>(m, zj);
     divjat0 += divy<Vw>(m, zj);
@@ -226,13 +250,32 @@ __global__ __launch_bounds__(256) void k_cbulk(const Model<R> *__restrict__ mp, 
         mud[k] = lane_dn(m1[k]);
         muu[k] = lane_up(m1[k]);
       }
+      if (tb >= 0) {
+        if (t.out_lane)
+          cbulk_cell<N, NC, R, P2, STATS, true>(m, tb, x, t, m0, m1, m2, mud, muu, divjat0, phi_o, mu_o, codes_out,
+                                                maps, fs, ps, mx, keep, st);
+#pragma unroll
+        for (int k = 0; k < NC; k++) {
+          m0[k] = m1[k];
+          m1[k] = m2[k];
+        }
+        continue;
+      }
       const int bl = t.out_lane ? bulk_of(cs, (int)(x - t.xa) + 2, lane) : -1;
       {
-        /* the row's interface cells, for k_ilist */
+        /* the row's interface cells (for k_ilist) and delta-phi candidates (for k_dlist) */
+        const int r = (int)(x - t.xa) + 2;
+        const int8_t k0 = cs[r * 64 + lane];
+        const bool cand = t.out_lane && (k0 < 0 || cs[(r - 1) * 64 + lane] != k0 || cs[(r + 1) * 64 + lane] != k0 ||
+                                         cs[r * 64 + lane - 1] != k0 || cs[r * 64 + lane + 1] != k0);
         const bool iface = t.out_lane && bl < 0;
-        const uint64_t mask = __ballot(iface);
-        if (iface) il[icnt + __popcll(mask & ((1ull << lane) - 1ull))] = (uint16_t)((x - t.xa) * 64 + lane);
-        icnt += __popcll(mask);
+        const uint16_t e = (uint16_t)((x - t.xa) * 64 + lane);
+        const uint64_t lt = (1ull << lane) - 1ull;
+        const uint64_t mi = __ballot(iface), mc = __ballot(cand);
+        if (iface) il[icnt + __popcll(mi & lt)] = e;
+        if (cand) cl[ccnt + __popcll(mc & lt)] = e;
+        icnt += __popcll(mi);
+        ccnt += __popcll(mc);
       }
       /* a row whose output lanes are all bulk in one phase (the common case) takes the phase as a scalar */
       const int b0 = __builtin_amdgcn_readlane(bl, 2);
@@ -249,19 +292,65 @@ __global__ __launch_bounds__(256) void k_cbulk(const Model<R> *__restrict__ mp, 
         m1[k] = m2[k];
       }
     }
-    /* one atomic per wavefront: reserve the tile's interface cells in the global list */
-    if (icnt > 0) {
-      int base = 0;
-      if (lane == 0) base = atomicAdd(icount, icnt);
-      base = __shfl(base, 0);
-      for (int i = lane; i < icnt; i += 64) {
-        const int e = il[i];
-        ilist[base + i] = (int32_t)((t.xa + (e >> 6)) * ps + (t.y - lane + (e & 63)) + kYOff);
+  }
+  /* one atomic per block and list: reserve the block's interface cells and candidates in the global lists */
+  if (lane == 0) {
+    wcnt[0][wv] = icnt;
+    wcnt[1][wv] = ccnt;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int l = 0; l < 2; l++) {
+      int tot = 0;
+      for (int v = 0; v < 4; v++) {
+        wbase[l][v] = tot;
+        tot += wcnt[l][v];
       }
+      const int b = tot > 0 ? atomicAdd(&counts[l], tot) : 0;
+      for (int v = 0; v < 4; v++) wbase[l][v] += b;
     }
+  __syncthreads();
+  for (int i = lane; i < icnt; i += 64) {
+    const int e = il[i];
+    ilist[wbase[0][wv] + i] = (int32_t)((t.xa + (e >> 6)) * ps + (t.y - lane + (e & 63)) + kYOff);
+  }
+  for (int i = lane; i < ccnt; i += 64) {
+    const int e = cl[i];
+    clist[wbase[1][wv] + i] = (int32_t)((t.xa + (e >> 6)) * ps + (t.y - lane + (e & 63)) + kYOff);
   }
   if (!STATS) return;
   stats_partials<N, NC>(st, partials, pstride, w, nwaves, lane);
+}
+
+/* Delta-phi candidates on the ring x or y in {2, mesh+3} around the interior (k_cbulk lists the interior ones):
+ * one thread per ring cell, appended to the same list. */
+__global__ __launch_bounds__(256) void k_dring(const int8_t *__restrict__ codes, int32_t *__restrict__ clist,
+                                               int *ccount, int64_t ps, int64_t mx, int64_t my) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int64_t nxr = my + 2, nyr = mx; /* rows x = 2, mx+3 over y in [2, my+3]; columns y = 2, my+3 over x in [3, mx+2] */
+  int64_t x = -1, y = -1;
+  if (i < 2 * nxr) {
+    x = (i < nxr) ? 2 : mx + 3;
+    y = 2 + (i % nxr);
+  } else if (i < 2 * nxr + 2 * nyr) {
+    const int64_t k = i - 2 * nxr;
+    y = (k < nyr) ? 2 : my + 3;
+    x = 3 + (k % nyr);
+  }
+  bool cand = false;
+  const int64_t c = x * ps + y + kYOff;
+  if (x >= 0) {
+    const int8_t k0 = codes[c];
+    cand = k0 < 0 || codes[c - ps] != k0 || codes[c + ps] != k0 || codes[c - 1] != k0 || codes[c + 1] != k0;
+  }
+  const uint64_t mask = __ballot(cand);
+  if (!mask) return;
+  const int leader = __ffsll((long long)mask) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(ccount, __popcll(mask));
+  base = __shfl(base, leader);
+  if (cand) clist[base + __popcll(mask & ((1ull << lane) - 1ull))] = (int32_t)c;
 }
 
 /* The general update of the interface cells listed by k_cbulk (any order): a grid-stride loop of full

@@ -429,49 +429,49 @@ static int launch_step_p(gp_handle *h, int mode) {
       hipLaunchKernelGGL((k_codes<N, R>), dim3((unsigned)((h->ry + 63) / 64), (unsigned)((h->rx + 3) / 4)), dim3(256),
                          0, h->stream, phi, h->codes[h->ccur], h->fs, h->ps, h->rx, h->ry);
     }
-    {
-      /* tiles of 64 y-cells by kDTileRows rows over x, y in [2, mesh+3]; the cell-classified path lists the
-       * candidates and evaluates them with full wavefronts (k_dlist) */
+    if (!h->use_cells) {
+      /* delta-phi of the candidate cells, tiles of 64 y-cells by kDTileRows rows over x, y in [2, mesh+3] */
       const int nty = (int)((my + 2 + 63) / 64), ntx = (int)((mx + 2 + kDTileRows - 1) / kDTileRows);
       ProfScope ps(h, GP_KERNEL_DPHI);
-      if (h->use_cells) {
-        HIPCHK(h, hipMemsetAsync(h->lcount, 0, 2 * sizeof(int), h->stream));
-        hipLaunchKernelGGL((k_dphi<N, NC, R, P2>), dim3((unsigned)((nty * ntx + 3) / 4)), dim3(256), 0, h->stream, m,
-                           phi, mu, codes, dphi, h->fs, h->ps, mx, my, nty, nty * ntx, h->clist, h->lcount);
-        hipLaunchKernelGGL((k_dlist<N, NC, R, P2>), dim3(kListBlocks), dim3(256), 0, h->stream, m, phi, mu, dphi,
-                           h->fs, h->ps, (const int32_t *)h->clist, (const int *)h->lcount);
-      } else {
-        hipLaunchKernelGGL((k_dphi<N, NC, R, P2>), dim3((unsigned)((nty * ntx + 3) / 4)), dim3(256), 0, h->stream, m,
-                           phi, mu, codes, dphi, h->fs, h->ps, mx, my, nty, nty * ntx, (int32_t *)nullptr,
-                           (int *)nullptr);
-      }
+      hipLaunchKernelGGL((k_dphi<N, NC, R, P2>), dim3((unsigned)((nty * ntx + 3) / 4)), dim3(256), 0, h->stream, m,
+                         phi, mu, codes, dphi, h->fs, h->ps, mx, my, nty, nty * ntx);
     }
     if (h->use_cells) {
       /* statistics partials: k_cbulk's wavefronts, then k_ilist's */
       const int ct = (int)h->cell_nwaves;
       const unsigned cblk = (unsigned)((ct + 3) / 4);
       const int64_t cstride = (int64_t)ct + 4 * kListBlocks;
+      HIPCHK(h, hipMemsetAsync(h->lcount, 0, 2 * sizeof(int), h->stream));
       {
         ProfScope ps(h, GP_KERNEL_CBULK);
         if (stats)
           hipLaunchKernelGGL((k_cbulk<N, NC, R, P2, true>), dim3(cblk), dim3(256), 0, h->stream, m, mu, codes, phi_o,
                              mu_o, codes_o, h->maps, h->partials, cstride, h->fs, h->ps, mx, my, h->cell_seg,
-                             h->march_nstrips, ct, (int)h->prev_valid, h->ilist, h->lcount + 1);
+                             h->march_nstrips, ct, (int)h->prev_valid, h->ilist, h->clist, h->lcount);
         else
           hipLaunchKernelGGL((k_cbulk<N, NC, R, P2, false>), dim3(cblk), dim3(256), 0, h->stream, m, mu, codes,
                              phi_o, mu_o, codes_o, h->maps, h->partials, cstride, h->fs, h->ps, mx, my, h->cell_seg,
-                             h->march_nstrips, ct, (int)h->prev_valid, h->ilist, h->lcount + 1);
+                             h->march_nstrips, ct, (int)h->prev_valid, h->ilist, h->clist, h->lcount);
+      }
+      {
+        /* delta-phi of the listed candidates (the ring's added first), full wavefronts */
+        ProfScope ps(h, GP_KERNEL_DPHI);
+        const int64_t nring = 2 * (my + 2) + 2 * mx;
+        hipLaunchKernelGGL(k_dring, dim3((unsigned)((nring + 255) / 256)), dim3(256), 0, h->stream, codes, h->clist,
+                           h->lcount + 1, h->ps, mx, my);
+        hipLaunchKernelGGL((k_dlist<N, NC, R, P2>), dim3(kListBlocks), dim3(256), 0, h->stream, m, phi, mu, dphi,
+                           h->fs, h->ps, (const int32_t *)h->clist, (const int *)h->lcount + 1);
       }
       {
         ProfScope ps(h, GP_KERNEL_CIFACE);
         if (stats)
           hipLaunchKernelGGL((k_ilist<N, NC, R, P2, true>), dim3(kListBlocks), dim3(256), 0, h->stream, m, phi, mu,
                              dphi, codes, phi_o, mu_o, codes_o, h->maps, h->partials + ct, cstride, h->fs, h->ps, mx,
-                             my, (const int32_t *)h->ilist, (const int *)h->lcount + 1);
+                             my, (const int32_t *)h->ilist, (const int *)h->lcount);
         else
           hipLaunchKernelGGL((k_ilist<N, NC, R, P2, false>), dim3(kListBlocks), dim3(256), 0, h->stream, m, phi, mu,
                              dphi, codes, phi_o, mu_o, codes_o, h->maps, h->partials + ct, cstride, h->fs, h->ps, mx,
-                             my, (const int32_t *)h->ilist, (const int *)h->lcount + 1);
+                             my, (const int32_t *)h->ilist, (const int *)h->lcount);
       }
       if (stats) {
         ProfScope ps(h, GP_KERNEL_STATS);

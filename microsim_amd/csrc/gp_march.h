@@ -866,7 +866,7 @@ template <int N, int NC, typename R, bool P2>
 __global__ __launch_bounds__(256) void k_dphi(const Model<R> *__restrict__ mp, const R *__restrict__ phi,
                                               const R *__restrict__ mu, const int8_t *__restrict__ codes,
                                               R *__restrict__ dphi, int64_t fs, int64_t ps, int64_t mx, int64_t my,
-                                              int nty, int ntiles, int32_t *__restrict__ clist, int *ccount) {
+                                              int nty, int ntiles) {
   __shared__ uint16_t list[4][kDTileRows * 64];
   /* the tile's codes, rows x0-1 .. x1+1, columns y-1 .. y+64 (slot j+1 holds y0+j), staged with one batch of
    * loads so that the candidate scan does not wait on memory row by row */
@@ -909,20 +909,6 @@ __global__ __launch_bounds__(256) void k_dphi(const Model<R> *__restrict__ mp, c
     }
   }
   __syncthreads();
-  if (clist) {
-    /* list mode: reserve the tile's candidates in the global list (one atomic per wavefront); k_dlist evaluates
-     * them with full wavefronts */
-    if (count > 0) {
-      int base = 0;
-      if (lane == 0) base = atomicAdd(ccount, count);
-      base = __shfl(base, 0);
-      for (int i = lane; i < count; i += 64) {
-        const int r = list[wv][i];
-        clist[base + i] = (int32_t)((x0 + r / 64) * ps + (2 + (int64_t)ty * 64 + r % 64) + kYOff);
-      }
-    }
-    return;
-  }
   for (int i = lane; i < count; i += 64) {
     const int r = list[wv][i];
     const int64_t c = (x0 + r / 64) * ps + (2 + (int64_t)ty * 64 + r % 64) + kYOff;
