@@ -9,7 +9,8 @@
  *
  * k_cbulk updates the bulk cells of a tile (60-cell y-strip x seg_len x-rows, one wavefront, as the march tiles):
  * it streams the mu rows with a deep prefetch and forms, per lane, the same operations in the same order as the
- * pure-phase step and k_bulk, so the results are bit-identical. phi of a bulk cell is e_b; when the output buffer
+ * pure-phase step and k_bulk, so the results are bit-identical. A tile whose staged codes are all one phase takes
+ * that phase as a scalar and skips the per-cell classification. phi of a bulk cell is e_b; when the output buffer
  * already holds e_b there (the cell was purely b two states back) it is not rewritten.
  * k_cbulk also lists the interface cells of its tile (LDS, then one atomic per wavefront into a global list);
  * k_ilist runs the general per-cell update (gp_model.h: jat_x, jat_y, mu_update -- bit-identical to the
@@ -277,13 +278,7 @@ __global__ __launch_bounds__(256) void k_cbulk(const Model<R> *__restrict__ mp, 
         icnt += __popcll(mi);
         ccnt += __popcll(mc);
       }
-      /* a row whose output lanes are all bulk in one phase (the common case) takes the phase as a scalar */
-      const int b0 = __builtin_amdgcn_readlane(bl, 2);
-      if (b0 >= 0 && __all(!t.out_lane || bl == b0)) {
-        if (t.out_lane)
-          cbulk_cell<N, NC, R, P2, STATS, true>(m, b0, x, t, m0, m1, m2, mud, muu, divjat0, phi_o, mu_o, codes_out,
-                                                maps, fs, ps, mx, keep, st);
-      } else if (bl >= 0)
+      if (bl >= 0)
         cbulk_cell<N, NC, R, P2, STATS, false>(m, bl, x, t, m0, m1, m2, mud, muu, divjat0, phi_o, mu_o, codes_out,
                                                maps, fs, ps, mx, keep, st);
 #pragma unroll
