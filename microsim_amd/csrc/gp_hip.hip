@@ -6,13 +6,14 @@
  *   0..mesh_y+5 starting at element kYOff, so the first interior cell y = 3 sits on a 64-element boundary;
  *   the row pitch ps is a multiple of 64 elements. phi (N planes) and mu (NC planes) are ping-ponged,
  *   dphi (N planes) holds delta-phi when it is asked for.
- * A time step is a single pass over the state in tiles (gp_march.h): k_march updates the tiles near an
- * interface -- it reads phi and mu once, recomputes delta-phi and the anti-trapping currents on the fly and
- * writes the new phi and mu (and the halo images of boundary cells) -- and k_bulk the tiles whose neighbourhood
- * was purely one phase (from the tile flags both kernels write for the next step); the statistics fold k_stats
- * follows when statistics are asked for.
- * The two-pass form (k_phi: delta-phi on x,y in [2, mesh+3]; k_mu: currents and update on the interior) runs
- * the smoothing steps, recomputes delta-phi for gp_download, and is selectable for steps with GP_PATH=twopass.
+ * A time step (default, gp_cells.h) classifies cells by the phase codes of the current state: k_cbulk updates the
+ * bulk cells (streaming mu; phi stays e_b) and lists the interface cells and the delta-phi candidates; k_dring adds
+ * the candidates of the boundary ring; k_dlist evaluates delta-phi of the listed candidates and k_ilist the
+ * general update of the listed interface cells, both with full wavefronts; k_stats folds the statistics when they
+ * are asked for. Every kernel writes the halo images of the boundary cells it updates.
+ * GP_PATH=march selects the tiled march (gp_march.h: k_dphi, then k_march near an interface and k_bulk on deep
+ * tiles); GP_PATH=twopass the two-pass form (k_phi: delta-phi on x,y in [2, mesh+3]; k_mu: currents and update
+ * on the interior), which also runs the smoothing steps and recomputes delta-phi for gp_download.
  * Reference routines replaced: see include/gp_hip.h.
  */
 #include <hip/hip_runtime.h>
@@ -1089,7 +1090,7 @@ int gp_adjoint_forward(gp_handle *h, int64_t nsteps, int64_t every) {
   if (!h || nsteps < 0) return h ? fail(h, GP_E_ARG, "bad nsteps") : GP_E_ARG;
   if (h->precision != GP_FP64) return fail(h, GP_E_ARG, "the adjoint is built for the FP64 mode");
   if (!pick_adj(h)) return fail(h, GP_E_ARG, "no adjoint instantiated for %d phases / %d components", h->N, h->NC);
-  if (!h->use_march) return fail(h, GP_E_ARG, "the adjoint records single-pass steps (unset GP_PATH)");
+  if (!h->use_march) return fail(h, GP_E_ARG, "the adjoint records single-pass steps (GP_PATH=twopass is set)");
   if (h->fs > 0x7fffffff) return fail(h, GP_E_ARG, "the adjoint indexes cells with 32 bits: grid too large");
   delete h->adj;
   h->adj = new Adjoint();
