@@ -150,9 +150,9 @@ def run_mode(case, precision: int, steps: int, warmup: int):
     }
 
 
-def adjoint_ratio(params, phi0, mu0, steps: int, every: int = 0):
-    """Cost of the FP64 adjoint against the forward on the bench workload. `steps` plain FP64 steps are timed,
-    then the same steps recorded for the adjoint (checkpoints every about sqrt(steps)) and the reverse sweep for a
+def adjoint_ratio(params, phi0, mu0, steps: int, every: int = 0, precision: int = None):
+    """Cost of the adjoint against the forward of the same precision (FP64 by default). `steps` plain steps are
+    timed, then the same steps recorded for the adjoint (checkpoints every about sqrt(steps)) and the reverse sweep for a
     random linear loss of the final state. From HIP events on the solver's stream: the forward step (its kernels),
     the transposed step (reload of the recomputed state, delta-phi, the adjoint kernels) and the segment
     recompute per step; the adjoint step is the transposed step plus the recompute share. Wall-clock times
@@ -162,7 +162,7 @@ def adjoint_ratio(params, phi0, mu0, steps: int, every: int = 0):
     from microsim_amd.solver import GPSolver
     rng = np.random.default_rng(5)
     wphi, wmu = rng.standard_normal(phi0.shape), rng.standard_normal(mu0.shape)
-    with GPSolver(params, _abi.GP_FP64) as s:
+    with GPSolver(params, _abi.GP_FP64 if precision is None else precision) as s:
         s.upload(phi0, mu0)
         s.step(2)
         s.upload(phi0, mu0)
@@ -365,6 +365,11 @@ def main():
         out["adjoint_binary256"]["workload"] = ("binary anisotropic seed (radius 60) 256x256, 2000 steps, "
                                                 "checkpoints every 100, FP64")
         log("adjoint binary 256", json.dumps(out["adjoint_binary256"]))
+        out["adjoint_binary256_fp32"] = adjoint_ratio(bcase.params, bcase.phi, bcase.mu, 2000, 100, _abi.GP_FP32)
+        out["adjoint_binary256_fp32"]["workload"] = ("binary anisotropic seed (radius 60) 256x256, 2000 steps, "
+                                                     "checkpoints every 100, FP32 trajectory, FP64 Jacobians and "
+                                                     "gradient accumulation")
+        log("adjoint binary 256 fp32", json.dumps(out["adjoint_binary256_fp32"]))
     if a.error_steps > 0:
         errs = [precision_errors(case, a.error_steps)]
         if a.mesh != 1024:

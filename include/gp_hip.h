@@ -190,7 +190,7 @@ int gp_profile_enable(gp_handle *h, int enable);
 int gp_profile_read(gp_handle *h, double *avg_ms, int64_t *count);
 const char *gp_kernel_name(int kind);
 
-/* Adjoint of the FP64 solver loop (no counterpart in the reference; for gradient-based calibration).
+/* Adjoint of the solver loop (no counterpart in the reference; for gradient-based calibration).
  *
  * The differentiable parameters theta are, in this order: for every phase pair a < b (a outer, b inner)
  * dab[a][b], tau_ab[a][b] and Gamma[a][b] (each entering the model symmetrically in a and b); the liquid's
@@ -200,7 +200,8 @@ int gp_theta_count(const gp_handle *h);
 int gp_theta_values(const gp_handle *h, double *theta);
 
 /* nsteps steps from the current state, as gp_step does, keeping a device checkpoint of the state every `every`
- * steps (every <= 0: about sqrt(nsteps)). FP64 handles only. */
+ * steps (every <= 0: about sqrt(nsteps)). Checkpoints hold the handle's precision: an FP32 handle records and
+ * recomputes its FP32 trajectory. */
 int gp_adjoint_forward(gp_handle *h, int64_t nsteps, int64_t every);
 
 /* Reverse sweep of the trajectory recorded by gp_adjoint_forward. Given the gradient of a scalar loss with
@@ -208,7 +209,9 @@ int gp_adjoint_forward(gp_handle *h, int64_t nsteps, int64_t every);
  * with respect to the state passed to gp_upload (gphi_0, gmu_0; the halo entries, which gp_upload overwrites
  * from the interior, get 0) and with respect to theta. Segments between checkpoints are recomputed; each step's
  * local Jacobians are evaluated exactly (forward-mode dual numbers through the step's own per-cell arithmetic)
- * and applied transposed. The handle holds the initial state afterwards. */
+ * and applied transposed. On an FP32 handle the Jacobians are those of the FP64 model at the FP32 states (the
+ * state is widened to FP64 for each transposed step); adjoint fields and parameter gradients are accumulated in
+ * FP64 in both precisions. The handle holds the initial state afterwards. */
 int gp_adjoint_backward(gp_handle *h, const double *gphi_T, const double *gmu_T, double *gphi_0, double *gmu_0,
                         double *gtheta);
 

@@ -1,12 +1,14 @@
-"""Differentiable trajectories of the FP64 solver loop (torch.autograd over libgp_hip's hand-written adjoint).
+"""Differentiable trajectories of the solver loop (torch.autograd over libgp_hip's hand-written adjoint).
 
 ``trajectory(params, theta, phi0, mu0, nsteps)`` runs ``nsteps`` steps of the grand-potential loop
 (``Grand_potential_Finite_difference_2D_serial/microsim_gp.c:102-106`` over ``solverloop/solverloop.h:21-71``:
 the phase-field update, its boundary conditions, the chemical-potential update, its boundary conditions) on the
-GPU in FP64 and returns the final ``(phi, mu)`` as float64 tensors. Gradients flow to ``theta`` (the model
-parameters listed by ``theta_names``), to ``phi0`` and to ``mu0``. The backward pass is the library's reverse sweep: checkpoints every
-``every`` steps, recomputation of each segment, and the transposed stencil of every step with its local Jacobians
-evaluated exactly. There is no CPU path; a missing library or device raises.
+GPU in FP64 (or in the FP32 production mode) and returns the final ``(phi, mu)`` as float64 tensors. Gradients
+flow to ``theta`` (the model parameters listed by ``theta_names``), to ``phi0`` and to ``mu0``. The backward pass
+is the library's reverse sweep: checkpoints every ``every`` steps, recomputation of each segment, and the
+transposed stencil of every step with its local Jacobians evaluated exactly. In FP32 the trajectory, its checkpoints and its recompute are FP32; each transposed step
+evaluates the FP64 model's Jacobians at the FP32 states, and adjoints and parameter gradients accumulate in FP64.
+There is no CPU path; a missing library or device raises.
 
 theta holds the parameters as the solver loop reads them: the pair mobility coefficients ``dab``, the pair
 relaxation times ``tau_ab`` and the interface energies ``Gamma`` (one value per pair a < b, used symmetrically),
@@ -73,9 +75,9 @@ def params_with_theta(params: _abi.GPParams, theta) -> _abi.GPParams:
 
 class _Trajectory(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, theta, phi0, mu0, params, nsteps, every):
+    def forward(ctx, theta, phi0, mu0, params, nsteps, every, precision):
         p = params_with_theta(params, theta.detach().cpu().numpy())
-        s = GPSolver(p, _abi.GP_FP64)
+        s = GPSolver(p, precision)
         s.upload(phi0.detach().cpu().numpy(), mu0.detach().cpu().numpy())
         s.adjoint_forward(int(nsteps), int(every))
         phi, mu = s.download()
@@ -93,15 +95,16 @@ class _Trajectory(torch.autograd.Function):
         g0, m0, th = s.adjoint_backward(gphi, gmu)
         dt, dp, dm = ctx.devices
         return (torch.from_numpy(th).to(dt), torch.from_numpy(g0).to(dp), torch.from_numpy(m0).to(dm),
-                None, None, None)
+                None, None, None, None)
 
 
 def trajectory(params: _abi.GPParams, theta: torch.Tensor, phi0: torch.Tensor, mu0: torch.Tensor, nsteps: int,
-               every: int = 0) -> Tuple[torch.Tensor, torch.Tensor]:
-    """Final (phi, mu) after nsteps FP64 steps from (phi0, mu0) with the parameters params overridden by theta.
+               every: int = 0, precision: int = _abi.GP_FP64) -> Tuple[torch.Tensor, torch.Tensor]:
+    """Final (phi, mu) after nsteps steps from (phi0, mu0) with the parameters params overridden by theta.
 
     phi0 [N, rows_x, rows_y] and mu0 [NC, rows_x, rows_y] are halo-inclusive, as the host stage lays them out;
     their halo entries are overwritten from the interior by the boundary conditions and get zero gradient.
     every: checkpoint interval of the reverse sweep (0: about sqrt(nsteps)); device memory for the adjoint is
-    (nsteps / every + every + 3) copies of the state."""
-    return _Trajectory.apply(theta, phi0, mu0, params, int(nsteps), int(every))
+    (nsteps / every + every + 3) copies of the state (the checkpoints in the handle's precision).
+    precision: _abi.GP_FP64 (default) or _abi.GP_FP32."""
+    return _Trajectory.apply(theta, phi0, mu0, params, int(nsteps), int(every), int(precision))

@@ -682,13 +682,17 @@ struct Adjoint {
   double *g[2] = {nullptr, nullptr};
   double *gdphi = nullptr, *gth = nullptr, *cbar = nullptr;
   int32_t *qlist = nullptr, *plist = nullptr; /* cells of the dual-number stages Q and P */
+  /* FP32 handles: the current state widened to FP64, its delta-phi in FP64 and the FP64 model, at which every
+   * transposed step is evaluated (the FP64 model's derivative at the FP32 trajectory) */
+  double *st64 = nullptr, *dphi64 = nullptr;
+  void *model64 = nullptr;
   int *counts = nullptr;                      /* their lengths */
   void *qmodels = nullptr, *pmodels = nullptr;
   int nq = 0, np = 0;
   ~Adjoint() {
     for (double *b : ckpt) (void)hipFree(b);
     for (double *b : seg) (void)hipFree(b);
-    void *bufs[] = {g[0], g[1], gdphi, gth, cbar, qlist, plist, counts, qmodels, pmodels};
+    void *bufs[] = {g[0], g[1], gdphi, gth, cbar, qlist, plist, counts, qmodels, pmodels, st64, dphi64, model64};
     for (void *b : bufs)
       if (b) (void)hipFree(b);
   }
@@ -698,15 +702,15 @@ static size_t state_elems(const gp_handle *h) { return (size_t)(h->N + h->NC) * 
 
 /* the current state into / from a state buffer laid out phi planes then mu planes */
 static int state_save(gp_handle *h, double *dst) {
-  const size_t pb = (size_t)h->N * h->fs * sizeof(double), mb = (size_t)h->NC * h->fs * sizeof(double);
+  const size_t es = esize(h), pb = (size_t)h->N * h->fs * es, mb = (size_t)h->NC * h->fs * es;
   HIPCHK(h, hipMemcpyAsync(dst, h->phi[h->cur], pb, hipMemcpyDeviceToDevice, h->stream));
-  HIPCHK(h, hipMemcpyAsync(dst + (size_t)h->N * h->fs, h->mu[h->cur], mb, hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync((char *)dst + pb, h->mu[h->cur], mb, hipMemcpyDeviceToDevice, h->stream));
   return GP_OK;
 }
 static int state_load(gp_handle *h, const double *src) {
-  const size_t pb = (size_t)h->N * h->fs * sizeof(double), mb = (size_t)h->NC * h->fs * sizeof(double);
+  const size_t es = esize(h), pb = (size_t)h->N * h->fs * es, mb = (size_t)h->NC * h->fs * es;
   HIPCHK(h, hipMemcpyAsync(h->phi[h->cur], src, pb, hipMemcpyDeviceToDevice, h->stream));
-  HIPCHK(h, hipMemcpyAsync(h->mu[h->cur], src + (size_t)h->N * h->fs, mb, hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->mu[h->cur], (const char *)src + pb, mb, hipMemcpyDeviceToDevice, h->stream));
   h->flags_valid = false;
   h->codes_valid = false;
   h->prev_valid = false;
@@ -760,11 +764,9 @@ static int adj_models(gp_handle *h) {
 /* one adjoint step: the current state is s_t (delta-phi of s_t in h->dphi); A->g[1] holds the adjoint of
  * s_{t+1} and becomes, in A->g[0] (then swapped into g[1]), the adjoint of s_t */
 template <int N, int NC, bool P2>
-static int adj_step_t(gp_handle *h) {
+static int adj_step_t(gp_handle *h, const double *phi, const double *mu, const double *dphi, const void *model) {
   Adjoint *A = h->adj;
   const int64_t mx = h->p.mesh_x, my = h->p.mesh_y, fs = h->fs;
-  const double *phi = (const double *)h->phi[h->cur], *mu = (const double *)h->mu[h->cur];
-  const double *dphi = (const double *)h->dphi;
   double *gn = A->g[1], *gc = A->g[0];
   const dim3 blk(256);
   const dim3 gall((unsigned)((h->ry + 63) / 64), (unsigned)((h->rx + 3) / 4));
@@ -774,7 +776,7 @@ static int adj_step_t(gp_handle *h) {
   HIPCHK(h, hipMemsetAsync(A->gdphi, 0, (size_t)N * fs * sizeof(double), h->stream));
   /* bulk cells of stage Q in closed form (plain stores), then the dual kernels add the rest */
   HIPCHK(h, hipMemsetAsync(A->counts, 0, 2 * sizeof(int), h->stream));
-  const Model<double> *md = (const Model<double> *)h->model;
+  const Model<double> *md = (const Model<double> *)model;
   hipLaunchKernelGGL((k_adj_cbar<N, NC>), gall, blk, 0, h->stream, md, phi, dphi, gn, gn + (size_t)N * fs, A->cbar,
                      A->qlist, A->counts, fs, h->ps, mx, my);
   const dim3 gb((unsigned)((my + 2 + 63) / 64), (unsigned)((mx + 2 + 3) / 4));
@@ -809,7 +811,37 @@ static int adj_fold(gp_handle *h) {
   return GP_OK;
 }
 
-typedef int (*adj_fn)(gp_handle *);
+typedef int (*adj_fn)(gp_handle *, const double *, const double *, const double *, const void *);
+
+/* FP32 handles: the current state widened into A->st64 and its delta-phi (FP64 model) into A->dphi64 */
+__global__ __launch_bounds__(256) void k_widen(const float *__restrict__ src, double *__restrict__ dst, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = (double)src[i];
+}
+template <int N, int NC, bool P2>
+static int adj_widen_t(gp_handle *h) {
+  Adjoint *A = h->adj;
+  const int64_t np = (int64_t)N * h->fs, nm = (int64_t)NC * h->fs;
+  hipLaunchKernelGGL(k_widen, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, h->stream,
+                     (const float *)h->phi[h->cur], A->st64, np);
+  hipLaunchKernelGGL(k_widen, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, h->stream,
+                     (const float *)h->mu[h->cur], A->st64 + np, nm);
+  const int64_t mx = h->p.mesh_x, my = h->p.mesh_y;
+  dim3 g1((unsigned)((my + 2 + kBX - 1) / kBX), (unsigned)((mx + 2 + kBY - 1) / kBY));
+  hipLaunchKernelGGL((k_phi<N, NC, double, false, P2>), g1, dim3(kBX, kBY), 0, h->stream,
+                     (const Model<double> *)A->model64, (const double *)A->st64, (const double *)(A->st64 + np),
+                     A->dphi64, h->fs, h->ps, mx, my);
+  HIPCHK(h, hipGetLastError());
+  return GP_OK;
+}
+typedef int (*widen_fn)(gp_handle *);
+static widen_fn pick_widen(const gp_handle *h) {
+#define GP_CASE(a, b) \
+  if (h->N == a && h->NC == b) return h->pow2 ? adj_widen_t<a, b, true> : adj_widen_t<a, b, false>;
+  GP_CASE(2, 1) GP_CASE(3, 2) GP_CASE(3, 1) GP_CASE(2, 2) GP_CASE(4, 2) GP_CASE(4, 1) GP_CASE(3, 3)
+#undef GP_CASE
+  return nullptr;
+}
 
 static adj_fn pick_adj(const gp_handle *h) {
 #define GP_CASE(a, b) \
@@ -1088,8 +1120,7 @@ int gp_theta_values(const gp_handle *h, double *theta) {
 
 int gp_adjoint_forward(gp_handle *h, int64_t nsteps, int64_t every) {
   if (!h || nsteps < 0) return h ? fail(h, GP_E_ARG, "bad nsteps") : GP_E_ARG;
-  if (h->precision != GP_FP64) return fail(h, GP_E_ARG, "the adjoint is built for the FP64 mode");
-  if (!pick_adj(h)) return fail(h, GP_E_ARG, "no adjoint instantiated for %d phases / %d components", h->N, h->NC);
+  if (!pick_adj(h) || !pick_widen(h)) return fail(h, GP_E_ARG, "no adjoint instantiated for %d phases / %d components", h->N, h->NC);
   if (!h->use_march) return fail(h, GP_E_ARG, "the adjoint records single-pass steps (GP_PATH=twopass is set)");
   if (h->fs > 0x7fffffff) return fail(h, GP_E_ARG, "the adjoint indexes cells with 32 bits: grid too large");
   delete h->adj;
@@ -1101,7 +1132,7 @@ int gp_adjoint_forward(gp_handle *h, int64_t nsteps, int64_t every) {
   const int64_t nck = (nsteps + every - 1) / every;
   for (int64_t k = 0; k < nck; k++) {
     double *b = nullptr;
-    if (hipMalloc((void **)&b, state_elems(h) * sizeof(double)) != hipSuccess)
+    if (hipMalloc((void **)&b, state_elems(h) * esize(h)) != hipSuccess)
       return fail(h, GP_E_OOM, "adjoint: checkpoint %lld does not fit in device memory", (long long)k);
     A->ckpt.push_back(b);
     int rc = state_save(h, b);
@@ -1129,8 +1160,17 @@ int gp_adjoint_backward(gp_handle *h, const double *gphi_T, const double *gmu_T,
               hipMalloc((void **)&A->counts, 2 * sizeof(int)) == hipSuccess;
     for (int64_t j = 0; ok && j < std::min(A->every, A->nsteps); j++) {
       double *b = nullptr;
-      ok = hipMalloc((void **)&b, se * sizeof(double)) == hipSuccess;
+      ok = hipMalloc((void **)&b, se * esize(h)) == hipSuccess;
       if (ok) A->seg.push_back(b);
+    }
+    if (ok && h->precision == GP_FP32) {
+      Model<double> m;
+      make_model(&h->p, &m);
+      ok = hipMalloc((void **)&A->st64, se * sizeof(double)) == hipSuccess &&
+           hipMalloc((void **)&A->dphi64, (size_t)h->N * h->fs * sizeof(double)) == hipSuccess &&
+           hipMalloc(&A->model64, sizeof m) == hipSuccess &&
+           hipMemcpyAsync(A->model64, &m, sizeof m, hipMemcpyHostToDevice, h->stream) == hipSuccess &&
+           hipStreamSynchronize(h->stream) == hipSuccess;
     }
     if (!ok) return fail(h, GP_E_OOM, "adjoint: buffers do not fit in device memory");
   }
@@ -1142,6 +1182,8 @@ int gp_adjoint_backward(gp_handle *h, const double *gphi_T, const double *gmu_T,
   HIPCHK(h, hipMemsetAsync(A->gth, 0, sizeof(double) * nth, h->stream));
   step_fn f = pick_any(h);
   adj_fn fa = pick_adj(h);
+  widen_fn fw = pick_widen(h);
+  const bool wide = h->precision == GP_FP64;
   for (int64_t k = (int64_t)A->ckpt.size() - 1; k >= 0; k--) {
     const int64_t n = std::min(A->every, A->nsteps - k * A->every);
     if ((rc = state_load(h, A->ckpt[k]))) return rc;
@@ -1152,8 +1194,14 @@ int gp_adjoint_backward(gp_handle *h, const double *gphi_T, const double *gmu_T,
     for (int64_t j = n - 1; j >= 0; j--) {
       ProfScope ps(h, GP_KERNEL_ADJ);
       if ((rc = state_load(h, A->seg[j]))) return rc;
-      if ((rc = f(h, MODE_DPHI_CUR))) return rc;
-      if ((rc = fa(h))) return rc;
+      if (wide) {
+        if ((rc = f(h, MODE_DPHI_CUR))) return rc;
+        rc = fa(h, (const double *)h->phi[h->cur], (const double *)h->mu[h->cur], (const double *)h->dphi, h->model);
+      } else {
+        if ((rc = fw(h))) return rc;
+        rc = fa(h, A->st64, A->st64 + (size_t)h->N * h->fs, A->dphi64, A->model64);
+      }
+      if (rc) return rc;
     }
   }
   if ((rc = adj_fold(h))) return rc;

@@ -95,7 +95,7 @@ class GPSolver:
         self._check(self.lib.gp_profile_read(self.h, avg, cnt))
         return {self.lib.gp_kernel_name(k).decode(): (avg[k], cnt[k]) for k in range(_abi.GP_NKERNELS) if cnt[k]}
 
-    # ---------------------------------------------------------------- adjoint (FP64 handles)
+    # ---------------------------------------------------------------- adjoint
     def theta(self) -> np.ndarray:
         """Differentiable parameters (gp_hip.h GP_THETA layout): per phase pair a < b dab, tau_ab, Gamma; the
         liquid's diagonal diffusivities; tau."""

@@ -15,7 +15,10 @@ halo-inclusive entry (so the transposed boundary conditions at the final state a
     rounding, which the atomic accumulation makes order-dependent);
   * the forward of the differentiable trajectory equals plain stepping bit for bit;
   * the full gradients (initial phi, mu and theta) equal those of torch.autograd through torch_step.py, an
-    independent FP64 torch restatement of the step, to rounding.
+    independent FP64 torch restatement of the step, to rounding;
+  * FP32 handles: the reverse sweep over the FP32 trajectory equals the chain of torch.autograd vector-Jacobian
+    products of the FP64 torch step taken at the FP32 states (what the FP32 adjoint is defined to compute), to
+    rounding; its gradients stay close to the FP64 adjoint's.
 The decks are the golden ones (binary, ternary and four-phase; isotropic and anisotropic, rotated; obstacle and
 well potentials; periodic, Neumann and mixed sides; dilute and LU solves; undercooling).
 """
@@ -31,6 +34,7 @@ CASES = ["bin_tanh_restart", "bin_aniso_neu", "tern_aniso_per", "tern_c1_mini", 
 NSTEPS = 20
 EVERY = 6
 RTOL = 1e-4
+FP32_CALIB_RTOL = 5e-4   # FP32 gradients at 256^2 over 2000 steps against FP64 differences: measured 8.9e-5, 2.3e-4
 
 
 def _say(msg):
@@ -52,9 +56,9 @@ def _loss(params, phi0, mu0, wphi, wmu, nsteps=NSTEPS):
     return float(np.sum(wphi * phi) + np.sum(wmu * mu))
 
 
-def _adjoint(params, phi0, mu0, wphi, wmu, nsteps=NSTEPS, every=EVERY):
+def _adjoint(params, phi0, mu0, wphi, wmu, nsteps=NSTEPS, every=EVERY, precision=_abi.GP_FP64):
     from microsim_amd.solver import GPSolver
-    with GPSolver(params, _abi.GP_FP64) as s:
+    with GPSolver(params, precision) as s:
         s.upload(phi0, mu0)
         s.adjoint_forward(nsteps, every)
         phiT, muT = s.download()
@@ -286,6 +290,23 @@ def test_calibration_gradient_binary_256():
     fdv = (loss_np(th0, phi0 + hh * v) - loss_np(th0, phi0 - hh * v)) / (2 * hh)
     print("  initial phi along v (%d cells): adj % .10e fd % .10e" % (iface.sum(), adj, fdv))
     assert abs(adj - fdv) <= 1e-4 * abs(fdv)
+    # FP32: the same loss over the FP32 trajectory (FP32 checkpoints and recompute, FP64 Jacobians and gradient
+    # accumulation), against the FP64 central differences above
+    th32 = torch.tensor(th0, requires_grad=True)
+    p32 = torch.tensor(phi0, requires_grad=True)
+    phiT, _ = adjoint.trajectory(params, th32, p32, torch.tensor(mu0), steps, every=100, precision=_abi.GP_FP32)
+    loss32 = torch.mean((phiT[:, 3:-3, 3:-3] - torch.from_numpy(target)) ** 2)
+    loss32.backward()
+    g32 = th32.grad.numpy()
+    for i in range(th0.size):
+        print("  fp32 %-14s adj % .10e  fd % .10e  rel %.2e" % (names[i], g32[i], fd[i],
+                                                                abs(g32[i] - fd[i]) / abs(fd[i])))
+    adj32 = float(np.sum(p32.grad.numpy() * v))
+    print("  fp32 loss %.10e (fp64 %.10e); initial phi along v: adj % .10e rel %.2e"
+          % (float(loss32.detach()), lval, adj32, abs(adj32 - fdv) / abs(fdv)))
+    err32 = np.abs(g32 - fd) * np.abs(th0)
+    assert np.all(err32 <= FP32_CALIB_RTOL * np.maximum(scale, 1e-3 * scale.max())), list(zip(names, g32, fd))
+    assert abs(adj32 - fdv) <= FP32_CALIB_RTOL * abs(fdv)
 
 
 @pytest.mark.gpu
@@ -309,3 +330,82 @@ def test_gradients_match_torch_autograd(name, rtol):
     ((phi * torch.from_numpy(wphi)).sum() + (mu * torch.from_numpy(wmu)).sum()).backward()
     for got, ref in ((g0, phi0.grad.numpy()), (m0, mu0.grad.numpy()), (gth, th.grad.numpy())):
         assert np.abs(got - ref).max() <= rtol * np.abs(ref).max()
+
+
+def _fp32_states(params, phi0, mu0, nsteps):
+    """The FP32 trajectory's states s_0 .. s_nsteps, widened to float64 (s_0 is the uploaded state after its
+    rounding to FP32 and the boundary copies)."""
+    from microsim_amd.solver import GPSolver
+    out = []
+    with GPSolver(params, _abi.GP_FP32) as s:
+        s.upload(phi0, mu0)
+        out.append(s.download())
+        for _ in range(nsteps):
+            s.step(1)
+            out.append(s.download())
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["bin_tanh_restart", "bin_aniso_neu", "tern_aniso_per", "tern_c1_mini", "quad_k3",
+                                  "bin_well_rot"])
+def test_fp32_adjoint_is_the_fp64_transpose_at_fp32_states(name):
+    """The FP32 reverse sweep against torch.autograd: one FP64 torch step per FP32 state, its vector-Jacobian
+    product taken with the cotangent of the next state, chained from the final state back to the upload (the
+    boundary copies of the upload transposed last). The FP32 rounding of the upload is treated as the identity.
+    Also: recording for the adjoint leaves the FP32 trajectory bit for bit as plain stepping makes it."""
+    import torch
+    import torch_step
+    from microsim_amd import adjoint
+    _say("fp32 transpose " + name)
+    case, _ = load_golden(name)
+    wphi, wmu = _weights(case, 51)
+    g0, m0, gth, phiT, muT = _adjoint(case.params, case.phi, case.mu, wphi, wmu, precision=_abi.GP_FP32)
+    states = _fp32_states(case.params, case.phi, case.mu, NSTEPS)
+    assert np.array_equal(states[-1][0], phiT) and np.array_equal(states[-1][1], muT)
+    th = torch.tensor(adjoint.theta_from_params(case.params), requires_grad=True)
+    gp, gm = torch.from_numpy(wphi), torch.from_numpy(wmu)
+    for t in range(NSTEPS - 1, -1, -1):
+        m = torch_step.TorchModel(case.params, th)
+        x = torch.tensor(states[t][0], requires_grad=True)
+        y = torch.tensor(states[t][1], requires_grad=True)
+        phi, mu = torch_step.step(m, x, y)
+        ((phi * gp).sum() + (mu * gm).sum()).backward()
+        gp, gm = x.grad, y.grad
+    m = torch_step.TorchModel(case.params)
+    x = torch.tensor(case.phi, requires_grad=True)
+    y = torch.tensor(case.mu, requires_grad=True)
+    bphi = torch.stack(torch_step.apply_bc(m, list(x), 0))
+    bmu = torch.stack(torch_step.apply_bc(m, list(y), 1))
+    ((bphi * gp).sum() + (bmu * gm).sum()).backward()
+    rtol = 1e-6 if name == "bin_well_rot" else 1e-10
+    for got, ref in ((g0, x.grad.numpy()), (m0, y.grad.numpy()), (gth, th.grad.numpy())):
+        err = np.abs(got - ref).max() / np.abs(ref).max()
+        print("  max |fp32 adjoint - torch| / max |torch| = %.3e" % err)
+        assert err <= rtol
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", CASES)
+def test_fp32_adjoint_close_to_fp64(name):
+    """The FP32 gradients against the FP64 adjoint's, relative to the largest entry of each (measured at most
+    2.1e-6, on tern_aniso_per's initial phi): the FP32 trajectory stays within 2.1e-7 of the FP64 one over these
+    steps, and the linearisations follow it. The initial-phi gradient is compared at the cells whose phases the
+    FP32 rounding leaves on the same branch: every phase exactly 0 or 1, or farther than 1e-6 from both. A tanh
+    tail cell such as (6.7e-10, 1 - 6.7e-10) becomes (6.7e-10, 1) in FP32, where the step's derivative is another
+    one (bin_tanh_restart: -6.0 against 2.6e-9 in FP64); there the FP32 gradient is its own trajectory's exact
+    one, as the transpose test above shows."""
+    _say("fp32 vs fp64 gradients " + name)
+    case, _ = load_golden(name)
+    wphi, wmu = _weights(case, 61)
+    r64 = _adjoint(case.params, case.phi, case.mu, wphi, wmu)
+    r32 = _adjoint(case.params, case.phi, case.mu, wphi, wmu, precision=_abi.GP_FP32)
+    p = case.phi
+    same = np.all((p == 0) | (p == 1) | ((p > 1e-6) & (p < 1 - 1e-6)), axis=0)
+    print("  initial phi compared on %d of %d cells" % (same.sum(), same.size))
+    for label, a, b in zip(("phi0", "mu0", "theta"), r32[:3], r64[:3]):
+        if label == "phi0":
+            a, b = a[:, same], b[:, same]
+        err = np.abs(a - b).max() / np.abs(b).max()
+        print("  %-6s max rel diff %.3e" % (label, err))
+        assert err <= 1e-5
