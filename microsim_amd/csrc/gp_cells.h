@@ -62,15 +62,56 @@ __device__ __forceinline__ void stage_codes(int8_t *cs, const int8_t *__restrict
   for (int r = 0; r < nr; r++) cs[r * 64 + lane] = codes[(t.xa - 2 + r) * ps + t.yl + kYOff];
 }
 
-/* the bulk update of one cell in phase b (UNIFORM: b is the same on every lane, read as a scalar) */
+/* FP32 bulk cells take the closed form through Model::Kbulk; FP64 keeps the reference's operations (bit-exact) */
+template <typename R>
+constexpr bool kLeanBulk = sizeof(R) < sizeof(double);
+
+/* the bulk update of one cell in phase b. UNIFORM: b is the same on every lane (a scalar); every lane computes,
+ * only lanes with `store` write, the mu statistics take every lane (the caller resets the lanes without a cell
+ * once per tile) and the phi extrema (e_b on every cell) are left to the caller, once per tile. Otherwise the
+ * caller calls it on the bulk lanes only, and it folds the full statistics. */
 template <int N, int NC, typename R, bool P2, bool STATS, bool UNIFORM, int NSA>
 __device__ __forceinline__ void cbulk_cell(const Model<R> &m, int b, int64_t x, const CellTile &t, const R *m0,
                                            const R *m1, const R *m2, const R *mud, const R *muu, R divjat0,
                                            R *__restrict__ phi_o, R *__restrict__ mu_o,
                                            int8_t *__restrict__ codes_out, const Maps &maps, int64_t fs, int64_t ps,
-                                           int64_t mx, int keep, R (&st)[NSA]) {
+                                           int64_t mx, bool kept, bool store, bool live, R (&st)[NSA]) {
   using Vw = View<R, P2>;
-  R Mb[NC][NC], pe[N];
+  R pe[N];
+#pragma unroll
+  for (int a = 0; a < N; a++) pe[a] = a == b ? R(1.0) : R(0.0);
+  R deltamu[NC], munew[NC];
+  if constexpr (kLeanBulk<R>) {
+    /* FP32: the pure-phase update in closed form, dmu = Kbulk[b] . lap(mu) (equal in exact arithmetic to the
+     * flux divergence through Mpure[b] followed by Mu_lin; divjat0 is a signed zero). The second differences
+     * are formed from first differences, so their rounding is relative to the increment, as in the flux form. */
+    R lap[NC];
+#pragma unroll
+    for (int l = 0; l < NC; l++) {
+      const R gx = (m2[l] - m1[l]) - (m1[l] - m0[l]);
+      const R gy = (mud[l] - m1[l]) - (m1[l] - muu[l]);
+      lap[l] = __builtin_fmaf(gx, m.lap_x, gy * m.lap_y);
+    }
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+      R s = R(0.0);
+#pragma unroll
+      for (int l = 0; l < NC; l++) {
+        R kk = m.Kbulk[0][k][l];
+        if (UNIFORM) {
+          kk = m.Kbulk[b][k][l];
+        } else {
+#pragma unroll
+          for (int a = 1; a < N; a++)
+            if (b == a) kk = m.Kbulk[a][k][l];
+        }
+        s = l == 0 ? kk * lap[0] : __builtin_fmaf(kk, lap[l], s);
+      }
+      deltamu[k] = s;
+      munew[k] = m1[k] + s;
+    }
+  } else {
+  R Mb[NC][NC];
 #pragma unroll
   for (int k = 0; k < NC; k++)
 #pragma unroll
@@ -84,9 +125,7 @@ __device__ __forceinline__ void cbulk_cell(const Model<R> &m, int b, int64_t x, 
           if (b == a) Mb[k][l] = m.Mpure[a][k][l];
       }
     }
-#pragma unroll
-  for (int a = 0; a < N; a++) pe[a] = a == b ? R(1.0) : R(0.0);
-  R divflux[NC], deltamu[NC], munew[NC];
+  R divflux[NC];
 #pragma unroll
   for (int k = 0; k < NC; k++) {
     divflux[k] = R(0.0);
@@ -145,23 +184,34 @@ __device__ __forceinline__ void cbulk_cell(const Model<R> &m, int b, int64_t x, 
       deltamu[k] = munew[k] - m1[k];
     }
   }
-  const int64_t c = x * ps + t.y + kYOff;
-  const int8_t code = (int8_t)b;
-  /* keep: the output buffers hold the state before the current one and its codes; a cell that was purely
-   * phase b there already holds phi = e_b, so only mu is written */
-  if (!(keep && codes_out[c] == code)) {
-#pragma unroll
-    for (int a = 0; a < N; a++) phi_o[a * fs + c] = pe[a];
-    codes_out[c] = code;
   }
+  if (store) {
+    const int64_t c = x * ps + t.y + kYOff;
+    const int8_t code = (int8_t)b;
+    /* kept: the output buffers hold the state before the current one and its codes; a cell that was purely
+     * phase b there already holds phi = e_b, so only mu is written */
+    if (!kept) {
 #pragma unroll
-  for (int k = 0; k < NC; k++) mu_o[k * fs + c] = munew[k];
-  if (t.img_y || x < 7 || x > mx - 2) {
-    write_images(codes_out, 1, &code, 0, maps, x, t.y, 0, ps);
-    write_images(phi_o, N, pe, 0, maps, x, t.y, fs, ps);
-    write_images(mu_o, NC, munew, 1, maps, x, t.y, fs, ps);
+      for (int a = 0; a < N; a++) phi_o[a * fs + c] = pe[a];
+      codes_out[c] = code;
+    }
+#pragma unroll
+    for (int k = 0; k < NC; k++) mu_o[k * fs + c] = munew[k];
+    if (t.img_y || x < 7 || x > mx - 2) {
+      write_images(codes_out, 1, &code, 0, maps, x, t.y, 0, ps);
+      write_images(phi_o, N, pe, 0, maps, x, t.y, fs, ps);
+      write_images(mu_o, NC, munew, 1, maps, x, t.y, fs, ps);
+    }
   }
-  if (STATS) {
+  if (STATS && UNIFORM && live) {
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+      st[2 * N + k] = fmax(st[2 * N + k], munew[k]);
+      st[2 * N + NC + k] = fmin(st[2 * N + NC + k], munew[k]);
+      st[3 * N + 2 * NC + k] += deltamu[k] * deltamu[k];
+      if (!isfinite(munew[k])) st[NSA - 1] += R(1.0);
+    }
+  } else if (STATS) {
     /* stats_add with delta-phi = 0 and phi = e_b: the squared increments of phi add +0 to sums that start at +0
      * and e_b is finite, so only the extrema of phi and the mu terms remain */
 #pragma unroll
@@ -179,7 +229,19 @@ __device__ __forceinline__ void cbulk_cell(const Model<R> &m, int b, int64_t x, 
   }
 }
 
-constexpr int kCellBulkDepth = 4; /* mu rows in flight ahead of the current one */
+/* mu rows in flight ahead of the row after the current one, per precision (register budget: FP64 rows take twice
+ * the registers); kCellRing = depth + 2 register slots hold rows x-1, x, x+1 and those in flight, and divides the
+ * tile's 16 rows */
+#ifndef GP_CELL_DEPTH_F32
+#define GP_CELL_DEPTH_F32 2
+#endif
+#ifndef GP_CELL_DEPTH_F64
+#define GP_CELL_DEPTH_F64 2
+#endif
+template <typename R>
+constexpr int kCellBulkDepth = sizeof(R) == 8 ? GP_CELL_DEPTH_F64 : GP_CELL_DEPTH_F32;
+template <typename R>
+constexpr int kCellRing = kCellBulkDepth<R> + 2;
 
 template <int N, int NC, typename R, bool P2, bool STATS>
 __global__ __launch_bounds__(256) void k_cbulk(const Model<R> *__restrict__ mp, const R *__restrict__ mu,
@@ -198,8 +260,8 @@ __global__ __launch_bounds__(256) void k_cbulk(const Model<R> *__restrict__ mp, 
   uint16_t *il = il_all[threadIdx.x >> 6], *cl = cl_all[threadIdx.x >> 6];
   int icnt = 0, ccnt = 0;
   const Model<R> &m = *mp;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int w = blockIdx.x * 4 + wv;
+  const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int w = blockIdx.x * 4 + wv; /* wave-uniform: the tile's rows and the row loop are scalar */
   int8_t *cs = cs_all[wv];
   R st[NSA];
   if (STATS) stats_init<N, NC>(st);
@@ -221,70 +283,95 @@ __global__ __launch_bounds__(256) void k_cbulk(const Model<R> *__restrict__ mp, 
     R divjat0 = divx<Vw>(m, zj);
     divjat0 += divy<Vw>(m, zj);
     const int64_t c0 = t.yl + kYOff;
-    R m0[NC], m1[NC], q[kCellBulkDepth][NC];
+    /* mu rows in a ring of kCellRing register slots, row r in slot (r - xa) mod kCellRing, so that no slot is
+     * ever copied: the row being updated reads rows x-1, x, x+1 where they were loaded, and the slot of x-1 then
+     * takes row x + kCellBulkDepth + 1. The phase codes of the output buffers (for `kept`) ride the same ring,
+     * kCellBulkDepth rows ahead. A loaded value is first read kCellBulkDepth rows after its load, and no wait
+     * in between covers a newer load (the counter of outstanding memory operations is in order). */
+    constexpr int RG = kCellRing<R>, D = kCellBulkDepth<R>;
+    R ring[RG][NC];
+    uint8_t kc[RG]; /* output codes as bytes (zero-extending loads, no sign extension after the load) */
+    const uint8_t *codes_ou = (const uint8_t *)codes_out;
+    /* every load is issued on every path (rows past the tile's end are clamped to its last row + 1, which is
+     * inside the halo): the wait before a row then never has fewer than the next rows' loads behind it */
 #pragma unroll
-    for (int k = 0; k < NC; k++) {
-      m0[k] = mu[k * fs + (t.xa - 1) * ps + c0];
-      m1[k] = mu[k * fs + t.xa * ps + c0];
+    for (int d = -1; d <= D; d++) {
+      const int64_t xr = t.xa + d <= t.xb + 1 ? t.xa + d : t.xb + 1;
+#pragma unroll
+      for (int k = 0; k < NC; k++) ring[(d + RG) % RG][k] = mu[k * fs + xr * ps + c0];
     }
 #pragma unroll
-    for (int d = 0; d < kCellBulkDepth; d++)
-      if (t.xa + 1 + d <= t.xb + 1) {
+    for (int d = 0; d < D; d++) kc[d] = codes_ou[(t.xa + d <= t.xb ? t.xa + d : t.xb) * ps + c0];
+    /* the prologue's loads complete here, once per tile: the loop's first row then waits on nothing, and the
+     * wait counts inside the loop follow from the loop body alone (vmcnt 0, expcnt and lgkmcnt unconstrained) */
+    __builtin_amdgcn_s_waitcnt(0x0f70);
+    /* the row loop, instantiated twice so that the tile-uniform form is a loop of its own (no per-row test) */
+    auto rows = [&](auto uniform) {
+      constexpr bool UNI = decltype(uniform)::value;
+      for (int64_t x0 = t.xa; x0 <= t.xb; x0 += RG) {
 #pragma unroll
-        for (int k = 0; k < NC; k++) q[d][k] = mu[k * fs + (t.xa + 1 + d) * ps + c0];
-      }
-    for (int64_t x = t.xa; x <= t.xb; x++) {
-      R m2[NC];
+        for (int j = 0; j < RG; j++) {
+          /* rows past the tile's last one (at the grid's end) run on clamped rows and write nothing: the loop
+           * has one exit and every path through its body issues every load */
+          const int64_t x = x0 + j;
+          const bool live = x <= t.xb;
+          const R *m0 = ring[(j + RG - 1) % RG], *m1 = ring[j], *m2 = ring[(j + 1) % RG];
+          R mud[NC], muu[NC];
 #pragma unroll
-      for (int k = 0; k < NC; k++) m2[k] = q[0][k];
+          for (int k = 0; k < NC; k++) {
+            mud[k] = lane_dn(m1[k]);
+            muu[k] = lane_up(m1[k]);
+          }
+          if (UNI) {
+            cbulk_cell<N, NC, R, P2, STATS, true>(m, tb, x, t, m0, m1, m2, mud, muu, divjat0, phi_o, mu_o,
+                                                  codes_out, maps, fs, ps, mx, keep && kc[j] == (uint8_t)tb,
+                                                  live && t.out_lane, live, st);
+          } else if (live) {
+            const int bl = t.out_lane ? bulk_of(cs, (int)(x - t.xa) + 2, lane) : -1;
+            {
+              /* the row's interface cells (for k_ilist) and delta-phi candidates (for k_dlist) */
+              const int r = (int)(x - t.xa) + 2;
+              const int8_t k0 = cs[r * 64 + lane];
+              const bool cand = t.out_lane && (k0 < 0 || cs[(r - 1) * 64 + lane] != k0 ||
+                                               cs[(r + 1) * 64 + lane] != k0 || cs[r * 64 + lane - 1] != k0 ||
+                                               cs[r * 64 + lane + 1] != k0);
+              const bool iface = t.out_lane && bl < 0;
+              const uint16_t e = (uint16_t)((x - t.xa) * 64 + lane);
+              const uint64_t lt = (1ull << lane) - 1ull;
+              const uint64_t mi = __ballot(iface), mc = __ballot(cand);
+              if (iface) il[icnt + __popcll(mi & lt)] = e;
+              if (cand) cl[ccnt + __popcll(mc & lt)] = e;
+              icnt += __popcll(mi);
+              ccnt += __popcll(mc);
+            }
+            if (bl >= 0)
+              cbulk_cell<N, NC, R, P2, STATS, false>(m, bl, x, t, m0, m1, m2, mud, muu, divjat0, phi_o, mu_o,
+                                                     codes_out, maps, fs, ps, mx, keep && kc[j] == (uint8_t)bl, true,
+                                                     true, st);
+          }
+          /* refill: row x + D + 1 into the slot of row x - 1, the output codes of row x + D */
+          const int64_t xn = x + D + 1 <= t.xb + 1 ? x + D + 1 : t.xb + 1;
 #pragma unroll
-      for (int d = 0; d + 1 < kCellBulkDepth; d++)
-#pragma unroll
-        for (int k = 0; k < NC; k++) q[d][k] = q[d + 1][k];
-      if (x + 1 + kCellBulkDepth <= t.xb + 1) {
-#pragma unroll
-        for (int k = 0; k < NC; k++) q[kCellBulkDepth - 1][k] = mu[k * fs + (x + 1 + kCellBulkDepth) * ps + c0];
-      }
-      R mud[NC], muu[NC];
-#pragma unroll
-      for (int k = 0; k < NC; k++) {
-        mud[k] = lane_dn(m1[k]);
-        muu[k] = lane_up(m1[k]);
-      }
-      if (tb >= 0) {
-        if (t.out_lane)
-          cbulk_cell<N, NC, R, P2, STATS, true>(m, tb, x, t, m0, m1, m2, mud, muu, divjat0, phi_o, mu_o, codes_out,
-                                                maps, fs, ps, mx, keep, st);
-#pragma unroll
-        for (int k = 0; k < NC; k++) {
-          m0[k] = m1[k];
-          m1[k] = m2[k];
+          for (int k = 0; k < NC; k++) ring[(j + RG - 1) % RG][k] = mu[k * fs + xn * ps + c0];
+          kc[(j + D) % RG] = codes_ou[(xn - 1) * ps + c0];
         }
-        continue;
       }
-      const int bl = t.out_lane ? bulk_of(cs, (int)(x - t.xa) + 2, lane) : -1;
-      {
-        /* the row's interface cells (for k_ilist) and delta-phi candidates (for k_dlist) */
-        const int r = (int)(x - t.xa) + 2;
-        const int8_t k0 = cs[r * 64 + lane];
-        const bool cand = t.out_lane && (k0 < 0 || cs[(r - 1) * 64 + lane] != k0 || cs[(r + 1) * 64 + lane] != k0 ||
-                                         cs[r * 64 + lane - 1] != k0 || cs[r * 64 + lane + 1] != k0);
-        const bool iface = t.out_lane && bl < 0;
-        const uint16_t e = (uint16_t)((x - t.xa) * 64 + lane);
-        const uint64_t lt = (1ull << lane) - 1ull;
-        const uint64_t mi = __ballot(iface), mc = __ballot(cand);
-        if (iface) il[icnt + __popcll(mi & lt)] = e;
-        if (cand) cl[ccnt + __popcll(mc & lt)] = e;
-        icnt += __popcll(mi);
-        ccnt += __popcll(mc);
-      }
-      if (bl >= 0)
-        cbulk_cell<N, NC, R, P2, STATS, false>(m, bl, x, t, m0, m1, m2, mud, muu, divjat0, phi_o, mu_o, codes_out,
-                                               maps, fs, ps, mx, keep, st);
+    };
+    if (tb >= 0)
+      rows(std::true_type{});
+    else
+      rows(std::false_type{});
+    if (STATS && tb >= 0) {
+      /* the tile-uniform rows folded the mu statistics of every lane: keep those of the tile's cells, and add
+       * the phi extrema of its cells (e_tb on each) once */
+      if (!t.out_lane) {
+        stats_init<N, NC>(st);
+      } else {
 #pragma unroll
-      for (int k = 0; k < NC; k++) {
-        m0[k] = m1[k];
-        m1[k] = m2[k];
+        for (int a = 0; a < N; a++) {
+          st[a] = fmax(st[a], a == tb ? R(1.0) : R(0.0));
+          st[N + a] = fmin(st[N + a], a == tb ? R(1.0) : R(0.0));
+        }
       }
     }
   }

@@ -61,6 +61,11 @@ struct Model {
   R dratio[P][NCM]; /* 1.0 - D[a][k][k]/D[L][k][k] */
   R DC[P][NCM][NCM][NCM]; /* D[b][k][q]*cmu[b][q][l] at [b][k][l][q]: the mobility product, formed once */
   R Mpure[P][NCM][NCM];   /* mobility of a cell that is purely phase b */
+  /* FP32 bulk form: a pure-phase-b cell's mu increment is Kbulk[b] . lap(mu), Kbulk[b] = H_b dt Mpure[b] with
+   * H_b the Hessian of Mu_lin (2 A[b][i][i] on the diagonal, A[b][i][j] off it), formed in FP64; lap_x = 1/dx^2,
+   * lap_y = 1/dy^2 */
+  R Kbulk[P][NCM][NCM];
+  R lap_x, lap_y;
 };
 
 /* Stencil accessor over SoA planes: field a at (x+i, y+j) relative to the centre index c.
@@ -924,6 +929,21 @@ inline int make_model(const gp_params *p, Model<R> *m) {
           for (int a = 0; a < N; a++)
             for (int q = 0; q < NC; q++) s += m->DC[a][k][l][q] * (a == b ? R(1.0) : R(0.0));
         m->Mpure[b][k][l] = s;
+      }
+  m->lap_x = (R)(1.0 / (p->dx * p->dx));
+  m->lap_y = (R)(1.0 / (p->dy * p->dy));
+  for (int b = 0; b < N; b++)
+    for (int k = 0; k < NC; k++)
+      for (int l = 0; l < NC; l++) {
+        double s = 0.0;
+        for (int j = 0; j < NC; j++) {
+          double mjl = 0.0;
+          for (int q = 0; q < NC; q++)
+            mjl += p->diffusivity[(b * NCM + j) * NCM + q] * p->cmu[(b * NCM + q) * NCM + l];
+          const double hkj = k == j ? 2.0 * p->A[(b * NCM + k) * NCM + k] : p->A[(b * NCM + k) * NCM + j];
+          s += hkj * p->dt * mjl;
+        }
+        m->Kbulk[b][k][l] = (R)s;
       }
   return 0;
 }

@@ -78,3 +78,54 @@ def test_driver_reproduces_reference_run(name):
                 assert c == pytest.approx(c2, rel=0.1), "%s: relative change %g vs %g" % (n, c, c2)
     finally:
         shutil.rmtree(d, ignore_errors=True)
+
+
+def _run_driver(d, env=None):
+    r = subprocess.run([DRIVER, "Input.in", "Filling.in", "out"], cwd=d, capture_output=True, text=True,
+                       timeout=120, env=env)
+    assert r.returncode == 0, r.stderr
+    return r
+
+
+def _set_keys(path, **keys):
+    text = open(path).read()
+    for k, v in keys.items():
+        text, n = re.subn(r"\b%s\s*=\s*[^;]*;" % k, "%s = %s;" % (k, v), text)
+        assert n == 1, k
+    open(path, "w").write(text)
+
+
+@pytest.mark.parametrize("name,precision", [("tern_aniso_per", "fp64"), ("quad_k3", "fp64"), ("bin_aniso_neu", "fp64"),
+                                            ("tern_aniso_per", "fp32")])
+def test_driver_restart_equals_uninterrupted_run(name, precision):
+    """Run 2n steps in one go; then run n steps, restart from DATA/out_<n>.vtk (RESTART = 1, STARTTIME = n,
+    the reader of Grand_potential_Finite_difference_2D_serial/microsim_gp.c:48-56) and run n more: the snapshot at
+    2n is the same file byte for byte. Binary VTK holds the FP64 state exactly, so the restarted run starts from
+    the state the uninterrupted one holds at n; in FP32 mode the state is FP32 and widens to FP64 exactly in the
+    snapshot, and narrows back exactly on upload."""
+    print("[gpu] driver restart", name, precision, flush=True)
+    sys.stdout.flush()
+    if not os.path.exists(DRIVER):
+        pytest.fail("microsim_gp is not built: run __graft_entry__.build()")
+    env = dict(os.environ, GP_PRECISION=precision)
+    n = 20
+    d1, d2 = tempfile.mkdtemp(prefix="gp_rs1_"), tempfile.mkdtemp(prefix="gp_rs2_")
+    try:
+        for d in (d1, d2):
+            for f in ("Input.in", "Filling.in"):
+                shutil.copy(os.path.join(GOLDEN, name, f), d)
+        _set_keys(os.path.join(d1, "Input.in"), NTIMESTEPS=2 * n, SAVET=n, TRACK_PROGRESS=n, RESTART=0,
+                  STARTTIME=0)
+        _run_driver(d1, env)
+        whole = open(os.path.join(d1, "DATA", "out_%d.vtk" % (2 * n)), "rb").read()
+        os.makedirs(os.path.join(d2, "DATA"))
+        shutil.copy(os.path.join(d1, "DATA", "out_%d.vtk" % n), os.path.join(d2, "DATA"))
+        _set_keys(os.path.join(d2, "Input.in"), NTIMESTEPS=n, SAVET=n, TRACK_PROGRESS=n, RESTART=1, STARTTIME=n)
+        _run_driver(d2, env)
+        assert not os.path.exists(os.path.join(d2, "DATA", "out_0.vtk"))
+        resumed = open(os.path.join(d2, "DATA", "out_%d.vtk" % (2 * n)), "rb").read()
+        assert resumed == whole
+        assert open(os.path.join(d2, "DATA", "shift.dat")).read() == "%d 0\n" % (2 * n)
+    finally:
+        shutil.rmtree(d1, ignore_errors=True)
+        shutil.rmtree(d2, ignore_errors=True)
