@@ -62,6 +62,17 @@ __device__ __forceinline__ void stage_codes(int8_t *cs, const int8_t *__restrict
   for (int r = 0; r < nr; r++) cs[r * 64 + lane] = codes[(t.xa - 2 + r) * ps + t.yl + kYOff];
 }
 
+/* a wave-uniform value read as a scalar: a per-lane choice among such values then stays a register select */
+__device__ __forceinline__ float scalar_val(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ double scalar_val(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 /* FP32 bulk cells take the closed form through Model::Kbulk; FP64 keeps the reference's operations (bit-exact) */
 template <typename R>
 constexpr bool kLeanBulk = sizeof(R) < sizeof(double);
@@ -97,13 +108,16 @@ __device__ __forceinline__ void cbulk_cell(const Model<R> &m, int b, int64_t x, 
       R s = R(0.0);
 #pragma unroll
       for (int l = 0; l < NC; l++) {
-        R kk = m.Kbulk[0][k][l];
+        R kk;
         if (UNIFORM) {
           kk = m.Kbulk[b][k][l];
         } else {
+          /* per-lane phase: pick among the phases' scalar values (kept scalar, so that the choice is a register
+           * select and not a per-lane load issued just before its use) */
+          kk = scalar_val(m.Kbulk[0][k][l]);
 #pragma unroll
           for (int a = 1; a < N; a++)
-            if (b == a) kk = m.Kbulk[a][k][l];
+            if (b == a) kk = scalar_val(m.Kbulk[a][k][l]);
         }
         s = l == 0 ? kk * lap[0] : __builtin_fmaf(kk, lap[l], s);
       }
@@ -119,10 +133,10 @@ __device__ __forceinline__ void cbulk_cell(const Model<R> &m, int b, int64_t x, 
       if (UNIFORM) {
         Mb[k][l] = m.Mpure[b][k][l];
       } else {
-        Mb[k][l] = m.Mpure[0][k][l];
+        Mb[k][l] = scalar_val(m.Mpure[0][k][l]);
 #pragma unroll
         for (int a = 1; a < N; a++)
-          if (b == a) Mb[k][l] = m.Mpure[a][k][l];
+          if (b == a) Mb[k][l] = scalar_val(m.Mpure[a][k][l]);
       }
     }
   R divflux[NC];
