@@ -55,13 +55,6 @@ struct CellTile {
   }
 };
 
-/* the tile's phase codes, rows xa-2 .. xb+2, into LDS (each lane its own column) */
-__device__ __forceinline__ void stage_codes(int8_t *cs, const int8_t *__restrict__ codes, const CellTile &t,
-                                            int64_t ps, int lane) {
-  const int nr = (int)(t.xb - t.xa) + 5;
-  for (int r = 0; r < nr; r++) cs[r * 64 + lane] = codes[(t.xa - 2 + r) * ps + t.yl + kYOff];
-}
-
 /* a wave-uniform value read as a scalar: a per-lane choice among such values then stays a register select */
 __device__ __forceinline__ float scalar_val(float v) {
   return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
@@ -280,7 +273,40 @@ __global__ __launch_bounds__(256) void k_cbulk(const Model<R> *__restrict__ mp, 
   R st[NSA];
   if (STATS) stats_init<N, NC>(st);
   const CellTile t(w, lane, mx, my, seg_len, nstrips);
-  if (w < nwaves) stage_codes(cs, codes_in, t, ps, lane);
+  const int64_t c0 = t.yl + kYOff;
+  /* mu rows in a ring of kCellRing register slots, row r in slot (r - xa) mod kCellRing, so that no slot is
+   * ever copied: the row being updated reads rows x-1, x, x+1 where they were loaded, and the slot of x-1 then
+   * takes row x + kCellBulkDepth + 1. The phase codes of the output buffers (for `kept`) ride the same ring,
+   * kCellBulkDepth rows ahead. A loaded value is first read kCellBulkDepth rows after its load, and no wait
+   * in between covers a newer load (the counter of outstanding memory operations is in order). */
+  constexpr int RG = kCellRing<R>, D = kCellBulkDepth<R>;
+  R ring[RG][NC];
+  uint8_t kc[RG]; /* output codes as bytes (zero-extending loads, no sign extension after the load) */
+  const uint8_t *codes_ou = (const uint8_t *)codes_out;
+  if (w < nwaves) {
+    /* the tile's loads all issue before the first wait: its phase codes (rows xa-2 .. xa+kCellRows-3, clamped
+     * to xb+2), the first mu rows and output codes of the ring (every load is issued on every path; rows past the
+     * tile's end are clamped to its last row + 1, which is inside the halo) */
+    int8_t cv[kCellRows];
+#pragma unroll
+    for (int r = 0; r < kCellRows; r++) {
+      const int64_t xr = t.xa - 2 + r <= t.xb + 2 ? t.xa - 2 + r : t.xb + 2;
+      cv[r] = codes_in[xr * ps + c0];
+    }
+#pragma unroll
+    for (int d = -1; d <= D; d++) {
+      const int64_t xr = t.xa + d <= t.xb + 1 ? t.xa + d : t.xb + 1;
+#pragma unroll
+      for (int k = 0; k < NC; k++) ring[(d + RG) % RG][k] = mu[k * fs + xr * ps + c0];
+    }
+#pragma unroll
+    for (int d = 0; d < D; d++) kc[d] = codes_ou[(t.xa + d <= t.xb ? t.xa + d : t.xb) * ps + c0];
+#pragma unroll
+    for (int r = 0; r < kCellRows; r++) cs[r * 64 + lane] = cv[r];
+    /* the prologue's loads complete here, once per tile: the loop's first row then waits on nothing, and the
+     * wait counts inside the loop follow from the loop body alone (vmcnt 0, expcnt and lgkmcnt unconstrained) */
+    __builtin_amdgcn_s_waitcnt(0x0f70);
+  }
   __syncthreads();
   if (w < nwaves) {
     /* a tile whose staged codes are all one phase b has only bulk cells of phase b: no per-cell test */
@@ -296,29 +322,6 @@ __global__ __launch_bounds__(256) void k_cbulk(const Model<R> *__restrict__ mp, 
     const R zj = R(0.0) * m.jat_fac; /* (j - j_prev) * jat_fac with both currents +0 */
     R divjat0 = divx<Vw>(m, zj);
     divjat0 += divy<Vw>(m, zj);
-    const int64_t c0 = t.yl + kYOff;
-    /* mu rows in a ring of kCellRing register slots, row r in slot (r - xa) mod kCellRing, so that no slot is
-     * ever copied: the row being updated reads rows x-1, x, x+1 where they were loaded, and the slot of x-1 then
-     * takes row x + kCellBulkDepth + 1. The phase codes of the output buffers (for `kept`) ride the same ring,
-     * kCellBulkDepth rows ahead. A loaded value is first read kCellBulkDepth rows after its load, and no wait
-     * in between covers a newer load (the counter of outstanding memory operations is in order). */
-    constexpr int RG = kCellRing<R>, D = kCellBulkDepth<R>;
-    R ring[RG][NC];
-    uint8_t kc[RG]; /* output codes as bytes (zero-extending loads, no sign extension after the load) */
-    const uint8_t *codes_ou = (const uint8_t *)codes_out;
-    /* every load is issued on every path (rows past the tile's end are clamped to its last row + 1, which is
-     * inside the halo): the wait before a row then never has fewer than the next rows' loads behind it */
-#pragma unroll
-    for (int d = -1; d <= D; d++) {
-      const int64_t xr = t.xa + d <= t.xb + 1 ? t.xa + d : t.xb + 1;
-#pragma unroll
-      for (int k = 0; k < NC; k++) ring[(d + RG) % RG][k] = mu[k * fs + xr * ps + c0];
-    }
-#pragma unroll
-    for (int d = 0; d < D; d++) kc[d] = codes_ou[(t.xa + d <= t.xb ? t.xa + d : t.xb) * ps + c0];
-    /* the prologue's loads complete here, once per tile: the loop's first row then waits on nothing, and the
-     * wait counts inside the loop follow from the loop body alone (vmcnt 0, expcnt and lgkmcnt unconstrained) */
-    __builtin_amdgcn_s_waitcnt(0x0f70);
     /* the row loop, instantiated twice so that the tile-uniform form is a loop of its own (no per-row test) */
     auto rows = [&](auto uniform) {
       constexpr bool UNI = decltype(uniform)::value;
