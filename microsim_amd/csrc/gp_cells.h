@@ -257,7 +257,7 @@ __global__ __launch_bounds__(256) void k_cbulk(const Model<R> *__restrict__ mp, 
                                                double *__restrict__ partials, int64_t pstride, int64_t fs, int64_t ps,
                                                int64_t mx, int64_t my, int seg_len, int nstrips, int nwaves,
                                                int keep, int32_t *__restrict__ ilist, int32_t *__restrict__ clist,
-                                               int *counts) {
+                                               int *counts, int *counts_next) {
   using Vw = View<R, P2>;
   constexpr int NS = 3 * N + 3 * NC + 1;
   constexpr int NSA = STATS ? NS : 1;
@@ -274,6 +274,8 @@ __global__ __launch_bounds__(256) void k_cbulk(const Model<R> *__restrict__ mp, 
   if (STATS) stats_init<N, NC>(st);
   const CellTile t(w, lane, mx, my, seg_len, nstrips);
   const int64_t c0 = t.yl + kYOff;
+  /* the next step's list lengths start at zero (the kernels that read them last ran before this one) */
+  if (blockIdx.x == 0 && threadIdx.x < 2) counts_next[threadIdx.x] = 0;
   /* mu rows in a ring of kCellRing register slots, row r in slot (r - xa) mod kCellRing, so that no slot is
    * ever copied: the row being updated reads rows x-1, x, x+1 where they were loaded, and the slot of x-1 then
    * takes row x + kCellBulkDepth + 1. The phase codes of the output buffers (for `kept`) ride the same ring,
@@ -418,7 +420,7 @@ __global__ __launch_bounds__(256) void k_cbulk(const Model<R> *__restrict__ mp, 
     clist[wbase[1][wv] + i] = (int32_t)((t.xa + (e >> 6)) * ps + (t.y - lane + (e & 63)) + kYOff);
   }
   if (!STATS) return;
-  stats_partials<N, NC>(st, partials, pstride, w, nwaves, lane);
+  stats_partials<N, NC, true>(st, partials, pstride, w, nwaves, lane);
 }
 
 /* Delta-phi candidates on the ring x or y in {2, mesh+3} around the interior (k_cbulk lists the interior ones):

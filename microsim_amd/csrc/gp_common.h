@@ -56,8 +56,16 @@ __device__ __forceinline__ double lane63(double v) {
   const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), 63), hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
-template <class Op>
-__device__ __forceinline__ double reduce(double v, double identity, Op op) {
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ float mov(float v, float identity) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(__float_as_int(identity), __float_as_int(v), CTRL, ROWMASK, 0xf, false));
+}
+__device__ __forceinline__ float lane63(float v) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+template <class T, class Op>
+__device__ __forceinline__ T reduce(T v, T identity, Op op) {
   v = op(v, mov<kQuad1032, 0xf>(v, identity));
   v = op(v, mov<kQuad2301, 0xf>(v, identity));
   v = op(v, mov<kRowHalfMirror, 0xf>(v, identity));
@@ -69,13 +77,20 @@ __device__ __forceinline__ double reduce(double v, double identity, Op op) {
 }  // namespace dpp
 
 __device__ __forceinline__ double wave_max(double v) {
-  return dpp::reduce(v, -INFINITY, [](double a, double b) { return fmax(a, b); });
+  return dpp::reduce<double>(v, -INFINITY, [](double a, double b) { return fmax(a, b); });
 }
 __device__ __forceinline__ double wave_min(double v) {
-  return dpp::reduce(v, INFINITY, [](double a, double b) { return fmin(a, b); });
+  return dpp::reduce<double>(v, INFINITY, [](double a, double b) { return fmin(a, b); });
 }
 __device__ __forceinline__ double wave_sum(double v) {
-  return dpp::reduce(v, 0.0, [](double a, double b) { return a + b; });
+  return dpp::reduce<double>(v, 0.0, [](double a, double b) { return a + b; });
+}
+/* extrema are exact in any precision: FP32 values take the single-register form */
+__device__ __forceinline__ float wave_max(float v) {
+  return dpp::reduce<float>(v, -INFINITY, [](float a, float b) { return fmaxf(a, b); });
+}
+__device__ __forceinline__ float wave_min(float v) {
+  return dpp::reduce<float>(v, INFINITY, [](float a, float b) { return fminf(a, b); });
 }
 
 }  // namespace gpk

@@ -68,7 +68,8 @@ struct gp_handle {
   int cell_seg, cell_nsegs;  /* tile grid of k_cbulk (same strips as the march, shorter segments) */
   int64_t cell_nwaves;
   int32_t *clist, *ilist;    /* delta-phi candidates and interface cells of the current step (cell offsets) */
-  int *lcount;               /* their lengths */
+  int *lcount;               /* their lengths: two pairs, used by alternate steps (lcount + 2 * lsel) */
+  int lsel;
   int *flags[2];            /* per-tile purity flags of the last two steps */
   int8_t *codes[2];         /* phase codes (pure phase b, or -1) of every cell: codes[ccur] of the current state */
   int ccur;
@@ -442,37 +443,40 @@ static int launch_step_p(gp_handle *h, int mode) {
       const int ct = (int)h->cell_nwaves;
       const unsigned cblk = (unsigned)((ct + 3) / 4);
       const int64_t cstride = (int64_t)ct + 4 * kListBlocks;
-      HIPCHK(h, hipMemsetAsync(h->lcount, 0, 2 * sizeof(int), h->stream));
+      /* this step's list lengths are zero (the previous step's k_cbulk, or the set-up, cleared them); this
+       * step's k_cbulk clears the other pair for the next step, which the previous step's kernels read */
+      int *lc = h->lcount + 2 * h->lsel, *lnext = h->lcount + 2 * (1 - h->lsel);
+      h->lsel = 1 - h->lsel;
       {
         ProfScope ps(h, GP_KERNEL_CBULK);
         if (stats)
           hipLaunchKernelGGL((k_cbulk<N, NC, R, P2, true>), dim3(cblk), dim3(256), 0, h->stream, m, mu, codes, phi_o,
                              mu_o, codes_o, h->maps, h->partials, cstride, h->fs, h->ps, mx, my, h->cell_seg,
-                             h->march_nstrips, ct, (int)h->prev_valid, h->ilist, h->clist, h->lcount);
+                             h->march_nstrips, ct, (int)h->prev_valid, h->ilist, h->clist, lc, lnext);
         else
           hipLaunchKernelGGL((k_cbulk<N, NC, R, P2, false>), dim3(cblk), dim3(256), 0, h->stream, m, mu, codes,
                              phi_o, mu_o, codes_o, h->maps, h->partials, cstride, h->fs, h->ps, mx, my, h->cell_seg,
-                             h->march_nstrips, ct, (int)h->prev_valid, h->ilist, h->clist, h->lcount);
+                             h->march_nstrips, ct, (int)h->prev_valid, h->ilist, h->clist, lc, lnext);
       }
       {
         /* delta-phi of the listed candidates (the ring's added first), full wavefronts */
         ProfScope ps(h, GP_KERNEL_DPHI);
         const int64_t nring = 2 * (my + 2) + 2 * mx;
         hipLaunchKernelGGL(k_dring, dim3((unsigned)((nring + 255) / 256)), dim3(256), 0, h->stream, codes, h->clist,
-                           h->lcount + 1, h->ps, mx, my);
+                           lc + 1, h->ps, mx, my);
         hipLaunchKernelGGL((k_dlist<N, NC, R, P2>), dim3(kListBlocks), dim3(256), 0, h->stream, m, phi, mu, dphi,
-                           h->fs, h->ps, (const int32_t *)h->clist, (const int *)h->lcount + 1);
+                           h->fs, h->ps, (const int32_t *)h->clist, (const int *)lc + 1);
       }
       {
         ProfScope ps(h, GP_KERNEL_CIFACE);
         if (stats)
           hipLaunchKernelGGL((k_ilist<N, NC, R, P2, true>), dim3(kListBlocks), dim3(256), 0, h->stream, m, phi, mu,
                              dphi, codes, phi_o, mu_o, codes_o, h->maps, h->partials + ct, cstride, h->fs, h->ps, mx,
-                             my, (const int32_t *)h->ilist, (const int *)h->lcount);
+                             my, (const int32_t *)h->ilist, (const int *)lc);
         else
           hipLaunchKernelGGL((k_ilist<N, NC, R, P2, false>), dim3(kListBlocks), dim3(256), 0, h->stream, m, phi, mu,
                              dphi, codes, phi_o, mu_o, codes_o, h->maps, h->partials + ct, cstride, h->fs, h->ps, mx,
-                             my, (const int32_t *)h->ilist, (const int *)h->lcount);
+                             my, (const int32_t *)h->ilist, (const int *)lc);
       }
       if (stats) {
         ProfScope ps(h, GP_KERNEL_STATS);
@@ -921,7 +925,7 @@ int gp_create(const gp_params *params, int precision, gp_handle **out) {
   ok = ok && hipMalloc((void **)&h->flags[1], sizeof(int) * h->march_nwaves) == hipSuccess;
   ok = ok && hipMalloc((void **)&h->clist, (size_t)h->fs * sizeof(int32_t)) == hipSuccess &&
        hipMalloc((void **)&h->ilist, (size_t)h->fs * sizeof(int32_t)) == hipSuccess &&
-       hipMalloc((void **)&h->lcount, 2 * sizeof(int)) == hipSuccess;
+       hipMalloc((void **)&h->lcount, 4 * sizeof(int)) == hipSuccess;
   ok = ok && hipMalloc((void **)&h->codes[0], (size_t)h->rx * h->ps) == hipSuccess &&
        hipMalloc((void **)&h->codes[1], (size_t)h->rx * h->ps) == hipSuccess;
   h->ccur = 0;
@@ -933,7 +937,9 @@ int gp_create(const gp_params *params, int precision, gp_handle **out) {
   }
   ok = hipMemsetAsync(h->phi[0], 0, pn, h->stream) == hipSuccess && hipMemsetAsync(h->phi[1], 0, pn, h->stream) == hipSuccess &&
        hipMemsetAsync(h->mu[0], 0, mn, h->stream) == hipSuccess && hipMemsetAsync(h->mu[1], 0, mn, h->stream) == hipSuccess &&
-       hipMemsetAsync(h->dphi, 0, pn, h->stream) == hipSuccess;
+       hipMemsetAsync(h->dphi, 0, pn, h->stream) == hipSuccess &&
+       hipMemsetAsync(h->lcount, 0, 4 * sizeof(int), h->stream) == hipSuccess;
+  h->lsel = 0;
   if (precision == GP_FP64) {
     Model<double> m;
     make_model(params, &m);

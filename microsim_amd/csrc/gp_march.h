@@ -197,17 +197,19 @@ __device__ __forceinline__ void stats_add(R (&st)[NSA], const R *pn, const R *dp
   }
 }
 
-/* wavefront reduction of the per-lane statistics into partials[i * pstride + w] (statistic-major), in FP64 */
-template <int N, int NC, int NSA, typename R>
+/* wavefront reduction of the per-lane statistics into partials[i * pstride + w] (statistic-major, FP64) */
+template <int N, int NC, bool PHI_SUMS_ZERO = false, int NSA, typename R>
 __device__ __forceinline__ void stats_partials(R (&st)[NSA], double *__restrict__ partials, int64_t pstride,
                                                int w, int nwaves, int lane) {
+  /* extrema reduce in the statistics' own precision (exact), sums in FP64. PHI_SUMS_ZERO: the kernel updates
+   * bulk cells only, whose squared phi increments are +0 on every lane, so those sums are +0 */
 #pragma unroll
   for (int i = 0; i < NSA; i++) {
     double r;
-    const double v = (double)st[i];
-    if (i < N || (i >= 2 * N && i < 2 * N + NC)) r = wave_max(v);
-    else if (i < 2 * N + 2 * NC) r = wave_min(v);
-    else r = wave_sum(v);
+    if (i < N || (i >= 2 * N && i < 2 * N + NC)) r = (double)wave_max(st[i]);
+    else if (i < 2 * N + 2 * NC) r = (double)wave_min(st[i]);
+    else if (PHI_SUMS_ZERO && i < 3 * N + 2 * NC) r = 0.0;
+    else r = wave_sum((double)st[i]);
     if (lane == 0 && w < nwaves) partials[(int64_t)i * pstride + w] = r;
   }
 }
