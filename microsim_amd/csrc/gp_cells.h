@@ -75,7 +75,8 @@ constexpr bool kLeanBulk = sizeof(R) < sizeof(double);
  * once per tile) and the phi extrema (e_b on every cell) are left to the caller, once per tile. Otherwise the
  * caller calls it on the bulk lanes only, and it folds the full statistics. */
 template <int N, int NC, typename R, bool P2, bool STATS, bool UNIFORM, int NSA>
-__device__ __forceinline__ void cbulk_cell(const Model<R> &m, int b, int64_t x, const CellTile &t, const R *m0,
+__device__ __forceinline__ void cbulk_cell(const Model<R> &m, int b, int64_t x, int64_t c, bool img,
+                                           const CellTile &t, const R *m0,
                                            const R *m1, const R *m2, const R *mud, const R *muu, R divjat0,
                                            R *__restrict__ phi_o, R *__restrict__ mu_o,
                                            int8_t *__restrict__ codes_out, const Maps &maps, int64_t fs, int64_t ps,
@@ -193,7 +194,6 @@ __device__ __forceinline__ void cbulk_cell(const Model<R> &m, int b, int64_t x, 
   }
   }
   if (store) {
-    const int64_t c = x * ps + t.y + kYOff;
     const int8_t code = (int8_t)b;
     /* kept: the output buffers hold the state before the current one and its codes; a cell that was purely
      * phase b there already holds phi = e_b, so only mu is written */
@@ -204,7 +204,7 @@ __device__ __forceinline__ void cbulk_cell(const Model<R> &m, int b, int64_t x, 
     }
 #pragma unroll
     for (int k = 0; k < NC; k++) mu_o[k * fs + c] = munew[k];
-    if (t.img_y || x < 7 || x > mx - 2) {
+    if (img) {
       write_images(codes_out, 1, &code, 0, maps, x, t.y, 0, ps);
       write_images(phi_o, N, pe, 0, maps, x, t.y, fs, ps);
       write_images(mu_o, NC, munew, 1, maps, x, t.y, fs, ps);
@@ -325,15 +325,23 @@ __global__ __launch_bounds__(256) void k_cbulk(const Model<R> *__restrict__ mp, 
     R divjat0 = divx<Vw>(m, zj);
     divjat0 += divy<Vw>(m, zj);
     /* the row loop, instantiated twice so that the tile-uniform form is a loop of its own (no per-row test) */
+    /* row offsets in 32 bits (the cells path runs on grids of fewer than 2^31 cells), kept incrementally */
+    const int pso = (int)ps, c0i = (int)c0, cyi = (int)(t.y + kYOff);
+    const int xa = (int)t.xa, xb = (int)t.xb, mxi = (int)mx;
+    const int xlast = (xb + 1) * pso;
+    const bool img_x = xa < 7 || xb > mxi - 2; /* the tile holds rows whose halo images are written */
     auto rows = [&](auto uniform) {
       constexpr bool UNI = decltype(uniform)::value;
-      for (int64_t x0 = t.xa; x0 <= t.xb; x0 += RG) {
+      int xoff = xa * pso;
+      for (int x0 = xa; x0 <= xb; x0 += RG) {
 #pragma unroll
         for (int j = 0; j < RG; j++) {
           /* rows past the tile's last one (at the grid's end) run on clamped rows and write nothing: the loop
            * has one exit and every path through its body issues every load */
-          const int64_t x = x0 + j;
-          const bool live = x <= t.xb;
+          const int x = x0 + j;
+          const bool live = x <= xb;
+          const int64_t c = (int64_t)xoff + cyi;
+          const bool img = t.img_y || (img_x && (x < 7 || x > mxi - 2));
           const R *m0 = ring[(j + RG - 1) % RG], *m1 = ring[j], *m2 = ring[(j + 1) % RG];
           R mud[NC], muu[NC];
 #pragma unroll
@@ -342,20 +350,20 @@ __global__ __launch_bounds__(256) void k_cbulk(const Model<R> *__restrict__ mp, 
             muu[k] = lane_up(m1[k]);
           }
           if (UNI) {
-            cbulk_cell<N, NC, R, P2, STATS, true>(m, tb, x, t, m0, m1, m2, mud, muu, divjat0, phi_o, mu_o,
+            cbulk_cell<N, NC, R, P2, STATS, true>(m, tb, x, c, img, t, m0, m1, m2, mud, muu, divjat0, phi_o, mu_o,
                                                   codes_out, maps, fs, ps, mx, keep && kc[j] == (uint8_t)tb,
                                                   live && t.out_lane, live, st);
           } else if (live) {
-            const int bl = t.out_lane ? bulk_of(cs, (int)(x - t.xa) + 2, lane) : -1;
+            const int bl = t.out_lane ? bulk_of(cs, x - xa + 2, lane) : -1;
             {
               /* the row's interface cells (for k_ilist) and delta-phi candidates (for k_dlist) */
-              const int r = (int)(x - t.xa) + 2;
+              const int r = x - xa + 2;
               const int8_t k0 = cs[r * 64 + lane];
               const bool cand = t.out_lane && (k0 < 0 || cs[(r - 1) * 64 + lane] != k0 ||
                                                cs[(r + 1) * 64 + lane] != k0 || cs[r * 64 + lane - 1] != k0 ||
                                                cs[r * 64 + lane + 1] != k0);
               const bool iface = t.out_lane && bl < 0;
-              const uint16_t e = (uint16_t)((x - t.xa) * 64 + lane);
+              const uint16_t e = (uint16_t)((x - xa) * 64 + lane);
               const uint64_t lt = (1ull << lane) - 1ull;
               const uint64_t mi = __ballot(iface), mc = __ballot(cand);
               if (iface) il[icnt + __popcll(mi & lt)] = e;
@@ -364,15 +372,16 @@ __global__ __launch_bounds__(256) void k_cbulk(const Model<R> *__restrict__ mp, 
               ccnt += __popcll(mc);
             }
             if (bl >= 0)
-              cbulk_cell<N, NC, R, P2, STATS, false>(m, bl, x, t, m0, m1, m2, mud, muu, divjat0, phi_o, mu_o,
+              cbulk_cell<N, NC, R, P2, STATS, false>(m, bl, x, c, img, t, m0, m1, m2, mud, muu, divjat0, phi_o, mu_o,
                                                      codes_out, maps, fs, ps, mx, keep && kc[j] == (uint8_t)bl, true,
                                                      true, st);
           }
           /* refill: row x + D + 1 into the slot of row x - 1, the output codes of row x + D */
-          const int64_t xn = x + D + 1 <= t.xb + 1 ? x + D + 1 : t.xb + 1;
+          const int xn = min(xoff + (D + 1) * pso, xlast);
 #pragma unroll
-          for (int k = 0; k < NC; k++) ring[(j + RG - 1) % RG][k] = mu[k * fs + xn * ps + c0];
-          kc[(j + D) % RG] = codes_ou[(xn - 1) * ps + c0];
+          for (int k = 0; k < NC; k++) ring[(j + RG - 1) % RG][k] = mu[k * fs + (int64_t)(xn + c0i)];
+          kc[(j + D) % RG] = codes_ou[(int64_t)(xn - pso + c0i)];
+          xoff += pso;
         }
       }
     };

@@ -906,7 +906,9 @@ int gp_create(const gp_params *params, int precision, gp_handle **out) {
   h->nparts_max = std::max(h->nblocks_mu, std::max(2 * h->march_nwaves, h->cell_nwaves + 4 * (int64_t)kListBlocks));
   const char *path = getenv("GP_PATH");
   h->use_march = !(path && strcmp(path, "twopass") == 0);
-  h->use_cells = h->use_march && !(path && strcmp(path, "march") == 0);
+  /* the cell lists hold 32-bit cell indices: grids of 2^31 cells or more (about 46000^2) take the tiled march,
+   * which gives the same results */
+  h->use_cells = h->use_march && !(path && strcmp(path, "march") == 0) && h->fs <= 0x7fffffff;
   h->prev_valid = false;
   h->fcur = 0;
   h->flags_valid = false;
