@@ -250,6 +250,36 @@ constexpr int kCellBulkDepth = sizeof(R) == 8 ? GP_CELL_DEPTH_F64 : GP_CELL_DEPT
 template <typename R>
 constexpr int kCellRing = kCellBulkDepth<R> + 2;
 
+/* Delta-phi candidates on the ring x or y in {2, mesh+3} around the interior (k_cbulk's tiles list the interior
+ * ones): ring cell i (one per thread of k_cbulk's extra blocks), appended to the same list. */
+__device__ __forceinline__ void dring_cell(int64_t i, const int8_t *__restrict__ codes, int32_t *__restrict__ clist,
+                                           int *ccount, int64_t ps, int64_t mx, int64_t my) {
+  const int lane = threadIdx.x & 63;
+  const int64_t nxr = my + 2, nyr = mx; /* rows x = 2, mx+3 over y in [2, my+3]; columns y = 2, my+3 over x in [3, mx+2] */
+  int64_t x = -1, y = -1;
+  if (i < 2 * nxr) {
+    x = (i < nxr) ? 2 : mx + 3;
+    y = 2 + (i % nxr);
+  } else if (i < 2 * nxr + 2 * nyr) {
+    const int64_t k = i - 2 * nxr;
+    y = (k < nyr) ? 2 : my + 3;
+    x = 3 + (k % nyr);
+  }
+  bool cand = false;
+  const int64_t c = x * ps + y + kYOff;
+  if (x >= 0) {
+    const int8_t k0 = codes[c];
+    cand = k0 < 0 || codes[c - ps] != k0 || codes[c + ps] != k0 || codes[c - 1] != k0 || codes[c + 1] != k0;
+  }
+  const uint64_t mask = __ballot(cand);
+  if (!mask) return;
+  const int leader = __ffsll((long long)mask) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(ccount, __popcll(mask));
+  base = __shfl(base, leader);
+  if (cand) clist[base + __popcll(mask & ((1ull << lane) - 1ull))] = (int32_t)c;
+}
+
 template <int N, int NC, typename R, bool P2, bool STATS>
 __global__ __launch_bounds__(256) void k_cbulk(const Model<R> *__restrict__ mp, const R *__restrict__ mu,
                                                const int8_t *__restrict__ codes_in, R *__restrict__ phi_o,
@@ -257,10 +287,17 @@ __global__ __launch_bounds__(256) void k_cbulk(const Model<R> *__restrict__ mp, 
                                                double *__restrict__ partials, int64_t pstride, int64_t fs, int64_t ps,
                                                int64_t mx, int64_t my, int seg_len, int nstrips, int nwaves,
                                                int keep, int32_t *__restrict__ ilist, int32_t *__restrict__ clist,
-                                               int *counts, int *counts_next) {
+                                               int *counts, int *counts_next, int tile_blocks) {
   using Vw = View<R, P2>;
   constexpr int NS = 3 * N + 3 * NC + 1;
   constexpr int NSA = STATS ? NS : 1;
+  if ((int)blockIdx.x >= tile_blocks) {
+    /* the blocks past the tiles list the delta-phi candidates of the ring x or y in {2, mesh+3} around the
+     * interior (one thread per ring cell; the tiles list the interior ones) */
+    dring_cell((int64_t)(blockIdx.x - tile_blocks) * blockDim.x + threadIdx.x, codes_in, clist, counts + 1, ps, mx,
+               my);
+    return;
+  }
   __shared__ int8_t cs_all[4][kCellRows * 64];
   __shared__ uint16_t il_all[4][kCellSegMax * 64], cl_all[4][kCellSegMax * 64];
   __shared__ int wcnt[2][4], wbase[2][4];
@@ -432,36 +469,6 @@ __global__ __launch_bounds__(256) void k_cbulk(const Model<R> *__restrict__ mp, 
   stats_partials<N, NC, true>(st, partials, pstride, w, nwaves, lane);
 }
 
-/* Delta-phi candidates on the ring x or y in {2, mesh+3} around the interior (k_cbulk lists the interior ones):
- * one thread per ring cell, appended to the same list. */
-__global__ __launch_bounds__(256) void k_dring(const int8_t *__restrict__ codes, int32_t *__restrict__ clist,
-                                               int *ccount, int64_t ps, int64_t mx, int64_t my) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x & 63;
-  const int64_t nxr = my + 2, nyr = mx; /* rows x = 2, mx+3 over y in [2, my+3]; columns y = 2, my+3 over x in [3, mx+2] */
-  int64_t x = -1, y = -1;
-  if (i < 2 * nxr) {
-    x = (i < nxr) ? 2 : mx + 3;
-    y = 2 + (i % nxr);
-  } else if (i < 2 * nxr + 2 * nyr) {
-    const int64_t k = i - 2 * nxr;
-    y = (k < nyr) ? 2 : my + 3;
-    x = 3 + (k % nyr);
-  }
-  bool cand = false;
-  const int64_t c = x * ps + y + kYOff;
-  if (x >= 0) {
-    const int8_t k0 = codes[c];
-    cand = k0 < 0 || codes[c - ps] != k0 || codes[c + ps] != k0 || codes[c - 1] != k0 || codes[c + 1] != k0;
-  }
-  const uint64_t mask = __ballot(cand);
-  if (!mask) return;
-  const int leader = __ffsll((long long)mask) - 1;
-  int base = 0;
-  if (lane == leader) base = atomicAdd(ccount, __popcll(mask));
-  base = __shfl(base, leader);
-  if (cand) clist[base + __popcll(mask & ((1ull << lane) - 1ull))] = (int32_t)c;
-}
 
 /* The general update of the interface cells listed by k_cbulk (any order): a grid-stride loop of full
  * wavefronts. Each cell gathers its stencil from memory in one batch: phi on the 3x3 neighbourhood, mu and

@@ -7,8 +7,8 @@
  *   the row pitch ps is a multiple of 64 elements. phi (N planes) and mu (NC planes) are ping-ponged,
  *   dphi (N planes) holds delta-phi when it is asked for.
  * A time step (default, gp_cells.h) classifies cells by the phase codes of the current state: k_cbulk updates the
- * bulk cells (streaming mu; phi stays e_b) and lists the interface cells and the delta-phi candidates; k_dring adds
- * the candidates of the boundary ring; k_dlist evaluates delta-phi of the listed candidates and k_ilist the
+ * bulk cells (streaming mu; phi stays e_b) and lists the interface cells and the delta-phi candidates (its blocks
+ * past the tiles add the candidates of the boundary ring); k_dlist evaluates delta-phi of the listed candidates and k_ilist the
  * general update of the listed interface cells, both with full wavefronts; k_stats folds the statistics when they
  * are asked for. Every kernel writes the halo images of the boundary cells it updates.
  * GP_PATH=march selects the tiled march (gp_march.h: k_dphi, then k_march near an interface and k_bulk on deep
@@ -447,23 +447,23 @@ static int launch_step_p(gp_handle *h, int mode) {
        * step's k_cbulk clears the other pair for the next step, which the previous step's kernels read */
       int *lc = h->lcount + 2 * h->lsel, *lnext = h->lcount + 2 * (1 - h->lsel);
       h->lsel = 1 - h->lsel;
+      /* k_cbulk's blocks past the tiles list the candidates of the boundary ring, one thread per ring cell */
+      const int64_t nring = 2 * (my + 2) + 2 * mx;
+      const unsigned rblk = (unsigned)((nring + 255) / 256);
       {
         ProfScope ps(h, GP_KERNEL_CBULK);
         if (stats)
-          hipLaunchKernelGGL((k_cbulk<N, NC, R, P2, true>), dim3(cblk), dim3(256), 0, h->stream, m, mu, codes, phi_o,
-                             mu_o, codes_o, h->maps, h->partials, cstride, h->fs, h->ps, mx, my, h->cell_seg,
-                             h->march_nstrips, ct, (int)h->prev_valid, h->ilist, h->clist, lc, lnext);
-        else
-          hipLaunchKernelGGL((k_cbulk<N, NC, R, P2, false>), dim3(cblk), dim3(256), 0, h->stream, m, mu, codes,
+          hipLaunchKernelGGL((k_cbulk<N, NC, R, P2, true>), dim3(cblk + rblk), dim3(256), 0, h->stream, m, mu, codes,
                              phi_o, mu_o, codes_o, h->maps, h->partials, cstride, h->fs, h->ps, mx, my, h->cell_seg,
-                             h->march_nstrips, ct, (int)h->prev_valid, h->ilist, h->clist, lc, lnext);
+                             h->march_nstrips, ct, (int)h->prev_valid, h->ilist, h->clist, lc, lnext, (int)cblk);
+        else
+          hipLaunchKernelGGL((k_cbulk<N, NC, R, P2, false>), dim3(cblk + rblk), dim3(256), 0, h->stream, m, mu, codes,
+                             phi_o, mu_o, codes_o, h->maps, h->partials, cstride, h->fs, h->ps, mx, my, h->cell_seg,
+                             h->march_nstrips, ct, (int)h->prev_valid, h->ilist, h->clist, lc, lnext, (int)cblk);
       }
       {
-        /* delta-phi of the listed candidates (the ring's added first), full wavefronts */
+        /* delta-phi of the listed candidates, full wavefronts */
         ProfScope ps(h, GP_KERNEL_DPHI);
-        const int64_t nring = 2 * (my + 2) + 2 * mx;
-        hipLaunchKernelGGL(k_dring, dim3((unsigned)((nring + 255) / 256)), dim3(256), 0, h->stream, codes, h->clist,
-                           lc + 1, h->ps, mx, my);
         hipLaunchKernelGGL((k_dlist<N, NC, R, P2>), dim3(kListBlocks), dim3(256), 0, h->stream, m, phi, mu, dphi,
                            h->fs, h->ps, (const int32_t *)h->clist, (const int *)lc + 1);
       }
