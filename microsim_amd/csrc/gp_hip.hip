@@ -41,7 +41,10 @@ namespace {
 constexpr int kBX = 64;          /* threads along y (one wavefront) */
 constexpr int kBY = 4;           /* rows per block */
 constexpr int kNStatMax = 3 * GP_MAXP + 3 * GP_MAXNC + 1;
-constexpr int kListBlocks = 2048; /* grid of the list-driven kernels (grid-stride loops), 4 wavefronts each */
+#ifndef GP_LIST_BLOCKS
+#define GP_LIST_BLOCKS 2048
+#endif
+constexpr int kListBlocks = GP_LIST_BLOCKS; /* grid of the list-driven kernels (grid-stride loops), 4 wavefronts each */
 
 char g_create_error[512];
 
@@ -240,8 +243,17 @@ __global__ __launch_bounds__(kStatThreads) void k_stats(const double *__restrict
   const bool mn = !mx && i < 2 * N + 2 * NC;
   double r = mx ? -INFINITY : mn ? INFINITY : 0.0;
   const double *row = partials + (int64_t)i * stride;
-  for (int64_t b = threadIdx.x; b < nparts; b += kStatThreads) {
-    double v = row[b];
+  /* a thread folds its partials in index order; the loads of eight of them are issued before the first is used */
+  int64_t b = threadIdx.x;
+  for (; b + 7 * kStatThreads < nparts; b += 8 * kStatThreads) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = row[b + u * kStatThreads];
+#pragma unroll
+    for (int u = 0; u < 8; u++) r = mx ? fmax(r, v[u]) : mn ? fmin(r, v[u]) : r + v[u];
+  }
+  for (; b < nparts; b += kStatThreads) {
+    const double v = row[b];
     r = mx ? fmax(r, v) : mn ? fmin(r, v) : r + v;
   }
   red[threadIdx.x] = r;
