@@ -13,7 +13,7 @@ Everything here produces files in the formats the serial ``microsim_gp`` driver 
   (``solverloop/file_writer.h:151-255``); the restart reader skips 8 header lines and 2 lines per field
   (``solverloop/file_writer.h:328-416``).
 
-The BASELINE workloads (binary tanh seed, ternary random seeds) are built by the ``config_*`` functions; the
+The benchmark workloads (binary tanh seed, ternary random seeds) are built by the ``config_*`` functions; the
 random multi-seed layouts are explicit ``FILLCYLINDER`` lines from a seeded Poisson-disc generator, because the
 reference's own random fillers are not reproducible.
 """
