@@ -25,7 +25,10 @@
 
 namespace gpk {
 
-constexpr int kCellSegMax = 16;                /* longest x-segment the kernels stage (LDS sized for it) */
+#ifndef GP_CELL_SEG_MAX
+#define GP_CELL_SEG_MAX 16
+#endif
+constexpr int kCellSegMax = GP_CELL_SEG_MAX;   /* longest x-segment the kernels stage (LDS sized for it) */
 constexpr int kCellRows = kCellSegMax + 4;     /* staged code rows per tile */
 
 /* b when the radius-2 diamond of lane l at staged row r is purely phase b, else -1 (l in 2..61) */
