@@ -78,9 +78,13 @@ class _Trajectory(torch.autograd.Function):
     def forward(ctx, theta, phi0, mu0, params, nsteps, every, precision):
         p = params_with_theta(params, theta.detach().cpu().numpy())
         s = GPSolver(p, precision)
-        s.upload(phi0.detach().cpu().numpy(), mu0.detach().cpu().numpy())
-        s.adjoint_forward(int(nsteps), int(every))
-        phi, mu = s.download()
+        try:
+            s.upload(phi0.detach().cpu().numpy(), mu0.detach().cpu().numpy())
+            s.adjoint_forward(int(nsteps), int(every))
+            phi, mu = s.download()
+        except BaseException:
+            s.close()
+            raise
         ctx.solver = s
         ctx.devices = (theta.device, phi0.device, mu0.device)
         return torch.from_numpy(phi).to(phi0.device), torch.from_numpy(mu).to(mu0.device)
@@ -88,11 +92,20 @@ class _Trajectory(torch.autograd.Function):
     @staticmethod
     def backward(ctx, gphi, gmu):
         s = ctx.solver
-        shp_phi = (s.N, s.rx, s.ry)
-        shp_mu = (s.NC, s.rx, s.ry)
-        gphi = np.zeros(shp_phi) if gphi is None else gphi.detach().to(torch.float64).cpu().numpy()
-        gmu = np.zeros(shp_mu) if gmu is None else gmu.detach().to(torch.float64).cpu().numpy()
-        g0, m0, th = s.adjoint_backward(gphi, gmu)
+        if s is None:
+            raise RuntimeError("trajectory: backward ran already; the recorded trajectory and its device buffers "
+                               "are released after the first backward (pass retain_graph=False)")
+        # the handle (checkpoints, segment states, adjoint buffers) is released as soon as the gradients are
+        # out, so that a caller holding the outputs (e.g. the previous iteration's loss) keeps no device memory
+        ctx.solver = None
+        try:
+            shp_phi = (s.N, s.rx, s.ry)
+            shp_mu = (s.NC, s.rx, s.ry)
+            gphi = np.zeros(shp_phi) if gphi is None else gphi.detach().to(torch.float64).cpu().numpy()
+            gmu = np.zeros(shp_mu) if gmu is None else gmu.detach().to(torch.float64).cpu().numpy()
+            g0, m0, th = s.adjoint_backward(gphi, gmu)
+        finally:
+            s.close()
         dt, dp, dm = ctx.devices
         return (torch.from_numpy(th).to(dt), torch.from_numpy(g0).to(dp), torch.from_numpy(m0).to(dm),
                 None, None, None, None)

@@ -95,6 +95,8 @@ def main(argv=None) -> int:
         if it < a.iters:
             opt.step()
             sched.step()
+        # the trajectory's device buffers go with its graph (the last iteration takes no backward)
+        del loss, phiT
     fw = np.mean([h["forward_s"] for h in hist[:-1]]) if len(hist) > 1 else float("nan")
     bw = np.mean([h["backward_s"] for h in hist[:-1]]) if len(hist) > 1 else float("nan")
     print(json.dumps({"summary": {

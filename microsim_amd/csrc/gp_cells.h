@@ -253,6 +253,18 @@ constexpr int kCellBulkDepth = sizeof(R) == 8 ? GP_CELL_DEPTH_F64 : GP_CELL_DEPT
 template <typename R>
 constexpr int kCellRing = kCellBulkDepth<R> + 2;
 
+/* k_cbulk keeps its row offsets in 32 bits. The largest it forms is the refill offset of the last tile: row
+ * x + D + 1 for a row x <= xb + RG - 1 of the unrolled ring (rows past xb run on clamped rows), i.e. below
+ * (xb + RG + D + 1) ps with xb <= mesh_x + 2, before it is clamped to row xb + 1; the in-row offset adds less than
+ * ps. A grid takes the cells path only when all of these fit: (rows_x + kCellSlack) * ps <= INT32_MAX. */
+constexpr int kCellSlack = 32;
+static_assert(kCellRing<float> + kCellBulkDepth<float> + 1 <= kCellSlack - 4 &&
+                  kCellRing<double> + kCellBulkDepth<double> + 1 <= kCellSlack - 4,
+              "the cells path's 32-bit offset bound needs a larger slack");
+__host__ __device__ constexpr bool cells_fit(int64_t rows_x, int64_t ps) {
+  return (rows_x + kCellSlack) * ps <= (int64_t)0x7fffffff;
+}
+
 /* Delta-phi candidates on the ring x or y in {2, mesh+3} around the interior (k_cbulk's tiles list the interior
  * ones): ring cell i (one per thread of k_cbulk's extra blocks), appended to the same list. */
 __device__ __forceinline__ void dring_cell(int64_t i, const int8_t *__restrict__ codes, int32_t *__restrict__ clist,

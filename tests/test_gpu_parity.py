@@ -76,15 +76,17 @@ def test_gpu_matches_oracle_dphi_and_halos(name):
     assert max_rel_err(dphi[sl], odphi[sl]) <= GATE
 
 
-@pytest.mark.parametrize("path", ["march", "twopass"])
+@pytest.mark.parametrize("path", ["cells", "march", "twopass"])
 @pytest.mark.parametrize("name", ["tern_c1_mini", "quad_k3", "bin_well_rot", "tern_aniso_per"])
 def test_gpu_step_paths_match_oracle(name, path, monkeypatch):
-    """The single-pass march kernel (default) and the two-pass kernels both reproduce the oracle."""
+    """Every step path reproduces the oracle: the cell-classified kernels (the default, GP_PATH unset), the tiled
+    march (GP_PATH=march: k_dphi, k_march, and k_bulk on deep tiles once the tile flags of a previous step exist)
+    and the two-pass kernels (GP_PATH=twopass)."""
     _say("[gpu] path", path, name)
-    if path == "twopass":
-        monkeypatch.setenv("GP_PATH", "twopass")
-    else:
+    if path == "cells":
         monkeypatch.delenv("GP_PATH", raising=False)
+    else:
+        monkeypatch.setenv("GP_PATH", path)
     case, _ = load_golden(name)
     s = _solver(case.params)
     try:
@@ -98,6 +100,52 @@ def test_gpu_step_paths_match_oracle(name, path, monkeypatch):
     ep, em = max_rel_err(phi, ophi), max_rel_err(mu, omu)
     _say("   max rel err phi %.3g mu %.3g" % (ep, em))
     assert ep <= GATE and em <= GATE
+
+
+@pytest.mark.parametrize("precision", [_abi.GP_FP64, _abi.GP_FP32])
+def test_gpu_march_deep_tiles(precision, monkeypatch):
+    """The tiled march over 60 steps of a 256^2 ternary seed deck, where most tiles lie inside the liquid: from the
+    second step on, k_bulk updates those tiles from the previous step's tile flags. FP64: against the oracle (1e-12)
+    and bit-identical to the cell-classified step. FP32: within FP32 rounding of the cell-classified step (their
+    bulk updates round differently)."""
+    from microsim_amd import configs, host
+    import os
+    import tempfile
+    _say("[gpu] march deep tiles", precision)
+    inp, fills, _ = configs.config_ternary(256, 8, seed=1234, min_spacing=60.0)
+    d = tempfile.mkdtemp(prefix="gp_march_")
+    configs.write_case(d, inp, fills)
+    case = host.load_case(os.path.join(d, "Input.in"), os.path.join(d, "Filling.in"))
+    n = 60
+    outs = {}
+    for path in ("march", "cells"):
+        if path == "cells":
+            monkeypatch.delenv("GP_PATH", raising=False)
+        else:
+            monkeypatch.setenv("GP_PATH", path)
+        s = _solver(case.params, precision)
+        try:
+            s.upload(case.phi, case.mu)
+            s.step(n)
+            outs[path] = s.download()
+        finally:
+            s.close()
+    if precision == _abi.GP_FP64:
+        ophi, omu = case.phi.copy(), case.mu.copy()
+        oracle_step(case.params, ophi, omu, n, threads=8)
+        ep, em = max_rel_err(outs["march"][0], ophi), max_rel_err(outs["march"][1], omu)
+        _say("   march vs oracle: max rel err phi %.3g mu %.3g" % (ep, em))
+        assert ep <= GATE and em <= GATE
+        nd = int((outs["march"][0] != outs["cells"][0]).sum() + (outs["march"][1] != outs["cells"][1]).sum())
+        _say("   march vs cells: differing values %d" % nd)
+        assert nd == 0
+    else:
+        # FP32 bulk cells: k_bulk takes the flux form, k_cbulk the closed form Kbulk . lap(mu); they agree to FP32
+        # rounding
+        ep = np.abs(outs["march"][0] - outs["cells"][0]).max()
+        em = np.abs(outs["march"][1] - outs["cells"][1]).max() / max(1.0, np.abs(outs["cells"][1]).max())
+        _say("   march vs cells: max abs diff phi %.3g, mu (relative to max |mu|) %.3g" % (ep, em))
+        assert ep <= 1e-5 and em <= 1e-5
 
 
 def test_gpu_stats_match_oracle():

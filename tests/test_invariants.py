@@ -6,6 +6,7 @@ The oracle's definition is pinned by closed forms on states where every term is 
   - a two-phase state whose phase fields are linear in x (phi_a = s x + t, phi_b = 1 - phi_a): then
     q_ab = phi_a grad phi_b - phi_b grad phi_a = -s x_hat exactly, |q|^2 = s^2, the 4-fold a_c of a q along
     an axis is 1 + dab, and the obstacle term is (16/pi^2) Gamma phi_a phi_b;
+  - a uniform four-phase mixture with third-order potential terms, under the obstacle and the well potential;
 and the device kernel (libgp_hip, gp_get_stats) is checked against the oracle on evolved states.
 """
 import ctypes as C
@@ -82,6 +83,50 @@ def test_invariants_linear_profile_closed_form():
     w = (16.0 / math.pi ** 2) * g * (pa * pb).sum()
     want = (bulk + p.epsilon * grad + w / p.epsilon) * area
     assert en == pytest.approx(want, rel=1e-10)
+
+
+def _h_closed(phi, a):
+    """h_a of N phases: 3 phi_a^2 - 2 phi_a^3 + 2 phi_a sum_{b<c; b,c != a} phi_b phi_c"""
+    N = len(phi)
+    s = sum(phi[b] * phi[c] for b in range(N) for c in range(b + 1, N) if a not in (b, c))
+    return 3 * phi[a] ** 2 - 2 * phi[a] ** 3 + 2 * phi[a] * s
+
+
+@pytest.mark.parametrize("fn_w", [1, 2])
+def test_invariants_uniform_mixture_potentials_closed_form(fn_w):
+    """A uniform four-phase mixture (no gradient): every cell carries sum_a h_a psi_a + w(phi)/eps with the
+    obstacle potential w = (16/pi^2) sum_{a<b} Gamma_ab phi_a phi_b or the well potential
+    w = 9 sum_{a<b} Gamma_ab phi_a^2 phi_b^2, plus the third-order term sum_{a<b<c} Gamma_abc phi_a phi_b phi_c in
+    both (quad_k3 has nonzero Gamma_abc)."""
+    case, _ = load_golden("quad_k3")
+    p = _abi.GPParams.from_buffer_copy(case.params)
+    p.fn_w = fn_w
+    N, NC = case.N, case.NC
+    assert N == 4 and any(v != 0.0 for v in p.gamma_abc)
+    ph = np.array([0.1, 0.2, 0.3, 0.4])
+    mu_val = np.array([0.05, -0.07])
+    phi = np.empty_like(case.phi)
+    for a in range(N):
+        phi[a] = ph[a]
+    mu = np.empty_like(case.mu)
+    for k in range(NC):
+        mu[k] = mu_val[k]
+    sol, en = _inv(p, phi, mu)
+    cells = int(p.mesh_x) * int(p.mesh_y)
+    area = p.dx * p.dy
+    h = [_h_closed(ph, a) for a in range(N)]
+    cs, psis = zip(*[_c_and_psi(p, mu_val, a) for a in range(N)])
+    G = np.array(p.gamma).reshape(6, 6)
+    G3 = np.array(p.gamma_abc).reshape(6, 6, 6)
+    if fn_w == 1:
+        w = (16.0 / math.pi ** 2) * sum(G[a, b] * ph[a] * ph[b] for a in range(N) for b in range(a + 1, N))
+    else:
+        w = 9.0 * sum(G[a, b] * ph[a] ** 2 * ph[b] ** 2 for a in range(N) for b in range(a + 1, N))
+    w += sum(G3[a, b, c] * ph[a] * ph[b] * ph[c] for a in range(N) for b in range(a + 1, N) for c in range(b + 1, N))
+    want_e = cells * area * (sum(h[a] * psis[a] for a in range(N)) + w / p.epsilon)
+    want_c = cells * area * sum(h[a] * cs[a] for a in range(N))
+    np.testing.assert_allclose(sol, want_c, rtol=1e-12)
+    assert en == pytest.approx(want_e, rel=1e-12)
 
 
 @pytest.mark.gpu
