@@ -11,16 +11,17 @@ the whole grid; one cell-update is one interior cell advanced by one step.
 
 Prints one JSON line. ``value`` is cell-updates/s of the first mode in --modes with the state resident in HBM,
 every step including the reference loop's per-step statistics (``no_stats`` gives the same steps without
-them); the other modes appear under ``modes``. ``roofline`` is for the dominant kernel: the algorithmic bytes
-2 (N + NC) sizeof(real) per cell-update (phi and mu read once and written once) times the cells that one launch
-advances (k_cbulk: the bulk cells, k_ilist: the interface cells, counted on the state before and after the timed
-steps), over that kernel's mean duration, timed with HIP events on the solver's stream inside the timed region.
-``roofline_step`` is the same for the whole step: every interior cell over the summed mean durations of the
-step's kernels (delta-phi, both updates, statistics fold); ``roofline_kernels`` lists each update kernel.
-A streaming-copy kernel is timed on the same device as a measured bandwidth reference. ``cpu_baseline`` times
-the serial reference binary (oracle/_ref, built from the reference sources) on a bounded sample: the same
-ternary model on 1024 x 1024 with 64 seeds, per-step wall time from two runs that differ only in their step
-count.
+them), wall clock of one gp_step call; the other modes appear under ``modes``. ``roofline`` is for the whole step:
+the algorithmic bytes 2 (N + NC) sizeof(real) per cell-update (phi and mu read once and written once) times every
+interior cell, over the step's duration from HIP events recorded on the solver's stream around the whole call
+(nothing is recorded between its launches). ``roofline_kernels`` gives each update kernel on its own: the cells one
+launch advances (k_cbulk: the bulk cells, k_ilist: the interface cells) over that kernel's mean duration, from a
+separate pass with events around every launch. ``late`` repeats the timings after --late-at steps of the same run,
+where the seeds have grown (its interface fraction is reported). A streaming-copy kernel is timed on the same device
+as a measured bandwidth reference. ``cpu_baseline`` times the serial reference binary (oracle/_ref, built from the
+reference sources) on a bounded sample: the same ternary model on 1024 x 1024 with 64 seeds, per-step wall time from
+two runs that differ only in their step count. ``configs`` reports FP64 throughput on configurations 0 (binary seed,
+200^2) and 1 (ternary, 1024^2) next to the serial reference on the same grids.
 """
 from __future__ import annotations
 
@@ -87,67 +88,91 @@ def cell_classes(phi: "np.ndarray"):
     return nb, int(ok.size) - nb
 
 
-def run_mode(case, precision: int, steps: int, warmup: int):
+def _timed_call(s, steps: int, stats: bool):
+    """(wall seconds, event ms) of one gp_step call of `steps` steps; the events bracket the whole call on the
+    solver's stream (GP_PROFILE_CALLS: nothing is recorded between its launches)."""
+    s.profile_read()
+    t0 = time.perf_counter()
+    st = s.step(steps, stats=stats)
+    s.synchronize()
+    el = time.perf_counter() - t0
+    ev = s.profile_read()["call"][0]
+    return el, ev, st
+
+
+def run_mode(case, precision: int, steps: int, warmup: int, late_at: int = 0):
     """Time `steps` solver steps with the reference loop's per-step statistics (cumulative max/min, sums of
-    squared increments) folded on the device every step, as the driver runs them."""
+    squared increments) folded on the device every step, as the driver runs them; then the same steps without the
+    statistics work. Both timings take the whole gp_step call: wall clock and HIP events on the solver's stream,
+    with no per-launch instrumentation. A separate pass brackets every launch with events for the per-kernel split.
+    late_at > 0: the run continues to step late_at and the timings are repeated there (the interface fraction
+    grows as the seeds grow)."""
+    from microsim_amd import _abi
     from microsim_amd.solver import GPSolver
     N, NC = case.N, case.NC
     cells = int(case.params.mesh_x) * int(case.params.mesh_y)
     es = 8 if precision == 0 else 4
+    b_alg = 2 * (N + NC) * es
+
+    def window(s, done):
+        phi_a, _ = s.download()
+        s.profile_enable(_abi.GP_PROFILE_CALLS)
+        el, ev, st = _timed_call(s, steps, True)
+        el_ns, ev_ns, _ = _timed_call(s, steps, False)
+        # per-kernel split: events around every launch (separate steps, not part of the timings above)
+        s.profile_enable(_abi.GP_PROFILE_LAUNCHES)
+        s.profile_read()
+        s.step(steps, stats=True)
+        prof = s.profile_read()
+        s.step(steps)
+        prof_ns = s.profile_read()
+        s.profile_enable(False)
+        phi_b, _ = s.download()
+        cls = [cell_classes(p) for p in (phi_a, phi_b)]
+        nbulk = 0.5 * (cls[0][0] + cls[1][0])
+        kcells = {"k_cbulk": nbulk, "k_ilist": cells - nbulk, "k_march": cells, "k_march_stats": cells}
+        per_kernel = {}
+        for k, (ms, cnt) in prof.items():
+            if k in kcells and ms > 0:
+                gbs = kcells[k] * b_alg / (ms * 1e-3) / 1e9
+                per_kernel[k] = {"ms": round(ms, 4), "cells": int(kcells[k]), "GBps": round(gbs, 1),
+                                 "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic_per_launch(k, precision)}
+        step_ms = ev / steps
+        step_gbs = cells * b_alg / (step_ms * 1e-3) / 1e9
+        traffic = [traffic_per_launch(k, precision) for k in prof]
+        return {
+            "at_step": done,
+            "value": cells * steps / el,
+            "ms_per_step": el * 1e3 / steps,
+            "event_ms_per_step": round(step_ms, 5),
+            "interface_cell_fraction": round(1.0 - nbulk / cells, 4),
+            "kernels_ms": {k: round(v[0], 4) for k, v in prof.items()},
+            "kernel_sum_ms": round(sum(ms for ms, cnt in prof.values()), 4),
+            "no_stats": {"value": cells * steps / el_ns, "ms_per_step": el_ns * 1e3 / steps,
+                         "event_ms_per_step": round(ev_ns / steps, 5),
+                         "kernels_ms": {k: round(v[0], 4) for k, v in prof_ns.items()}},
+            "roofline": {"bound": "hbm", "achieved": round(step_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(step_gbs / HBM_PEAK_GBS, 4),
+                         "traffic": sum(traffic) if traffic and all(t is not None for t in traffic) else None,
+                         "kernel": "whole step (%s), HIP events around one call of %d steps"
+                                   % (" + ".join(sorted(prof)), steps),
+                         "alg_bytes_per_cell_update": b_alg, "cells_per_launch": cells},
+            "roofline_kernels": per_kernel,
+            "nonfinite": st["nonfinite"],
+        }
+
     with GPSolver(case.params, precision) as s:
         s.upload(case.phi, case.mu)
         s.step(warmup, stats=True)
         s.synchronize()
-        s.profile_enable(True)
-        s.profile_read()
-        phi_a, _ = s.download()
-        t0 = time.perf_counter()
-        st = s.step(steps, stats=True)
-        s.synchronize()
-        el = time.perf_counter() - t0
-        prof = s.profile_read()
-        phi_b, _ = s.download()
-        # the same steps without the statistics work (a caller that does not ask for diagnostics)
-        t0 = time.perf_counter()
-        s.step(steps)
-        s.synchronize()
-        el_ns = time.perf_counter() - t0
-        prof_ns = s.profile_read()
-        s.profile_enable(False)
-    step_ms = sum(ms for ms, cnt in prof.values())
-    b_alg = 2 * (N + NC) * es
-    # cells each update kernel advances per launch: the mean of the bulk / interface split before and after the
-    # timed steps
-    cls = [cell_classes(p) for p in (phi_a, phi_b)]
-    nbulk = 0.5 * (cls[0][0] + cls[1][0])
-    kcells = {"k_cbulk": nbulk, "k_ilist": cells - nbulk, "k_march": cells, "k_march_stats": cells}
-    per_kernel = {}
-    for k, (ms, cnt) in prof.items():
-        if k in kcells and ms > 0:
-            gbs = kcells[k] * b_alg / (ms * 1e-3) / 1e9
-            per_kernel[k] = {"ms": round(ms, 4), "cells": int(kcells[k]), "GBps": round(gbs, 1),
-                             "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic_per_launch(k, precision)}
-    dom = max(per_kernel, key=lambda k: per_kernel[k]["ms"])
-    d = per_kernel[dom]
-    step_gbs = cells * b_alg / (step_ms * 1e-3) / 1e9
-    return {
-        "value": cells * steps / el,
-        "ms_per_step": el * 1e3 / steps,
-        "kernels_ms": {k: round(v[0], 4) for k, v in prof.items()},
-        "step_kernels_ms": round(step_ms, 4),
-        "dominant": dom,
-        "bulk_cell_fraction": round(nbulk / cells, 4),
-        "no_stats": {"value": cells * steps / el_ns, "ms_per_step": el_ns * 1e3 / steps,
-                     "kernels_ms": {k: round(v[0], 4) for k, v in prof_ns.items()}},
-        "roofline": {"bound": "hbm", "achieved": d["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": d["frac"], "traffic": d["traffic"], "kernel": dom,
-                     "alg_bytes_per_cell_update": b_alg, "cells_per_launch": d["cells"]},
-        "roofline_step": {"achieved": round(step_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                          "frac": round(step_gbs / HBM_PEAK_GBS, 4),
-                          "kernels": sorted(prof), "cells_per_step": cells, "alg_bytes_per_cell_update": b_alg},
-        "roofline_kernels": per_kernel,
-        "nonfinite": st["nonfinite"],
-    }
+        done = warmup
+        res = window(s, done)
+        done += 4 * steps
+        if late_at > done:
+            s.step(late_at - done, stats=True)
+            done = late_at
+            res["late"] = window(s, done)
+    return res
 
 
 def adjoint_ratio(params, phi0, mu0, steps: int, every: int = 0, precision: int = None):
@@ -277,29 +302,84 @@ def precision_errors(case, steps: int, checkpoints=(100, 500, 1000, 2000, 5000, 
     return res
 
 
-def cpu_baseline(sample_steps: int = 20):
-    """Serial reference binary (1 core) on the 1024^2 ternary sample; falls back to the oracle port."""
-    from microsim_amd import configs
+def reference_rate(inp, fills, restart_state, sample_steps: int):
+    """Cell-updates/s of the serial reference binary (oracle/_ref, built from the reference sources, 1 core) on the
+    given case: the wall-time difference of a (2 + sample_steps)-step and a 2-step run (set-up and the initial
+    write cancel). None when the binary is absent or fails."""
     ref = os.path.join(REPO, "oracle", "_ref", "microsim_gp")
+    if not os.path.exists(ref):
+        return None
+    from microsim_amd import configs
     times = {}
-    if os.path.exists(ref):
-        for n in (2, 2 + sample_steps):
-            inp, fills, _ = configs.config_ternary(1024, 64, seed=1234, steps=n, savet=n)
-            inp.track_progress = n
-            d = tempfile.mkdtemp(prefix="gp_cpu_")
-            configs.write_case(d, inp, fills)
-            t = time.perf_counter()
-            r = subprocess.run([ref, "Input.in", "Filling.in", "o"], cwd=d, capture_output=True, text=True)
-            times[n] = time.perf_counter() - t
-            if r.returncode != 0:
-                times = {}
-                break
-        if times:
-            per = (times[2 + sample_steps] - times[2]) / sample_steps
-            return {"value": 1024 * 1024 / per, "unit": "cell-updates/s", "cores": 1, "kind": "reference",
-                    "sample": "serial reference microsim_gp (gcc -O2), ternary 3-phase 1024x1024, 64 seeds, "
-                              "%d steps (difference of a %d- and a 2-step run)" % (sample_steps, 2 + sample_steps)}
-    return None
+    for n in (2, 2 + sample_steps):
+        inp.ntimesteps, inp.savet, inp.track_progress = n, n, n
+        d = tempfile.mkdtemp(prefix="gp_cpu_")
+        configs.write_case(d, inp, fills, restart_state=restart_state)
+        t = time.perf_counter()
+        r = subprocess.run([ref, "Input.in", "Filling.in", "o"], cwd=d, capture_output=True, text=True)
+        times[n] = time.perf_counter() - t
+        if r.returncode != 0:
+            return None
+    per = (times[2 + sample_steps] - times[2]) / sample_steps
+    return int(inp.mesh_x) * int(inp.mesh_y) / per
+
+
+def cpu_baseline(sample_steps: int = 20):
+    """Serial reference binary (1 core) on the 1024^2 ternary sample (64 seeds: the seed density of the headline
+    workload)."""
+    from microsim_amd import configs
+    inp, fills, _ = configs.config_ternary(1024, 64, seed=1234)
+    v = reference_rate(inp, fills, None, sample_steps)
+    if v is None:
+        return None
+    return {"value": v, "unit": "cell-updates/s", "cores": 1, "kind": "reference",
+            "sample": "serial reference microsim_gp (gcc -O2), ternary 3-phase 1024x1024, 64 seeds (the headline "
+                      "workload's seed density), %d steps (difference of a %d- and a 2-step run)"
+                      % (sample_steps, 2 + sample_steps)}
+
+
+def config_rates(cpu_c1=None, c0_steps: int = 2000, c1_steps: int = 200):
+    """FP64 throughput of the two reference-scale configurations next to the serial reference on the same grid:
+    config 0 (binary two-phase anisotropic seed, 200^2, tanh seed of radius 10 via a restart file, Neumann sides)
+    and config 1 (ternary three-phase, 1024^2, 64 seeds, periodic). GPU: wall clock of one gp_step call with the
+    per-step statistics."""
+    from microsim_amd import _abi, configs, host
+    from microsim_amd.solver import GPSolver
+    out = {}
+    # config 0
+    inp, _ = configs.config_c0(200, c0_steps, c0_steps)
+    d = tempfile.mkdtemp(prefix="gp_c0_")
+    configs.write_case(d, inp, [])
+    phi_i, mu_i = configs.tanh_seed_state(200, 200, 10.0, 2.0, [2.0 * 1.0 * 0.34])
+    case0 = host.state_case(os.path.join(d, "Input.in"), phi_i, mu_i)
+    with GPSolver(case0.params, _abi.GP_FP64) as s:
+        s.upload(case0.phi, case0.mu)
+        s.step(20, stats=True)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.step(c0_steps, stats=True)
+        s.synchronize()
+        el = time.perf_counter() - t0
+    g0 = 200 * 200 * c0_steps / el
+    c0 = reference_rate(inp, [], (phi_i, mu_i, "o"), 200)
+    out["c0_binary200_fp64"] = {"gpu": g0, "gpu_ms_per_step": el * 1e3 / c0_steps, "cpu_reference_1core": c0,
+                                "gpu_over_cpu": (g0 / c0) if c0 else None, "unit": "cell-updates/s",
+                                "steps": c0_steps}
+    # config 1
+    case1, _ = make_case(1024, 64)
+    with GPSolver(case1.params, _abi.GP_FP64) as s:
+        s.upload(case1.phi, case1.mu)
+        s.step(5, stats=True)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.step(c1_steps, stats=True)
+        s.synchronize()
+        el = time.perf_counter() - t0
+    g1 = 1024 * 1024 * c1_steps / el
+    out["c1_ternary1024_fp64"] = {"gpu": g1, "gpu_ms_per_step": el * 1e3 / c1_steps,
+                                  "cpu_reference_1core": cpu_c1, "gpu_over_cpu": (g1 / cpu_c1) if cpu_c1 else None,
+                                  "unit": "cell-updates/s", "steps": c1_steps}
+    return out
 
 
 def oracle_threads(case_small, nthreads: int, steps: int = 3):
@@ -323,6 +403,9 @@ def main():
     ap.add_argument("--seeds", type=int, default=1024)
     ap.add_argument("--modes", default="fp32,fp64")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--late-at", type=int, default=5000,
+                    help="repeat the timings after this many steps of the same run (0: skip)")
+    ap.add_argument("--no-configs", action="store_true", help="skip the config 0 / config 1 FP64 figures")
     ap.add_argument("--adjoint-steps", type=int, default=20, help="steps of the adjoint timing (0: skip)")
     ap.add_argument("--error-steps", type=int, default=10000,
                     help="steps of the FP32-vs-FP64 error study at the bench mesh and at 1024^2 (0: skip)")
@@ -338,7 +421,7 @@ def main():
     res = {}
     for mode in a.modes.split(","):
         prec = _abi.GP_FP32 if mode == "fp32" else _abi.GP_FP64
-        res[mode] = run_mode(case, prec, a.steps, a.warmup)
+        res[mode] = run_mode(case, prec, a.steps, a.warmup, a.late_at)
         log(mode, json.dumps(res[mode]))
     first = a.modes.split(",")[0]
     r0 = res[first]
@@ -350,7 +433,7 @@ def main():
                    "mesh": [a.mesh, a.mesh], "phases": 3, "components": 3, "seeds": a.seeds,
                    "parallelism": "single-gpu"},
         "roofline": dict(r0["roofline"], measured_copy_GBps=copy["GBps"]),
-        "roofline_step": r0["roofline_step"],
+        "roofline_kernels": r0["roofline_kernels"],
         "copy_kernel": copy,
         "modes": {m: {k: v for k, v in r.items()} for m, r in res.items()},
     }
@@ -376,6 +459,7 @@ def main():
             errs.append(precision_errors(make_case(1024, 64)[0], a.error_steps))
         out["fp32_error_vs_fp64"] = errs
         log("fp32 error", json.dumps(errs))
+    cb = None
     if not a.no_cpu:
         cb = cpu_baseline()
         if cb is not None:
@@ -387,6 +471,9 @@ def main():
                                              "kind": "port", "unit": "cell-updates/s"}
             except Exception as e:  # extra figure only
                 log("threaded oracle skipped:", e)
+    if not a.no_configs:
+        out["configs"] = config_rates(cb["value"] if cb else None)
+        log("configs", json.dumps(out["configs"]))
     print(json.dumps(out), flush=True)
 
 

@@ -171,10 +171,14 @@ void gp_destroy(gp_handle *h);
 /* Library version string, e.g. "gp_hip 0.1 gfx950". */
 const char *gp_version(void);
 
-/* Measurement support (no counterpart in the reference; for benchmarks and profilers). While enabled, every
- * kernel launch issued by gp_step / gp_smooth is bracketed by HIP events recorded on the handle's stream.
- * gp_profile_read synchronises, returns the mean duration in ms and the launch count per kernel kind
- * (arrays of GP_NKERNELS) accumulated since the previous read, and resets them. */
+/* Measurement support (no counterpart in the reference; for benchmarks and profilers). In GP_PROFILE_LAUNCHES mode
+ * every kernel launch issued by gp_step / gp_smooth is bracketed by HIP events recorded on the handle's stream; in
+ * GP_PROFILE_CALLS mode only each whole gp_step / gp_smooth call is (kind GP_KERNEL_CALL), which adds nothing
+ * between the launches of a call. gp_profile_read synchronises, returns the mean duration in ms and the count per
+ * kind (arrays of GP_NKERNELS) accumulated since the previous read, and resets them. */
+#define GP_PROFILE_OFF 0
+#define GP_PROFILE_LAUNCHES 1
+#define GP_PROFILE_CALLS 2
 #define GP_KERNEL_PHI 0      /* delta-phi kernel */
 #define GP_KERNEL_MU 1       /* anti-trapping + mu/phi update kernel */
 #define GP_KERNEL_STATS 2    /* statistics fold */
@@ -185,8 +189,10 @@ const char *gp_version(void);
 #define GP_KERNEL_ADJ 7      /* one transposed step of gp_adjoint_backward (its segment recompute not included) */
 #define GP_KERNEL_CBULK 8    /* update of the bulk cells (cell-classified step) */
 #define GP_KERNEL_CIFACE 9   /* update of the interface cells (cell-classified step, k_ilist) */
-#define GP_NKERNELS 10
-int gp_profile_enable(gp_handle *h, int enable);
+#define GP_KERNEL_CALL 10    /* one whole gp_step / gp_smooth call (GP_PROFILE_CALLS mode) */
+#define GP_NKERNELS 11
+/* mode: GP_PROFILE_OFF, GP_PROFILE_LAUNCHES (any other nonzero value) or GP_PROFILE_CALLS */
+int gp_profile_enable(gp_handle *h, int mode);
 int gp_profile_read(gp_handle *h, double *avg_ms, int64_t *count);
 const char *gp_kernel_name(int kind);
 

@@ -7,10 +7,13 @@ Outputs go to microsim_amd/lib/: libgp_host.so, libgp_hip.so (gfx950 code object
 """
 from __future__ import annotations
 
+import concurrent.futures
 import os
+import re
 import shutil
 import subprocess
 import sys
+import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
@@ -45,14 +48,37 @@ def build_host():
                                   os.path.join(HERE, "host", "gp_host.c"), "-lm"])
 
 
-def build_hip():
+def combos():
+    """The (phases, mu fields) pairs the kernels are instantiated for (GP_COMBOS in csrc/gp_internal.h)."""
+    txt = open(os.path.join(HERE, "csrc", "gp_internal.h")).read()
+    m = re.search(r"#define GP_COMBOS\(X\)(.*)", txt)
+    return [(int(a), int(b)) for a, b in re.findall(r"X\((\d+),\s*(\d+)\)", m.group(1))]
+
+
+def build_hip(jobs: int = 0):
+    """Compile gp_hip.hip (C ABI) and gp_ops.hip once per (phases, mu fields) pair in parallel, then link."""
     os.makedirs(LIB, exist_ok=True)
-    srcs = [os.path.join(HERE, "csrc", f) for f in sorted(os.listdir(os.path.join(HERE, "csrc")))
-            if f.endswith(".hip")]
+    csrc = os.path.join(HERE, "csrc")
+    objdir = tempfile.mkdtemp(prefix="gp_hip_obj_")
+    units = [(os.path.join(csrc, "gp_hip.hip"), [], "gp_hip.o")]
+    units += [(os.path.join(csrc, "gp_ops.hip"), ["-DGP_N=%d" % n, "-DGP_NC=%d" % nc], "gp_ops_%d_%d.o" % (n, nc))
+              for n, nc in combos()]
+    cflags = [f for f in HIP_FLAGS if f != "-shared"]
+
+    def compile_unit(u):
+        src, defs, obj = u
+        _run([hipcc()] + cflags + defs + ["-I" + INC, "-I" + csrc, "-c", "-o", os.path.join(objdir, obj), src])
+
+    jobs = jobs or min(len(units), max(1, int(os.environ.get("MAX_JOBS") or os.cpu_count() or 1)))
+    with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as ex:
+        for f in [ex.submit(compile_unit, u) for u in units]:
+            f.result()
     out = os.path.join(LIB, "libgp_hip.so")
     tmp = out + ".tmp"
-    _run([hipcc()] + HIP_FLAGS + ["-I" + INC, "-I" + os.path.join(HERE, "csrc"), "-o", tmp] + srcs)
+    _run([hipcc(), "-shared", "-fPIC", "--offload-arch=%s" % ARCH, "-o", tmp] +
+         [os.path.join(objdir, u[2]) for u in units])
     os.replace(tmp, out)
+    shutil.rmtree(objdir, ignore_errors=True)
 
 
 def build_driver():

@@ -1,7 +1,7 @@
 /*
  * gp_adjoint.h -- adjoint (reverse-mode gradient) of the FP64 solver step, with checkpoint and recompute.
  *
- * Included by gp_hip.hip after the handle and its helpers; not a public header.
+ * Included by the kernel translation units after the handle and its helpers; not a public header.
  *
  * One forward step maps the state s_t (phi, mu, halo-inclusive) to s_{t+1} in three stages, each a per-cell
  * function of a fixed stencil:
@@ -11,119 +11,19 @@
  *      stencil (jat_x / jat_y / mu_update: calculate_fluxes_concentration_2D and
  *      calculate_divergence_concentration_2D),
  *   B  the boundary copies (each halo cell takes the value of one interior cell).
- * The adjoint step runs them backwards, transposed: for a cell, the block of its local Jacobian that a kernel
- * needs is evaluated exactly with forward-mode dual numbers (gp_dual.h) through the same per-cell functions the
- * forward step uses, K input directions per pass, and the products with the incoming adjoint are scatter-added
- * (FP64 atomics) onto the stencil's inputs and onto the parameter gradient. Cells whose incoming adjoint is zero
- * are skipped. B's transpose adds each halo cell's adjoint to its source cell.
- *
- * Parameters theta and the dual model that seeds them: gp_theta.h.
+ * The adjoint step runs them backwards, transposed. A cell of stage P or Q is transposed in one pass by the
+ * reverse-mode per-cell functions of gp_reverse.h (its forward values recomputed, then the incoming adjoint carried
+ * back through every operation), and the result is scatter-added (FP64 atomics) onto the stencil's inputs and onto
+ * the parameter gradient. Bulk cells of stage Q are transposed in closed form by a gather (k_adj_bulk). Cells whose
+ * incoming adjoint is zero are skipped. B's transpose adds each halo cell's adjoint to its source cell.
  */
 #pragma once
 
 #include <vector>
 
-#include "gp_theta.h"
+#include "gp_reverse.h"
 
 namespace gpk {
-
-/* Directions per dual pass. Measured on the 4096^2 ternary workload (reverse step, FP64): K = 8 113 ms, 4 92 ms,
- * 2 64 ms, 1 43 ms -- the per-cell functions in Dual<K> are register-bound, and one direction per pass keeps them
- * in registers at the price of recomputing the values every pass. */
-#ifndef GP_ADJ_K
-#define GP_ADJ_K 1
-#endif
-constexpr int kAdjK = GP_ADJ_K;
-
-/* The parameters each stage depends on, seeded as directions after the stage's state inputs: stage P
- * (phi_delta) reads the pair parameters and tau but no diffusivity; stage Q (currents and mu update) reads only
- * the liquid's diffusivities. Direction j of a stage's parameter block is theta entry stage_theta(stage, j). */
-enum { kStageP = 0, kStageQ = 1 };
-__host__ __device__ inline int stage_theta_count(int stage, int N, int NC) {
-  return stage == kStageP ? 3 * (N * (N - 1) / 2) + 1 : NC;
-}
-__host__ __device__ inline int stage_theta(int stage, int j, int N, int NC) {
-  const int np3 = 3 * (N * (N - 1) / 2);
-  if (stage == kStageP) return j < np3 ? j : np3 + NC;
-  return np3 + j;
-}
-
-/* Stencil accessor that seeds the inputs of one stage. MODE 0 (stage P): phi on the 3x3 stencil, mu at the
- * centre. MODE 1 (stage Q): phi on the 3x3 stencil, mu and delta-phi on the 5-point stencil. Directions:
- *   phi_a at (i,j)       a*9 + (i+1)*3 + (j+1)
- *   mu_k                 9N + k (P)  |  9N + k*5 + o(i,j) (Q)
- *   dphi_a (Q)           9N + 5NC + a*5 + o(i,j)
- * with o(0,0) = 0, o(1,0) = 1, o(-1,0) = 2, o(0,1) = 3, o(0,-1) = 4. */
-__device__ __forceinline__ int five_point(int i, int j) {
-  return (i == 0 && j == 0) ? 0 : (i == 1 && j == 0) ? 1 : (i == -1 && j == 0) ? 2 : (i == 0 && j == 1) ? 3 : 4;
-}
-
-template <int N, int NC, int K, bool P2, int MODE>
-struct SeedView {
-  static constexpr bool kPow2 = P2;
-  const double *phi, *mu, *dphi;
-  int64_t fs, ps, c;
-  int si, sj, chunk0;
-  __device__ Dual<K> seed(double x, int dir) const {
-    Dual<K> r(x);
-    const int k = dir - chunk0;
-    if (k >= 0 && k < K) r.d[k] = 1.0;
-    return r;
-  }
-  __device__ Dual<K> ph(int a, int i, int j) const {
-    const int oi = si + i, oj = sj + j;
-    return seed(phi[a * fs + c + oi * ps + oj], a * 9 + (oi + 1) * 3 + (oj + 1));
-  }
-  __device__ Dual<K> m(int k, int i, int j) const {
-    const int oi = si + i, oj = sj + j;
-    const int dir = MODE == 0 ? 9 * N + k : 9 * N + k * 5 + five_point(oi, oj);
-    return seed(mu[k * fs + c + oi * ps + oj], dir);
-  }
-  __device__ Dual<K> dp(int a, int i, int j) const {
-    const int oi = si + i, oj = sj + j;
-    return seed(dphi[a * fs + c + oi * ps + oj], 9 * N + 5 * NC + a * 5 + five_point(oi, oj));
-  }
-  __device__ SeedView shift(int i, int j) const {
-    SeedView v = *this;
-    v.si += i;
-    v.sj += j;
-    return v;
-  }
-};
-
-/* scatter one direction's adjoint contribution g onto the stencil input it belongs to */
-template <int N, int NC, int MODE>
-__device__ __forceinline__ void scatter_dir(int dir, double g, int64_t c, int64_t fs, int64_t ps, double *gphi,
-                                           double *gmu, double *gdphi, double *th, int nth) {
-  if (dir < 9 * N) {
-    const int a = dir / 9, r = dir % 9, i = r / 3 - 1, j = r % 3 - 1;
-    atomicAdd(&gphi[a * fs + c + i * ps + j], g);
-    return;
-  }
-  dir -= 9 * N;
-  const int nmu = MODE == 0 ? NC : 5 * NC;
-  const int o2i[5] = {0, 1, -1, 0, 0}, o2j[5] = {0, 0, 0, 1, -1};
-  if (dir < nmu) {
-    if (MODE == 0) {
-      atomicAdd(&gmu[dir * fs + c], g);
-    } else {
-      const int k = dir / 5, o = dir % 5;
-      atomicAdd(&gmu[k * fs + c + o2i[o] * ps + o2j[o]], g);
-    }
-    return;
-  }
-  dir -= nmu;
-  if (MODE == 1) {
-    if (dir < 5 * N) {
-      const int a = dir / 5, o = dir % 5;
-      atomicAdd(&gdphi[a * fs + c + o2i[o] * ps + o2j[o]], g);
-      return;
-    }
-    dir -= 5 * N;
-  }
-  if (dir < stage_theta_count(MODE == 0 ? kStageP : kStageQ, N, NC))
-    th[stage_theta(MODE == 0 ? kStageP : kStageQ, dir, N, NC)] += g;
-}
 
 /* block sum of the per-thread parameter adjoints, one atomic per block and parameter */
 template <int NTH>
@@ -141,9 +41,6 @@ __device__ __forceinline__ void theta_flush(const double *th, int nth, double *g
       if (s != 0.0) atomicAdd(&gtheta[t], s);
     }
 }
-
-template <int N, int NC>
-constexpr int kTheta = 3 * (N * (N - 1) / 2) + NC + 1;
 
 /* ---------------------------------------------------------------- stage Q of bulk cells, in closed form
  *
@@ -324,43 +221,80 @@ __global__ __launch_bounds__(256) void k_adj_bulk(const Model<double> *__restric
     }
 }
 
-/* Stage Q transposed for directions [chunk0, chunk0 + K), for the listed cells (interior, not bulk, nonzero
- * incoming adjoint; k_adj_bulk covers the bulk cells): a grid-stride loop over the list. */
+/* scatter one cell's stencil adjoint (RevAcc) onto the state adjoints with atomics (zero entries skipped) */
+template <int N, int NC>
+__device__ __forceinline__ void scatter_acc(const RevAcc<N, NC> &acc, int64_t c, int64_t fs, int64_t ps,
+                                            double *gphi, double *gmu, double *gdphi) {
+  const int oi[5] = {0, 1, -1, 0, 0}, oj[5] = {0, 0, 0, 1, -1};
+#pragma unroll
+  for (int a = 0; a < N; a++)
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+        if (acc.p[a][i][j] != 0.0) atomicAdd(&gphi[a * fs + c + (i - 1) * ps + (j - 1)], acc.p[a][i][j]);
+#pragma unroll
+  for (int o = 0; o < 5; o++) {
+    const int64_t co = c + oi[o] * ps + oj[o];
+#pragma unroll
+    for (int k = 0; k < NC; k++)
+      if (acc.u[k][o] != 0.0) atomicAdd(&gmu[k * fs + co], acc.u[k][o]);
+    if (gdphi)
+#pragma unroll
+      for (int a = 0; a < N; a++)
+        if (acc.d[a][o] != 0.0) atomicAdd(&gdphi[a * fs + co], acc.d[a][o]);
+  }
+}
+
+/* the stencil of cell c: phi on the 3x3 neighbourhood, mu and (when dphi is given) delta-phi on the 5-point one */
+template <int N, int NC>
+__device__ __forceinline__ void gather_stencil(const double *__restrict__ phi, const double *__restrict__ mu,
+                                               const double *__restrict__ dphi, int64_t c, int64_t fs, int64_t ps,
+                                               RegStore<N, NC, double> &rs) {
+  const int oi[5] = {0, 1, -1, 0, 0}, oj[5] = {0, 0, 0, 1, -1};
+#pragma unroll
+  for (int a = 0; a < N; a++)
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) rs.p[a][i][j] = phi[a * fs + c + (i - 1) * ps + (j - 1)];
+#pragma unroll
+  for (int o = 0; o < 5; o++) {
+    const int64_t co = c + oi[o] * ps + oj[o];
+#pragma unroll
+    for (int k = 0; k < NC; k++) rs.u[k][o] = mu[k * fs + co];
+#pragma unroll
+    for (int a = 0; a < N; a++) rs.d[a][o] = dphi ? dphi[a * fs + co] : 0.0;
+  }
+}
+
+/* Stage Q transposed by reverse mode (gp_reverse.h: cell_update_vjp), one pass per listed cell (interior, not bulk,
+ * nonzero incoming adjoint; k_adj_bulk covers the bulk cells): a grid-stride loop over the list. */
 template <int N, int NC, bool P2>
-__global__ __launch_bounds__(256) void k_adj_q(const Model<Dual<kAdjK>> *__restrict__ mp, const double *__restrict__ phi,
-                                               const double *__restrict__ mu, const double *__restrict__ dphi,
-                                               const double *__restrict__ gphi_n, const double *__restrict__ gmu_n,
-                                               double *gphi, double *gmu, double *gdphi, double *gtheta,
-                                               const int32_t *__restrict__ list, const int *__restrict__ count,
-                                               int64_t fs, int64_t ps, int chunk0) {
-  constexpr int K = kAdjK;
-  constexpr int NTH = kTheta<N, NC>;
+__global__ __launch_bounds__(256) void k_adj_qrev(const Model<double> *__restrict__ mp, const double *__restrict__ phi,
+                                                  const double *__restrict__ mu, const double *__restrict__ dphi,
+                                                  const double *__restrict__ gphi_n, const double *__restrict__ gmu_n,
+                                                  double *gphi, double *gmu, double *gdphi, double *gtheta,
+                                                  const int32_t *__restrict__ list, const int *__restrict__ count,
+                                                  int64_t fs, int64_t ps) {
+  constexpr int NTH = kThetaN<N, NC>;
   double th[NTH];
   for (int t = 0; t < NTH; t++) th[t] = 0.0;
+  const Model<double> &m = *mp;
   const int n = *count;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int64_t c = list[i];
     const int64_t y = (c - kYOff) % ps;
-    double go[N + NC];
+    double go[N], gu[NC];
     for (int a = 0; a < N; a++) go[a] = gphi_n[a * fs + c];
-    for (int k = 0; k < NC; k++) go[N + k] = gmu_n[k * fs + c];
-    {
-      const Model<Dual<K>> &m = *mp;
-      SeedView<N, NC, K, P2, 1> v{phi, mu, dphi, fs, ps, c, 0, 0, chunk0};
-      Dual<K> jx0[NC], jxb[NC], jy0[NC], jyl[NC], d[N], pn[N], mn[NC], dm[NC];
-      jat_x<N, NC>(m, v, 0, 0, jx0);
-      jat_x<N, NC>(m, v, -1, 0, jxb);
-      jat_y<N, NC>(m, v, 0, 0, y, jy0);
-      jat_y<N, NC>(m, v, 0, -1, y - 1, jyl);
-      for (int a = 0; a < N; a++) d[a] = v.dp(a, 0, 0);
-      mu_update<N, NC>(m, v, false, d, jx0, jxb, jy0, jyl, pn, mn, dm);
-      for (int k = 0; k < K; k++) {
-        double g = 0.0;
-        for (int a = 0; a < N; a++) g += go[a] * pn[a].d[k];
-        for (int q = 0; q < NC; q++) g += go[N + q] * mn[q].d[k];
-        if (g != 0.0) scatter_dir<N, NC, 1>(chunk0 + k, g, c, fs, ps, gphi, gmu, gdphi, th, NTH);
-      }
-    }
+    for (int k = 0; k < NC; k++) gu[k] = gmu_n[k * fs + c];
+    RegStore<N, NC, double> rs;
+    gather_stencil<N, NC>(phi, mu, dphi, c, fs, ps, rs);
+    RevAcc<N, NC> acc;
+    acc.clear();
+    cell_update_vjp<N, NC>(m, RegView<N, NC, double, P2>{&rs, 0, 0}, y, go, gu, acc);
+    scatter_acc<N, NC>(acc, c, fs, ps, gphi, gmu, gdphi);
+    for (int t = 0; t < NTH; t++) th[t] += acc.th[t];
   }
   theta_flush<NTH>(th, NTH, gtheta);
 }
@@ -402,39 +336,36 @@ __global__ __launch_bounds__(256) void k_adj_plist(const double *__restrict__ ph
   list_append(flag, (int32_t)c, plist, pcount);
 }
 
+/* Stage P transposed by reverse mode (gp_reverse.h: phi_delta_vjp), one pass per listed cell */
 template <int N, int NC, bool P2>
-__global__ __launch_bounds__(256) void k_adj_p(const Model<Dual<kAdjK>> *__restrict__ mp, const double *__restrict__ phi,
-                                               const double *__restrict__ mu, const double *__restrict__ gdphi,
-                                               double *gphi, double *gmu, double *gtheta,
-                                               const int32_t *__restrict__ list, const int *__restrict__ count,
-                                               int64_t fs, int64_t ps, int chunk0) {
-  constexpr int K = kAdjK;
-  constexpr int NTH = kTheta<N, NC>;
+__global__ __launch_bounds__(256) void k_adj_prev(const Model<double> *__restrict__ mp, const double *__restrict__ phi,
+                                                  const double *__restrict__ mu, const double *__restrict__ gdphi,
+                                                  double *gphi, double *gmu, double *gtheta,
+                                                  const int32_t *__restrict__ list, const int *__restrict__ count,
+                                                  int64_t fs, int64_t ps) {
+  constexpr int NTH = kThetaN<N, NC>;
   double th[NTH];
   for (int t = 0; t < NTH; t++) th[t] = 0.0;
+  const Model<double> &m = *mp;
   const int n = *count;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int64_t c = list[i];
     double go[N];
     for (int a = 0; a < N; a++) go[a] = gdphi[a * fs + c];
-    {
-      const Model<Dual<K>> &m = *mp;
-      SeedView<N, NC, K, P2, 0> v{phi, mu, nullptr, fs, ps, c, 0, 0, chunk0};
-      Dual<K> d[N];
-      phi_delta<N, NC>(m, v, false, d);
-      for (int k = 0; k < K; k++) {
-        double g = 0.0;
-        for (int a = 0; a < N; a++) g += go[a] * d[a].d[k];
-        if (g != 0.0) scatter_dir<N, NC, 0>(chunk0 + k, g, c, fs, ps, gphi, gmu, nullptr, th, NTH);
-      }
-    }
+    RegStore<N, NC, double> rs;
+    gather_stencil<N, NC>(phi, mu, nullptr, c, fs, ps, rs);
+    RevAcc<N, NC> acc;
+    acc.clear();
+    phi_delta_vjp<N, NC>(m, RegView<N, NC, double, P2>{&rs, 0, 0}, go, acc);
+    scatter_acc<N, NC>(acc, c, fs, ps, gphi, gmu, nullptr);
+    for (int t = 0; t < NTH; t++) th[t] += acc.th[t];
   }
   theta_flush<NTH>(th, NTH, gtheta);
 }
 
 /* Stage B transposed for one field: every halo cell's adjoint goes to the interior cell it copies (x maps of
  * the field for halo rows, y maps for halo columns, both for corners), and the halo adjoint is cleared. */
-__global__ __launch_bounds__(256) void k_adj_bc(double *g, int ncomp, Maps maps, int f, int64_t fs, int64_t ps,
+static __global__ __launch_bounds__(256) void k_adj_bc(double *g, int ncomp, Maps maps, int f, int64_t fs, int64_t ps,
                                                 int64_t rx, int64_t ry) {
   const int64_t y = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
   const int64_t x = (int64_t)blockIdx.y * 4 + (threadIdx.x >> 6);

@@ -1,0 +1,524 @@
+/*
+ * gp_step_impl.h -- the kernels of the step, the invariants and the transposed step, and their launch sequences,
+ * as templates over (N, NC). Included by gp_ops.hip, which instantiates them for one pair; not a public header.
+ */
+#pragma once
+
+#include "gp_internal.h"
+#include "gp_cells.h"
+#include "gp_march.h"
+#include "gp_adjoint.h"
+
+namespace gpk {
+
+/* ------------------------------------------------------------------------------------------------ kernels */
+
+template <int N, int NC, typename R, bool SMOOTH, bool P2>
+__global__ __launch_bounds__(kBX *kBY) void k_phi(const Model<R> *__restrict__ mp, const R *__restrict__ phi,
+                                                  const R *__restrict__ mu, R *__restrict__ dphi, int64_t fs,
+                                                  int64_t ps, int64_t mx, int64_t my) {
+  const int64_t y = 2 + (int64_t)blockIdx.x * kBX + threadIdx.x;
+  const int64_t x = 2 + (int64_t)blockIdx.y * kBY + threadIdx.y;
+  if (x > mx + 3 || y > my + 3) return;
+  const Model<R> &m = *mp;
+  const int64_t c = x * ps + y + kYOff;
+  View<R, P2> v{phi, mu, nullptr, fs, ps, c};
+  R d[N];
+  phi_delta<N, NC>(m, v, SMOOTH, d);
+  for (int a = 0; a < N; a++) dphi[a * fs + c] = d[a];
+}
+
+template <int N, int NC, typename R, bool SMOOTH, bool P2>
+__global__ __launch_bounds__(kBX *kBY) void k_mu(const Model<R> *__restrict__ mp, const R *__restrict__ phi,
+                                                 const R *__restrict__ mu, const R *__restrict__ dphi,
+                                                 R *__restrict__ phi_o, R *__restrict__ mu_o, Maps maps,
+                                                 double *__restrict__ partials, int64_t fs, int64_t ps, int64_t mx,
+                                                 int64_t my) {
+  const int64_t y = 3 + (int64_t)blockIdx.x * kBX + threadIdx.x;
+  const int64_t x = 3 + (int64_t)blockIdx.y * kBY + threadIdx.y;
+  const bool live = (x <= mx + 2 && y <= my + 2);
+  constexpr int NS = 3 * N + 3 * NC + 1;
+  double st[NS];
+  for (int a = 0; a < N; a++) {
+    st[a] = -INFINITY;
+    st[N + a] = INFINITY;
+    st[2 * N + 2 * NC + a] = 0.0;
+  }
+  for (int k = 0; k < NC; k++) {
+    st[2 * N + k] = -INFINITY;
+    st[2 * N + NC + k] = INFINITY;
+    st[3 * N + 2 * NC + k] = 0.0;
+  }
+  st[NS - 1] = 0.0;
+  if (live) {
+    const Model<R> &m = *mp;
+    const int64_t c = x * ps + y + kYOff;
+    View<R, P2> v{phi, mu, dphi, fs, ps, c};
+    R jx0[NC], jxb[NC], jy0[NC], jyl[NC], d[N], pn[N], mn[NC], dm[NC];
+    if (!SMOOTH) {
+      jat_x<N, NC>(m, v, 0, 0, jx0);
+      jat_x<N, NC>(m, v, -1, 0, jxb);
+      jat_y<N, NC>(m, v, 0, 0, y, jy0);
+      jat_y<N, NC>(m, v, 0, -1, y - 1, jyl);
+    } else {
+      for (int k = 0; k < NC; k++) jx0[k] = jxb[k] = jy0[k] = jyl[k] = R(0);
+    }
+    for (int a = 0; a < N; a++) d[a] = dphi[a * fs + c];
+    mu_update<N, NC>(m, v, SMOOTH, d, jx0, jxb, jy0, jyl, pn, mn, dm);
+    for (int a = 0; a < N; a++) phi_o[a * fs + c] = pn[a];
+    for (int k = 0; k < NC; k++) mu_o[k * fs + c] = mn[k];
+    if (x < 7 || x > mx - 2 || y < 7 || y > my - 2) {
+      write_images(phi_o, N, pn, 0, maps, x, y, fs, ps);
+      write_images(mu_o, NC, mn, 1, maps, x, y, fs, ps);
+    }
+    for (int a = 0; a < N; a++) {
+      double pv = (double)pn[a], dv = (double)d[a];
+      st[a] = pv;
+      st[N + a] = pv;
+      st[2 * N + 2 * NC + a] = dv * dv;
+      if (!isfinite(pv)) st[NS - 1] += 1.0;
+    }
+    for (int k = 0; k < NC; k++) {
+      double mv = (double)mn[k], dv = (double)dm[k];
+      st[2 * N + k] = mv;
+      st[2 * N + NC + k] = mv;
+      st[3 * N + 2 * NC + k] = dv * dv;
+      if (!isfinite(mv)) st[NS - 1] += 1.0;
+    }
+  }
+  /* block reduction: wave shuffles, then the kBY waves through LDS */
+  __shared__ double red[kBY][NS];
+  for (int i = 0; i < NS; i++) {
+    double r;
+    if (i < N || (i >= 2 * N && i < 2 * N + NC)) r = wave_max(st[i]);
+    else if (i < 2 * N + 2 * NC) r = wave_min(st[i]);
+    else r = wave_sum(st[i]);
+    if (threadIdx.x == 0) red[threadIdx.y][i] = r;
+  }
+  __syncthreads();
+  if (threadIdx.y == 0 && threadIdx.x < NS) {
+    const int i = threadIdx.x;
+    double r = red[0][i];
+    for (int w = 1; w < kBY; w++) {
+      if (i < N || (i >= 2 * N && i < 2 * N + NC)) r = fmax(r, red[w][i]);
+      else if (i < 2 * N + 2 * NC) r = fmin(r, red[w][i]);
+      else r += red[w][i];
+    }
+    const int64_t bid = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+    const int64_t nparts = (int64_t)gridDim.x * gridDim.y;
+    partials[i * nparts + bid] = r;  /* statistic-major: the fold reads each statistic contiguously */
+  }
+}
+
+/* Fold the per-block partials into the cumulative statistics: one block per statistic, each reading its
+ * contiguous row of partials (statistic-major layout) in a fixed order, so the result is deterministic. */
+constexpr int kStatThreads = 1024;
+
+template <int N, int NC>
+__global__ __launch_bounds__(kStatThreads) void k_stats(const double *__restrict__ partials, int64_t stride,
+                                                        int64_t nparts, double *__restrict__ cum) {
+  constexpr int NS = 3 * N + 3 * NC + 1;
+  __shared__ double red[kStatThreads];
+  const int i = blockIdx.x;
+  if (i >= NS) return;
+  const bool mx = (i < N || (i >= 2 * N && i < 2 * N + NC));
+  const bool mn = !mx && i < 2 * N + 2 * NC;
+  double r = mx ? -INFINITY : mn ? INFINITY : 0.0;
+  const double *row = partials + (int64_t)i * stride;
+  /* a thread folds its partials in index order; the loads of eight of them are issued before the first is used */
+  int64_t b = threadIdx.x;
+  for (; b + 7 * kStatThreads < nparts; b += 8 * kStatThreads) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = row[b + u * kStatThreads];
+#pragma unroll
+    for (int u = 0; u < 8; u++) r = mx ? fmax(r, v[u]) : mn ? fmin(r, v[u]) : r + v[u];
+  }
+  for (; b < nparts; b += kStatThreads) {
+    const double v = row[b];
+    r = mx ? fmax(r, v) : mn ? fmin(r, v) : r + v;
+  }
+  red[threadIdx.x] = r;
+  __syncthreads();
+  for (int s = kStatThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      double o = red[threadIdx.x + s];
+      red[threadIdx.x] = mx ? fmax(red[threadIdx.x], o) : mn ? fmin(red[threadIdx.x], o) : red[threadIdx.x] + o;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double v = red[0];
+    if (mx) cum[i] = fmax(cum[i], v);
+    else if (mn) cum[i] = fmin(cum[i], v);
+    else if (i < NS - 1) cum[i] = v;
+    else cum[i] += v;
+  }
+}
+
+/* Global solute and grand-potential energy of the current state (no reference counterpart; the FP64 oracle's
+ * oracle_invariants defines them): per interior cell
+ *   c_k    = sum_a h_a(phi) c_a^k(mu) dx dy,
+ *   energy = [ sum_a h_a psi_a(mu) + eps sum_{a<b} Gamma_ab ac(q_ab)^2 |q_ab|^2 + w(phi) / eps ] dx dy,
+ * with q_ab from the centred gradients. Cells are evaluated in the state's precision and accumulated in FP64:
+ * a block sum per quantity into partials[i * nblocks + block], folded by k_fold_sum. */
+template <int N, int NC, typename R>
+__global__ __launch_bounds__(kBX *kBY) void k_invariants(const Model<R> *__restrict__ mp, const R *__restrict__ phi,
+                                                         const R *__restrict__ mu, double *__restrict__ partials,
+                                                         int64_t fs, int64_t ps, int64_t yoff, int64_t mx,
+                                                         int64_t my) {
+  const int64_t y = 3 + (int64_t)blockIdx.x * kBX + threadIdx.x;
+  const int64_t x = 3 + (int64_t)blockIdx.y * kBY + threadIdx.y;
+  double v[NC + 1];
+  for (int i = 0; i <= NC; i++) v[i] = 0.0;
+  if (x <= mx + 2 && y <= my + 2) {
+    const Model<R> &m = *mp;
+    const int64_t c = x * ps + y + yoff;
+    const double area = (double)m.dx * (double)m.dy;
+    R ph[N], mv[NC], gxc[N], gyc[N], h[N];
+    for (int a = 0; a < N; a++) {
+      ph[a] = phi[a * fs + c];
+      gxc[a] = (phi[a * fs + c + ps] - phi[a * fs + c - ps]) / (R(2.0) * m.dx);
+      gyc[a] = (phi[a * fs + c + 1] - phi[a * fs + c - 1]) / m.two_dy;
+    }
+    for (int k = 0; k < NC; k++) mv[k] = mu[k * fs + c];
+    for (int a = 0; a < N; a++) h[a] = hphi<N>(ph, a);
+    R e = R(0.0);
+    for (int a = 0; a < N; a++) {
+      R cc[NC], psi = R(0.0);
+      for (int k = 0; k < NC; k++) {
+        cc[k] = c_mu<NC>(m, mv, a, k);
+        v[k] += (double)(h[a] * cc[k]) * area;
+        psi -= mv[k] * cc[k];
+      }
+      psi += free_energy<NC>(m, cc, a);
+      e += h[a] * psi;
+    }
+    R grad = R(0.0), w = R(0.0);
+    for (int a = 0; a < N; a++)
+      for (int b = a + 1; b < N; b++) {
+        R r[2] = {ph[a] * gxc[b] - ph[b] * gxc[a], ph[a] * gyc[b] - ph[b] * gyc[a]}, q[2];
+        rot2(m.rot[a][b], r, q);
+        const R ac = m.fn_aniso ? aniso_ac(m.dab[a][b], q) : R(1.0);
+        grad += m.gamma[a][b] * ac * ac * qnorm2(q);
+        if (m.fn_w == 1) w += m.w_obst * m.gamma[a][b] * ph[a] * ph[b];
+        else w += R(9.0) * m.gamma[a][b] * ph[a] * ph[a] * ph[b] * ph[b];
+        for (int cc = b + 1; cc < N; cc++) w += m.gamma_abc[a][b][cc] * ph[a] * ph[b] * ph[cc];
+      }
+    e += m.eps * grad + w / m.eps;
+    v[NC] = (double)e * area;
+  }
+  __shared__ double red[NC + 1][kBY];
+  const int lane = threadIdx.x & 63;
+  for (int i = 0; i <= NC; i++) {
+    const double r = wave_sum(v[i]);
+    if (lane == 0) red[i][threadIdx.y] = r;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    const int64_t nb = (int64_t)gridDim.x * gridDim.y, bid = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+    for (int i = 0; i <= NC; i++) {
+      double t = 0.0;
+      for (int j = 0; j < kBY; j++) t += red[i][j];
+      partials[i * nb + bid] = t;
+    }
+  }
+}
+
+/* out[i] = sum_b partials[i * count + b]: one block per quantity, fixed order (deterministic) */
+static __global__ __launch_bounds__(kStatThreads) void k_fold_sum(const double *__restrict__ partials, int64_t count,
+                                                          double *__restrict__ out) {
+  __shared__ double red[kStatThreads];
+  const double *row = partials + (int64_t)blockIdx.x * count;
+  double r = 0.0;
+  for (int64_t b = threadIdx.x; b < count; b += kStatThreads) r += row[b];
+  red[threadIdx.x] = r;
+  __syncthreads();
+  for (int st = kStatThreads / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+/* solute[NC] and energy of the current state into h->inv_dev */
+/* solute[NC] and energy of a state into out (device, NC + 1 doubles) on `stream`, with the block sums in partials
+ * (at least (NC + 1) * invariant_blocks(mesh) doubles): the handle's own state (its plane layout) or a caller's in the
+ * reference's plain halo-inclusive layout (fs = rows_x * rows_y, ps = rows_y, yoff = 0) */
+template <int N, int NC, typename R>
+static int launch_invariants_on(gp_handle *h, const void *phi, const void *mu, int64_t fs, int64_t ps, int64_t yoff,
+                                double *partials, double *out, hipStream_t stream) {
+  const int64_t mx = h->p.mesh_x, my = h->p.mesh_y;
+  dim3 blk(kBX, kBY);
+  dim3 g2((unsigned)((my + kBX - 1) / kBX), (unsigned)((mx + kBY - 1) / kBY));
+  hipLaunchKernelGGL((k_invariants<N, NC, R>), g2, blk, 0, stream, (const Model<R> *)h->model, (const R *)phi,
+                     (const R *)mu, partials, fs, ps, yoff, mx, my);
+  hipLaunchKernelGGL(k_fold_sum, dim3(NC + 1), dim3(kStatThreads), 0, stream, (const double *)partials,
+                     (int64_t)g2.x * g2.y, out);
+  HIPCHK(h, hipGetLastError());
+  return GP_OK;
+}
+
+/* solute[NC] and energy of the handle's current state into h->inv_dev */
+template <int N, int NC, typename R>
+static int launch_invariants(gp_handle *h) {
+  return launch_invariants_on<N, NC, R>(h, h->phi[h->cur], h->mu[h->cur], h->fs, h->ps, kYOff, h->partials,
+                                        h->inv_dev, h->stream);
+}
+
+/* ------------------------------------------------------------------------------------------------ dispatch */
+
+template <int N, int NC, typename R, bool P2>
+static int launch_step_p(gp_handle *h, int mode) {
+  const R *phi = (const R *)h->phi[h->cur], *mu = (const R *)h->mu[h->cur];
+  R *phi_o = (R *)h->phi[1 - h->cur], *mu_o = (R *)h->mu[1 - h->cur];
+  R *dphi = (R *)h->dphi;
+  const Model<R> *m = (const Model<R> *)h->model;
+  const int64_t mx = h->p.mesh_x, my = h->p.mesh_y;
+  dim3 blk(kBX, kBY);
+  dim3 g1((unsigned)((my + 2 + kBX - 1) / kBX), (unsigned)((mx + 2 + kBY - 1) / kBY));
+  dim3 g2((unsigned)((my + kBX - 1) / kBX), (unsigned)((mx + kBY - 1) / kBY));
+  const bool smooth = mode == MODE_SMOOTH;
+  if (mode == MODE_DPHI_CUR) {
+    /* delta-phi of the current state (the adjoint's recompute) */
+    hipLaunchKernelGGL((k_phi<N, NC, R, false, P2>), g1, blk, 0, h->stream, m, phi, mu, dphi, h->fs, h->ps, mx, my);
+    HIPCHK(h, hipGetLastError());
+    return GP_OK;
+  }
+  if (mode == MODE_DPHI_PREV) {
+    /* delta-phi of the last step, recomputed from the previous state (still in the other buffer) */
+    const R *pp = (const R *)h->phi[1 - h->cur], *mp = (const R *)h->mu[1 - h->cur];
+    hipLaunchKernelGGL((k_phi<N, NC, R, false, P2>), g1, blk, 0, h->stream, m, pp, mp, dphi, h->fs, h->ps, mx, my);
+    HIPCHK(h, hipGetLastError());
+    return GP_OK;
+  }
+  if (mode == MODE_MARCH || mode == MODE_MARCH_STATS) {
+    /* delta-phi of the candidate cells, then the update: by cell class (k_cbulk + k_ilist) or by march tiles
+     * (k_march near an interface, k_bulk on deep tiles). The update kernels read the current state and its phase
+     * codes and write disjoint cells of the other buffers (state and codes). */
+    const int nt = (int)h->march_nwaves;
+    const unsigned nblk = (unsigned)((nt + 3) / 4);
+    const int *fin = h->flags_valid ? h->flags[h->fcur] : nullptr;
+    int *fout = h->flags[1 - h->fcur];
+    const int64_t pstride = 2 * (int64_t)nt;
+    const int8_t *codes = h->codes[h->ccur];
+    int8_t *codes_o = h->codes[1 - h->ccur];
+    const bool stats = mode == MODE_MARCH_STATS;
+    if (!h->codes_valid) {
+      hipLaunchKernelGGL((k_codes<N, R>), dim3((unsigned)((h->ry + 63) / 64), (unsigned)((h->rx + 3) / 4)), dim3(256),
+                         0, h->stream, phi, h->codes[h->ccur], h->fs, h->ps, h->rx, h->ry);
+    }
+    if (!h->use_cells) {
+      /* delta-phi of the candidate cells, tiles of 64 y-cells by kDTileRows rows over x, y in [2, mesh+3] */
+      const int nty = (int)((my + 2 + 63) / 64), ntx = (int)((mx + 2 + kDTileRows - 1) / kDTileRows);
+      ProfScope ps(h, GP_KERNEL_DPHI);
+      hipLaunchKernelGGL((k_dphi<N, NC, R, P2>), dim3((unsigned)((nty * ntx + 3) / 4)), dim3(256), 0, h->stream, m,
+                         phi, mu, codes, dphi, h->fs, h->ps, mx, my, nty, nty * ntx);
+    }
+    if (h->use_cells) {
+      /* statistics partials: k_cbulk's wavefronts, then k_ilist's */
+      const int ct = (int)h->cell_nwaves;
+      const unsigned cblk = (unsigned)((ct + 3) / 4);
+      const int64_t cstride = (int64_t)ct + 4 * kListBlocks;
+      /* this step's list lengths are zero (the previous step's k_cbulk, or the set-up, cleared them); this
+       * step's k_cbulk clears the other pair for the next step, which the previous step's kernels read */
+      int *lc = h->lcount + 2 * h->lsel, *lnext = h->lcount + 2 * (1 - h->lsel);
+      h->lsel = 1 - h->lsel;
+      /* k_cbulk's blocks past the tiles list the candidates of the boundary ring, one thread per ring cell */
+      const int64_t nring = 2 * (my + 2) + 2 * mx;
+      const unsigned rblk = (unsigned)((nring + 255) / 256);
+      {
+        ProfScope ps(h, GP_KERNEL_CBULK);
+        if (stats)
+          hipLaunchKernelGGL((k_cbulk<N, NC, R, P2, true>), dim3(cblk + rblk), dim3(256), 0, h->stream, m, mu, codes,
+                             phi_o, mu_o, codes_o, h->maps, h->partials, cstride, h->fs, h->ps, mx, my, h->cell_seg,
+                             h->march_nstrips, ct, (int)h->prev_valid, h->ilist, h->clist, lc, lnext, (int)cblk);
+        else
+          hipLaunchKernelGGL((k_cbulk<N, NC, R, P2, false>), dim3(cblk + rblk), dim3(256), 0, h->stream, m, mu, codes,
+                             phi_o, mu_o, codes_o, h->maps, h->partials, cstride, h->fs, h->ps, mx, my, h->cell_seg,
+                             h->march_nstrips, ct, (int)h->prev_valid, h->ilist, h->clist, lc, lnext, (int)cblk);
+      }
+      {
+        /* delta-phi of the listed candidates, full wavefronts */
+        ProfScope ps(h, GP_KERNEL_DPHI);
+        hipLaunchKernelGGL((k_dlist<N, NC, R, P2>), dim3(kListBlocks), dim3(256), 0, h->stream, m, phi, mu, dphi,
+                           h->fs, h->ps, (const int32_t *)h->clist, (const int *)lc + 1);
+      }
+      {
+        ProfScope ps(h, GP_KERNEL_CIFACE);
+        if (stats)
+          hipLaunchKernelGGL((k_ilist<N, NC, R, P2, true>), dim3(kListBlocks), dim3(256), 0, h->stream, m, phi, mu,
+                             dphi, codes, phi_o, mu_o, codes_o, h->maps, h->partials + ct, cstride, h->fs, h->ps, mx,
+                             my, (const int32_t *)h->ilist, (const int *)lc);
+        else
+          hipLaunchKernelGGL((k_ilist<N, NC, R, P2, false>), dim3(kListBlocks), dim3(256), 0, h->stream, m, phi, mu,
+                             dphi, codes, phi_o, mu_o, codes_o, h->maps, h->partials + ct, cstride, h->fs, h->ps, mx,
+                             my, (const int32_t *)h->ilist, (const int *)lc);
+      }
+      if (stats) {
+        ProfScope ps(h, GP_KERNEL_STATS);
+        hipLaunchKernelGGL((k_stats<N, NC>), dim3(3 * N + 3 * NC + 1), dim3(kStatThreads), 0, h->stream,
+                           h->partials, cstride, cstride, h->stats_dev);
+      }
+      HIPCHK(h, hipGetLastError());
+      h->cur = 1 - h->cur;
+      h->ccur = 1 - h->ccur;
+      h->flags_valid = false; /* this path forms no tile flags */
+      h->codes_valid = true;
+      return GP_OK;
+    }
+    if (mode == MODE_MARCH) {
+      {
+        ProfScope ps(h, GP_KERNEL_FUSED);
+        hipLaunchKernelGGL((k_march<N, NC, R, P2, false>), dim3(nblk), dim3(256), 0, h->stream, m, phi, mu, dphi,
+                           phi_o, mu_o, codes_o, h->maps, h->partials, pstride, fin, fout, h->fs, h->ps, mx, my,
+                           h->march_seg, h->march_nstrips, h->march_nsegs, nt);
+      }
+      if (fin) {
+        ProfScope ps(h, GP_KERNEL_BULK);
+        hipLaunchKernelGGL((k_bulk<N, NC, R, P2, false>), dim3(nblk), dim3(256), 0, h->stream, m, mu, phi_o, mu_o,
+                           codes_o, h->partials + nt, pstride, fin, fout, h->fs, h->ps, mx, h->march_seg,
+                           h->march_nstrips, h->march_nsegs, nt);
+      }
+    } else {
+      {
+        ProfScope ps(h, GP_KERNEL_FUSED_STATS);
+        hipLaunchKernelGGL((k_march<N, NC, R, P2, true>), dim3(nblk), dim3(256), 0, h->stream, m, phi, mu, dphi,
+                           phi_o, mu_o, codes_o, h->maps, h->partials, pstride, fin, fout, h->fs, h->ps, mx, my,
+                           h->march_seg, h->march_nstrips, h->march_nsegs, nt);
+      }
+      if (fin) {
+        ProfScope ps(h, GP_KERNEL_BULK);
+        hipLaunchKernelGGL((k_bulk<N, NC, R, P2, true>), dim3(nblk), dim3(256), 0, h->stream, m, mu, phi_o, mu_o,
+                           codes_o, h->partials + nt, pstride, fin, fout, h->fs, h->ps, mx, h->march_seg,
+                           h->march_nstrips, h->march_nsegs, nt);
+      }
+      ProfScope ps(h, GP_KERNEL_STATS);
+      hipLaunchKernelGGL((k_stats<N, NC>), dim3(3 * N + 3 * NC + 1), dim3(kStatThreads), 0, h->stream,
+                         h->partials, pstride, fin ? pstride : (int64_t)nt, h->stats_dev);
+    }
+    HIPCHK(h, hipGetLastError());
+    h->cur = 1 - h->cur;
+    h->ccur = 1 - h->ccur;
+    h->fcur = 1 - h->fcur;
+    h->flags_valid = true;
+    h->codes_valid = true;
+    return GP_OK;
+  }
+  h->flags_valid = false; /* the two-pass kernels form neither tile flags nor phase codes */
+  h->codes_valid = false;
+  if (smooth) {
+    {
+      ProfScope ps(h, GP_KERNEL_PHI);
+      hipLaunchKernelGGL((k_phi<N, NC, R, true, P2>), g1, blk, 0, h->stream, m, phi, mu, dphi, h->fs, h->ps, mx, my);
+    }
+    {
+      ProfScope ps(h, GP_KERNEL_MU);
+      hipLaunchKernelGGL((k_mu<N, NC, R, true, P2>), g2, blk, 0, h->stream, m, phi, mu, dphi, phi_o, mu_o, h->maps,
+                         h->partials, h->fs, h->ps, mx, my);
+    }
+  } else {
+    {
+      ProfScope ps(h, GP_KERNEL_PHI);
+      hipLaunchKernelGGL((k_phi<N, NC, R, false, P2>), g1, blk, 0, h->stream, m, phi, mu, dphi, h->fs, h->ps, mx, my);
+    }
+    {
+      ProfScope ps(h, GP_KERNEL_MU);
+      hipLaunchKernelGGL((k_mu<N, NC, R, false, P2>), g2, blk, 0, h->stream, m, phi, mu, dphi, phi_o, mu_o, h->maps,
+                         h->partials, h->fs, h->ps, mx, my);
+    }
+    {
+      ProfScope ps(h, GP_KERNEL_STATS);
+      hipLaunchKernelGGL((k_stats<N, NC>), dim3(3 * N + 3 * NC + 1), dim3(kStatThreads), 0, h->stream,
+                         h->partials, h->nblocks_mu, h->nblocks_mu, h->stats_dev);
+    }
+  }
+  HIPCHK(h, hipGetLastError());
+  h->cur = 1 - h->cur;
+  return GP_OK;
+}
+
+template <int N, int NC, typename R>
+static int launch_step(gp_handle *h, int mode) {
+  return h->pow2 ? launch_step_p<N, NC, R, true>(h, mode) : launch_step_p<N, NC, R, false>(h, mode);
+}
+
+/* ------------------------------------------------------------------------------------------------ adjoint */
+
+/* one adjoint step: the current state is s_t (delta-phi of s_t in h->dphi); A->g[1] holds the adjoint of
+ * s_{t+1} and becomes, in A->g[0] (then swapped into g[1]), the adjoint of s_t */
+template <int N, int NC, bool P2>
+static int adj_step_t(gp_handle *h, const double *phi, const double *mu, const double *dphi, const void *model) {
+  Adjoint *A = h->adj;
+  const int64_t mx = h->p.mesh_x, my = h->p.mesh_y, fs = h->fs;
+  double *gn = A->g[1], *gc = A->g[0];
+  const dim3 blk(256);
+  const dim3 gall((unsigned)((h->ry + 63) / 64), (unsigned)((h->rx + 3) / 4));
+  hipLaunchKernelGGL(k_adj_bc, gall, blk, 0, h->stream, gn, N, h->maps, 0, fs, h->ps, h->rx, h->ry);
+  hipLaunchKernelGGL(k_adj_bc, gall, blk, 0, h->stream, gn + (size_t)N * fs, NC, h->maps, 1, fs, h->ps, h->rx, h->ry);
+  HIPCHK(h, hipMemsetAsync(gc, 0, state_elems(h) * sizeof(double), h->stream));
+  HIPCHK(h, hipMemsetAsync(A->gdphi, 0, (size_t)N * fs * sizeof(double), h->stream));
+  /* bulk cells of stage Q in closed form (plain stores), then the reverse-mode kernels add the rest */
+  HIPCHK(h, hipMemsetAsync(A->counts, 0, 2 * sizeof(int), h->stream));
+  const Model<double> *md = (const Model<double> *)model;
+  hipLaunchKernelGGL((k_adj_cbar<N, NC>), gall, blk, 0, h->stream, md, phi, dphi, gn, gn + (size_t)N * fs, A->cbar,
+                     A->qlist, A->counts, fs, h->ps, mx, my);
+  const dim3 gb((unsigned)((my + 2 + 63) / 64), (unsigned)((mx + 2 + 3) / 4));
+  hipLaunchKernelGGL((k_adj_bulk<N, NC>), gb, blk, 0, h->stream, md, phi, mu, dphi, (const double *)A->cbar, gn, gc,
+                     gc + (size_t)N * fs, A->gdphi, A->gth, 3 * (N * (N - 1) / 2), fs, h->ps, mx, my);
+  /* the other cells by reverse mode, one pass per listed cell (grid-stride loops) */
+  const dim3 gl(kAdjBlocks);
+  hipLaunchKernelGGL((k_adj_qrev<N, NC, P2>), gl, blk, 0, h->stream, md, phi, mu, dphi, gn, gn + (size_t)N * fs, gc,
+                     gc + (size_t)N * fs, A->gdphi, A->gth, (const int32_t *)A->qlist, (const int *)A->counts, fs,
+                     h->ps);
+  hipLaunchKernelGGL((k_adj_plist<N>), gb, blk, 0, h->stream, phi, (const double *)A->gdphi, A->plist, A->counts + 1, fs,
+                     h->ps, mx, my);
+  hipLaunchKernelGGL((k_adj_prev<N, NC, P2>), gl, blk, 0, h->stream, md, phi, mu, (const double *)A->gdphi, gc,
+                     gc + (size_t)N * fs, A->gth, (const int32_t *)A->plist, (const int *)A->counts + 1, fs, h->ps);
+  HIPCHK(h, hipGetLastError());
+  std::swap(A->g[0], A->g[1]);
+  return GP_OK;
+}
+
+/* FP32 handles: the current state widened into A->st64 and its delta-phi (FP64 model) into A->dphi64 */
+static __global__ __launch_bounds__(256) void k_widen(const float *__restrict__ src, double *__restrict__ dst, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = (double)src[i];
+}
+template <int N, int NC, bool P2>
+static int adj_widen_t(gp_handle *h) {
+  Adjoint *A = h->adj;
+  const int64_t np = (int64_t)N * h->fs, nm = (int64_t)NC * h->fs;
+  hipLaunchKernelGGL(k_widen, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, h->stream,
+                     (const float *)h->phi[h->cur], A->st64, np);
+  hipLaunchKernelGGL(k_widen, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, h->stream,
+                     (const float *)h->mu[h->cur], A->st64 + np, nm);
+  const int64_t mx = h->p.mesh_x, my = h->p.mesh_y;
+  dim3 g1((unsigned)((my + 2 + kBX - 1) / kBX), (unsigned)((mx + 2 + kBY - 1) / kBY));
+  hipLaunchKernelGGL((k_phi<N, NC, double, false, P2>), g1, dim3(kBX, kBY), 0, h->stream,
+                     (const Model<double> *)A->model64, (const double *)A->st64, (const double *)(A->st64 + np),
+                     A->dphi64, h->fs, h->ps, mx, my);
+  HIPCHK(h, hipGetLastError());
+  return GP_OK;
+}
+
+template <int N, int NC>
+static int adj_step_any(gp_handle *h, const double *phi, const double *mu, const double *dphi, const void *model) {
+  return h->pow2 ? adj_step_t<N, NC, true>(h, phi, mu, dphi, model) : adj_step_t<N, NC, false>(h, phi, mu, dphi, model);
+}
+template <int N, int NC>
+static int adj_widen_any(gp_handle *h) {
+  return h->pow2 ? adj_widen_t<N, NC, true>(h) : adj_widen_t<N, NC, false>(h);
+}
+
+/* the kernel set of (N, NC) in the given precision */
+template <int N, int NC>
+const GpOps *make_ops(int precision) {
+  static const GpOps f64 = {launch_step<N, NC, double>, launch_invariants<N, NC, double>, adj_step_any<N, NC>,
+                            adj_widen_any<N, NC>, launch_invariants_on<N, NC, double>};
+  static const GpOps f32 = {launch_step<N, NC, float>, launch_invariants<N, NC, float>, adj_step_any<N, NC>,
+                            adj_widen_any<N, NC>, launch_invariants_on<N, NC, float>};
+  return precision == GP_FP64 ? &f64 : &f32;
+}
+
+}  // namespace gpk

@@ -8,11 +8,9 @@
 #pragma once
 
 #include "gp_dual.h"
-#include "gp_model.h"
+#include "gp_reverse.h"
 
 namespace gpk {
-
-inline int theta_count(int N, int NC) { return 3 * (N * (N - 1) / 2) + NC + 1; }
 
 /* Model<Dual<K>> whose theta entries carry the tangents w(t, k) (derivative of theta t along direction k), with
  * the coefficients derived from the diffusivities (D cmu, 1 - D_a/D_L, the pure-phase mobilities) re-formed in

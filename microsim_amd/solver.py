@@ -84,8 +84,11 @@ class GPSolver:
                                          _abi.dptr(dphi) if with_dphi else None))
         return (phi, mu, dphi) if with_dphi else (phi, mu)
 
-    def profile_enable(self, on: bool = True) -> None:
-        self._check(self.lib.gp_profile_enable(self.h, int(on)))
+    def profile_enable(self, on=True) -> None:
+        """True / _abi.GP_PROFILE_LAUNCHES: events around every kernel launch; _abi.GP_PROFILE_CALLS: around every
+        whole step / smooth call only (kind "call"); False: off."""
+        mode = on if not isinstance(on, bool) else (_abi.GP_PROFILE_LAUNCHES if on else _abi.GP_PROFILE_OFF)
+        self._check(self.lib.gp_profile_enable(self.h, int(mode)))
 
     def profile_read(self) -> dict:
         """{kernel name: (mean ms per launch, launches)} since the previous read (HIP events on the handle's

@@ -8,9 +8,14 @@
 // (microsim_amd/csrc/gp_theta.h), exact to rounding and taking the branches the values take. The tests set it
 // against finite differences (where the trajectory is smooth) and against the device adjoint (a dot-product
 // test, which holds whatever the branches).
+//
+// model_vjp runs the adjoint of the same steps with the reverse-mode per-cell functions of
+// microsim_amd/csrc/gp_reverse.h (the ones the device adjoint runs), on every cell and without the device's bulk
+// shortcuts: <J^T w, v> from it must equal <w, J v> from model_jvp.
 #include <cstring>
 #include <vector>
 
+#include "gp_reverse.h"
 #include "gp_theta.h"
 
 using namespace gpk;
@@ -104,6 +109,125 @@ static int run_jvp(const gp_params *p, double *phi, double *mu, const double *vp
   return 0;
 }
 
+/* stencil of the cell at index c (halo-inclusive grid, y fastest): phi 3x3, mu and delta-phi on the 5-point one */
+template <int N, int NC>
+static void gather(const double *phi, const double *mu, const double *dphi, int64_t n, int64_t ry, int64_t c,
+                   RegStore<N, NC, double> &rs) {
+  const int oi[5] = {0, 1, -1, 0, 0}, oj[5] = {0, 0, 0, 1, -1};
+  for (int a = 0; a < N; a++)
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) rs.p[a][i][j] = phi[a * n + c + (i - 1) * ry + (j - 1)];
+  for (int o = 0; o < 5; o++) {
+    const int64_t co = c + oi[o] * ry + oj[o];
+    for (int k = 0; k < NC; k++) rs.u[k][o] = mu[k * n + co];
+    for (int a = 0; a < N; a++) rs.d[a][o] = dphi ? dphi[a * n + co] : 0.0;
+  }
+}
+
+template <int N, int NC>
+static void scatter(const RevAcc<N, NC> &acc, int64_t n, int64_t ry, int64_t c, double *gphi, double *gmu,
+                    double *gdphi, double *gth) {
+  const int oi[5] = {0, 1, -1, 0, 0}, oj[5] = {0, 0, 0, 1, -1};
+  for (int a = 0; a < N; a++)
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) gphi[a * n + c + (i - 1) * ry + (j - 1)] += acc.p[a][i][j];
+  for (int o = 0; o < 5; o++) {
+    const int64_t co = c + oi[o] * ry + oj[o];
+    for (int k = 0; k < NC; k++) gmu[k * n + co] += acc.u[k][o];
+    if (gdphi)
+      for (int a = 0; a < N; a++) gdphi[a * n + co] += acc.d[a][o];
+  }
+  for (int t = 0; t < kThetaN<N, NC>; t++) gth[t] += acc.th[t];
+}
+
+/* the boundary copies transposed: the y maps (applied last) first, then the x maps; halo adjoints are cleared */
+static void bc_transpose(const gp_params *p, double *g, int ncomp, int f) {
+  const int64_t rx = p->mesh_x + 6, ry = p->mesh_y + 6, n = rx * ry;
+  const int64_t hx[6] = {0, 1, 2, rx - 3, rx - 2, rx - 1}, hy[6] = {0, 1, 2, ry - 3, ry - 2, ry - 1};
+  for (int c = 0; c < ncomp; c++) {
+    double *v = g + c * n;
+    for (int s = 5; s >= 0; s--)
+      for (int64_t x = rx - 1; x >= 0; x--) {
+        v[x * ry + p->halo_src_y[f][s]] += v[x * ry + hy[s]];
+        v[x * ry + hy[s]] = 0.0;
+      }
+    for (int s = 5; s >= 0; s--)
+      for (int64_t y = ry - 1; y >= 0; y--) {
+        v[(int64_t)p->halo_src_x[f][s] * ry + y] += v[hx[s] * ry + y];
+        v[hx[s] * ry + y] = 0.0;
+      }
+  }
+}
+
+template <int N, int NC, bool P2>
+static int run_vjp(const gp_params *p, const double *phi0, const double *mu0, const double *wphi, const double *wmu,
+                   double *gphi0, double *gmu0, double *gtheta, int64_t nsteps) {
+  Model<double> m;
+  if (make_model(p, &m)) return -1;
+  if (P2 && !m.pow2) return -3;
+  const int64_t rx = p->mesh_x + 6, ry = p->mesh_y + 6, n = rx * ry;
+  /* the trajectory: states s_0 .. s_{T-1} and the delta-phi of each step */
+  std::vector<std::vector<double>> sp(nsteps), sm(nsteps), sd(nsteps);
+  std::vector<double> ph(phi0, phi0 + N * n), mu(mu0, mu0 + NC * n);
+  apply_bc(p, ph.data(), mu.data(), N, NC);
+  for (int64_t t = 0; t < nsteps; t++) {
+    sp[t] = ph;
+    sm[t] = mu;
+    sd[t].assign(N * n, 0.0);
+    advance<N, NC, P2>(m, p, ph.data(), mu.data(), sd[t].data(), 1, 0);
+  }
+  std::vector<double> gp(wphi, wphi + N * n), gm(wmu, wmu + NC * n), gd(N * n);
+  std::vector<double> gpn(N * n), gmn(NC * n);
+  double th[kThetaN<N, NC>] = {0};
+  for (int64_t t = nsteps - 1; t >= 0; t--) {
+    bc_transpose(p, gp.data(), N, 0);
+    bc_transpose(p, gm.data(), NC, 1);
+    std::fill(gpn.begin(), gpn.end(), 0.0);
+    std::fill(gmn.begin(), gmn.end(), 0.0);
+    std::fill(gd.begin(), gd.end(), 0.0);
+    const double *P = sp[t].data(), *M = sm[t].data(), *D = sd[t].data();
+    /* stage Q (interior), then stage P ([2, mesh+3]^2) */
+    for (int64_t x = 3; x <= p->mesh_x + 2; x++)
+      for (int64_t y = 3; y <= p->mesh_y + 2; y++) {
+        const int64_t c = x * ry + y;
+        RegStore<N, NC, double> rs;
+        gather<N, NC>(P, M, D, n, ry, c, rs);
+        RevAcc<N, NC> acc;
+        acc.clear();
+        double go[N], gu[NC];
+        for (int a = 0; a < N; a++) go[a] = gp[a * n + c];
+        for (int k = 0; k < NC; k++) gu[k] = gm[k * n + c];
+        cell_update_vjp<N, NC>(m, RegView<N, NC, double, P2>{&rs, 0, 0}, y, go, gu, acc);
+        scatter<N, NC>(acc, n, ry, c, gpn.data(), gmn.data(), gd.data(), th);
+      }
+    for (int64_t x = 2; x <= p->mesh_x + 3; x++)
+      for (int64_t y = 2; y <= p->mesh_y + 3; y++) {
+        const int64_t c = x * ry + y;
+        double go[N];
+        bool nz = false;
+        for (int a = 0; a < N; a++) {
+          go[a] = gd[a * n + c];
+          nz = nz || go[a] != 0.0;
+        }
+        if (!nz) continue;
+        RegStore<N, NC, double> rs;
+        gather<N, NC>(P, M, nullptr, n, ry, c, rs);
+        RevAcc<N, NC> acc;
+        acc.clear();
+        phi_delta_vjp<N, NC>(m, RegView<N, NC, double, P2>{&rs, 0, 0}, go, acc);
+        scatter<N, NC>(acc, n, ry, c, gpn.data(), gmn.data(), nullptr, th);
+      }
+    gp.swap(gpn);
+    gm.swap(gmn);
+  }
+  bc_transpose(p, gp.data(), N, 0); /* the initial state's halos are copies of its interior */
+  bc_transpose(p, gm.data(), NC, 1);
+  std::copy(gp.begin(), gp.end(), gphi0);
+  std::copy(gm.begin(), gm.end(), gmu0);
+  for (int t = 0; t < kThetaN<N, NC>; t++) gtheta[t] = th[t];
+  return 0;
+}
+
 #define GP_CASES(CASE) CASE(2, 1) CASE(3, 2) CASE(3, 1) CASE(2, 2) CASE(4, 2) CASE(4, 1) CASE(3, 3)
 
 /* pow2: 1 forces the power-of-two spacing path (error -3 when dx, dy are not powers of two), 0 the division path */
@@ -126,6 +250,97 @@ extern "C" int model_jvp(const gp_params *p, double *phi, double *mu, const doub
   if (N == a && NC == b)                                                                  \
     return pow2 ? run_jvp<a, b, true>(p, phi, mu, vphi, vmu, vtheta, tphi, tmu, nsteps) \
                 : run_jvp<a, b, false>(p, phi, mu, vphi, vmu, vtheta, tphi, tmu, nsteps);
+  GP_CASES(CASE)
+#undef CASE
+  return -2;
+}
+
+/* Gradient of <wphi, phi_T> + <wmu, mu_T> after nsteps steps from (phi, mu) with respect to the initial state
+ * (halo entries 0: they are copies of the interior) and theta, by the reverse-mode per-cell functions. */
+extern "C" int model_vjp(const gp_params *p, const double *phi, const double *mu, const double *wphi,
+                         const double *wmu, double *gphi, double *gmu, double *gtheta, int64_t nsteps, int pow2) {
+  const int N = p->nphases, NC = p->ncomp - 1;
+#define CASE(a, b)                                                                        \
+  if (N == a && NC == b)                                                                  \
+    return pow2 ? run_vjp<a, b, true>(p, phi, mu, wphi, wmu, gphi, gmu, gtheta, nsteps) \
+                : run_vjp<a, b, false>(p, phi, mu, wphi, wmu, gphi, gmu, gtheta, nsteps);
+  GP_CASES(CASE)
+#undef CASE
+  return -2;
+}
+
+/* Per-cell check of the reverse-mode functions against forward-mode duals: for the cell at index c of a state
+ * (phi, mu and delta-phi halo-inclusive), stage 0 (P: phi_delta) or 1 (Q: currents and update), the transposed
+ * local Jacobian applied to w (N outputs for P, N + NC for Q) by the reverse functions (vjp) and column by column
+ * in dual numbers (jvp). Inputs in the order phi (a, i, j) 9N, mu (k, slot) 5NC, delta-phi (a, slot) 5N, theta. */
+template <int N, int NC, bool P2>
+static int cell_check_t(const gp_params *p, const double *phi, const double *mu, const double *dphi, int64_t c,
+                        int stage, const double *w, double *vjp, double *jvp) {
+  using D = Dual<1>;
+  const int64_t ry = p->mesh_y + 6, n = (p->mesh_x + 6) * ry;
+  const int64_t y = c % ry;
+  Model<double> m;
+  if (make_model(p, &m)) return -1;
+  RegStore<N, NC, double> rs;
+  gather<N, NC>(phi, mu, dphi, n, ry, c, rs);
+  RevAcc<N, NC> acc;
+  acc.clear();
+  if (stage == 0) phi_delta_vjp<N, NC>(m, RegView<N, NC, double, P2>{&rs, 0, 0}, w, acc);
+  else cell_update_vjp<N, NC>(m, RegView<N, NC, double, P2>{&rs, 0, 0}, y, w, w + N, acc);
+  const int nth = kThetaN<N, NC>;
+  int o = 0;
+  for (int a = 0; a < N; a++)
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) vjp[o++] = acc.p[a][i][j];
+  for (int k = 0; k < NC; k++)
+    for (int s = 0; s < 5; s++) vjp[o++] = acc.u[k][s];
+  for (int a = 0; a < N; a++)
+    for (int s = 0; s < 5; s++) vjp[o++] = acc.d[a][s];
+  for (int t = 0; t < nth; t++) vjp[o++] = acc.th[t];
+  const int nin = o;
+  for (int dir = 0; dir < nin; dir++) {
+    Model<D> md;
+    make_model(p, &md);
+    const int tdir = dir - (9 * N + 5 * NC + 5 * N);
+    make_dual_model_w<1>(p, [=](int t, int) { return t == tdir ? 1.0 : 0.0; }, &md);
+    RegStore<N, NC, D> ds;
+    for (int a = 0; a < N; a++)
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) ds.p[a][i][j] = D(rs.p[a][i][j]);
+    for (int k = 0; k < NC; k++)
+      for (int s = 0; s < 5; s++) ds.u[k][s] = D(rs.u[k][s]);
+    for (int a = 0; a < N; a++)
+      for (int s = 0; s < 5; s++) ds.d[a][s] = D(rs.d[a][s]);
+    if (dir < 9 * N) ds.p[dir / 9][(dir % 9) / 3][dir % 3].d[0] = 1.0;
+    else if (dir < 9 * N + 5 * NC) ds.u[(dir - 9 * N) / 5][(dir - 9 * N) % 5].d[0] = 1.0;
+    else if (dir < 9 * N + 5 * NC + 5 * N) ds.d[(dir - 9 * N - 5 * NC) / 5][(dir - 9 * N - 5 * NC) % 5].d[0] = 1.0;
+    const RegView<N, NC, D, P2> v{&ds, 0, 0};
+    double s = 0.0;
+    if (stage == 0) {
+      D d[N];
+      phi_delta<N, NC>(md, v, false, d);
+      for (int a = 0; a < N; a++) s += w[a] * d[a].d[0];
+    } else {
+      D jx0[NC], jxb[NC], jy0[NC], jyl[NC], d[N], pn[N], mn[NC], dm[NC];
+      jat_x<N, NC>(md, v, 0, 0, jx0);
+      jat_x<N, NC>(md, v, -1, 0, jxb);
+      jat_y<N, NC>(md, v, 0, 0, y, jy0);
+      jat_y<N, NC>(md, v, 0, -1, y - 1, jyl);
+      for (int a = 0; a < N; a++) d[a] = v.dp(a, 0, 0);
+      mu_update<N, NC>(md, v, false, d, jx0, jxb, jy0, jyl, pn, mn, dm);
+      for (int a = 0; a < N; a++) s += w[a] * pn[a].d[0];
+      for (int k = 0; k < NC; k++) s += w[N + k] * mn[k].d[0];
+    }
+    jvp[dir] = s;
+  }
+  return nin;
+}
+
+extern "C" int cell_check(const gp_params *p, const double *phi, const double *mu, const double *dphi, int64_t c,
+                          int stage, const double *w, double *vjp, double *jvp) {
+  const int N = p->nphases, NC = p->ncomp - 1;
+#define CASE(a, b) \
+  if (N == a && NC == b) return cell_check_t<a, b, false>(p, phi, mu, dphi, c, stage, w, vjp, jvp);
   GP_CASES(CASE)
 #undef CASE
   return -2;

@@ -61,8 +61,25 @@ def model_lib():
                                    C.c_int]
         lib.model_jvp.argtypes = [C.POINTER(_abi.GPParams), _abi._dp, _abi._dp, _abi._dp, _abi._dp, _abi._dp,
                                   _abi._dp, _abi._dp, C.c_int64, C.c_int]
+        lib.model_vjp.argtypes = [C.POINTER(_abi.GPParams), _abi._dp, _abi._dp, _abi._dp, _abi._dp, _abi._dp,
+                                  _abi._dp, _abi._dp, C.c_int64, C.c_int]
         _model = lib
     return _model
+
+
+def model_vjp(params, phi, mu, wphi, wmu, nsteps):
+    """(gphi0, gmu0, gtheta): the gradient of <wphi, phi_T> + <wmu, mu_T> with respect to the initial state and
+    theta, by the reverse-mode per-cell functions on the host (gp_reverse.h), FP64."""
+    from microsim_amd import adjoint
+    nth = len(adjoint.theta_from_params(params))
+    g0, m0, th = np.zeros_like(phi), np.zeros_like(mu), np.zeros(nth)
+    rc = model_lib().model_vjp(C.byref(params), _abi.dptr(np.ascontiguousarray(phi)),
+                               _abi.dptr(np.ascontiguousarray(mu)),
+                               _abi.dptr(np.ascontiguousarray(wphi, dtype=np.float64)),
+                               _abi.dptr(np.ascontiguousarray(wmu, dtype=np.float64)), _abi.dptr(g0),
+                               _abi.dptr(m0), _abi.dptr(th), nsteps, 0)
+    assert rc == 0
+    return g0, m0, th
 
 
 def model_jvp(params, phi, mu, vphi, vmu, vtheta, nsteps):

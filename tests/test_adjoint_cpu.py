@@ -18,6 +18,7 @@ import numpy as np
 import pytest
 
 import torch_step
+import gp_testlib
 from gp_testlib import golden_cases, load_golden, model_jvp, oracle_step
 from microsim_amd import adjoint
 
@@ -123,3 +124,25 @@ def test_torch_autograd_matches_forward_mode(name):
     _, _, tphi, tmu = model_jvp(case.params, case.phi, case.mu, vphi, vmu, vth, 5)
     rhs = np.concatenate([(wphi * tphi).ravel(), (wmu * tmu).ravel()])
     assert abs(lhs.sum() - rhs.sum()) <= 1e-13 * np.abs(rhs).sum()
+
+
+@pytest.mark.parametrize("name", golden_cases())
+def test_reverse_mode_is_the_transposed_tangent(name):
+    """The reverse-mode per-cell functions (gp_reverse.h, which the device adjoint runs), composed over every cell of
+    five steps on the host (model_vjp), against the forward-mode tangent (model_jvp): the dot-product identity
+    <J^T w, v> = <w, J v> for a random tangent over every interior cell and every parameter, checked for the phi,
+    mu and theta blocks separately (a wrong term in any block cannot cancel against another)."""
+    case, _ = load_golden(name)
+    n = 5
+    vphi, vmu, vth = directions(case, 1, phi_cells="all")
+    rng = np.random.default_rng(7)
+    wphi, wmu = rng.standard_normal(case.phi.shape), rng.standard_normal(case.mu.shape)
+    g0, m0, gth = gp_testlib.model_vjp(case.params, case.phi, case.mu, wphi, wmu, n)
+    zero_p, zero_m, zero_t = np.zeros_like(vphi), np.zeros_like(vmu), np.zeros_like(vth)
+    for vp, vm, vt, got in ((vphi, zero_m, zero_t, (g0 * vphi).sum()), (zero_p, vmu, zero_t, (m0 * vmu).sum()),
+                            (zero_p, zero_m, vth, (gth * vth).sum())):
+        _, _, tp, tm = model_jvp(case.params, case.phi, case.mu, vp, vm, vt, n)
+        want = (wphi * tp).sum() + (wmu * tm).sum()
+        assert got == pytest.approx(want, rel=1e-11, abs=1e-13 * (np.abs(wphi * tp).sum() + np.abs(wmu * tm).sum()))
+    # the halo entries of the initial-state gradient are zero (they are copies of the interior)
+    assert not g0[:, :3].any() and not g0[:, -3:].any() and not g0[:, :, :3].any() and not m0[:, :, -3:].any()
