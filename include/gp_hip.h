@@ -160,6 +160,23 @@ int gp_step(gp_handle *h, int64_t nsteps, gp_stats *stats);
 /* Diagnostics of the current state without stepping (max/min, solute, energy; rel_change of the last step). */
 int gp_get_stats(gp_handle *h, gp_stats *stats);
 
+/* Solute and energy (the gp_stats fields solute[NC] and energy, as gp_get_stats computes them) of a caller's
+ * device-resident state, with no device allocation and no synchronisation.
+ * No counterpart in the reference. It is for callers that cannot allocate device memory: code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc.
+ *   phi, mu  device arrays in the reference's halo-inclusive layout (as gp_upload takes on the host: each component
+ *            rows_x*rows_y elements, index x*rows_y + y, rows = mesh + 6), of the handle's element type (double for
+ *            GP_FP64, float for GP_FP32); the halo cells must hold the boundary images (gp_download gives such arrays)
+ *   out      device array of NC + 1 doubles: solute[0..NC-1], then energy
+ *   stream   hipStream_t to enqueue on, or NULL for the handle's stream
+ * The per-block sums go to a device array the library holds for the life of the process, one per GPU, of
+ * GP_NOALLOC_SCRATCH_DOUBLES doubles: a grid needs (NC + 1) * ceil(mesh_y/64) * ceil(mesh_x/4) of them (8192^2 with
+ * NC = 2 fits), larger grids are rejected with GP_E_ARG. Every call on one GPU shares that array: calls that overlap
+ * in time on one GPU (other streams or handles) race and must be ordered by the caller. The handle's device must be
+ * current. */
+#define GP_NOALLOC_SCRATCH_DOUBLES (1 << 20)
+int gp_invariants_noalloc(gp_handle *h, const void *phi, const void *mu, double *out, void *stream);
+
 /* Block until all work queued on the handle's stream is done. */
 int gp_synchronize(gp_handle *h);
 

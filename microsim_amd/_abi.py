@@ -34,6 +34,8 @@ GP_PROFILE_OFF = 0
 GP_PROFILE_LAUNCHES = 1
 GP_PROFILE_CALLS = 2
 
+GP_NOALLOC_SCRATCH_DOUBLES = 1 << 20
+
 P, NCM = GP_MAXP, GP_MAXNC
 
 
@@ -149,6 +151,7 @@ def _declare_hip(lib):
     lib.gp_theta_values.argtypes = [h, _dp]
     lib.gp_adjoint_forward.argtypes = [h, C.c_int64, C.c_int64]
     lib.gp_adjoint_backward.argtypes = [h, _dp, _dp, _dp, _dp, _dp]
+    lib.gp_invariants_noalloc.argtypes = [h, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
 
 
 _host = None

@@ -18,6 +18,8 @@
  * (phases, mu fields) in gp_step_impl.h, instantiated once per supported pair by gp_ops.hip.
  * Reference routines replaced: see include/gp_hip.h.
  */
+#include <mutex>
+
 #include "gp_internal.h"
 #include "gp_adjoint.h"
 #include "gp_cells.h"
@@ -26,6 +28,33 @@ using namespace gpk;
 
 namespace {
 char g_create_error[512];
+
+/* Scratch of gp_invariants_noalloc: the per-block sums of k_invariants, in a device array the library holds for the
+ * life of the process (one copy per GPU). Host code never passes `scratch` itself to a kernel (in host code the
+ * name stands for a host-side shadow, not a device address): it uses the device address that hipGetSymbolAddress
+ * returned for the current GPU, looked up once per GPU when the first handle is created there. */
+static __device__ __align__(256) double scratch[GP_NOALLOC_SCRATCH_DOUBLES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double *g_scratch_ptr[kMaxDevices];
+bool g_scratch_done[kMaxDevices];
+
+/* the device address of `scratch` on GPU `dev` (current), looked up on the first call for that GPU */
+int scratch_lookup(int dev, double **out, char *err) {
+  if (dev < 0 || dev >= kMaxDevices) return snprintf(err, 512, "device index %d out of range", dev), GP_E_ARG;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_done[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess)
+      return snprintf(err, 512, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e)), GP_E_HIP;
+    g_scratch_ptr[dev] = (double *)p;
+    g_scratch_done[dev] = true;
+  }
+  *out = g_scratch_ptr[dev];
+  return GP_OK;
+}
 }  // namespace
 
 int gp_fail(gp_handle *h, int code, const char *fmt, ...) {
@@ -246,6 +275,15 @@ int gp_create(const gp_params *params, int precision, gp_handle **out) {
     delete h;
     return gp_fail(nullptr, GP_E_NODEV, "hipGetDevice failed");
   }
+  {
+    /* device address of gp_invariants_noalloc's scratch on this GPU (looked up once per GPU) */
+    double *sp = nullptr;
+    rc = scratch_lookup(h->device, &sp, g_create_error);
+    if (rc) {
+      delete h;
+      return rc;
+    }
+  }
   h->nblocks_mu = ((params->mesh_y + kBX - 1) / kBX) * ((params->mesh_x + kBY - 1) / kBY);
   march_plan(h);
   h->nparts_max = std::max(h->nblocks_mu, std::max(2 * h->march_nwaves, h->cell_nwaves + 4 * (int64_t)kListBlocks));
@@ -437,6 +475,30 @@ int gp_get_stats(gp_handle *h, gp_stats *st) {
   st->energy = inv[NC];
   st->step = h->steps;
   return GP_OK;
+}
+
+int gp_invariants_noalloc(gp_handle *h, const void *phi, const void *mu, double *out, void *stream) {
+  if (!h || !phi || !mu || !out) return h ? gp_fail(h, GP_E_ARG, "null argument") : GP_E_ARG;
+  const int64_t mx = h->p.mesh_x, my = h->p.mesh_y;
+  /* the block grid of k_invariants (one block per 64 y-cells by 4 x-rows of the interior) and its block sums */
+  const int64_t nblocks = ((my + 63) / 64) * ((mx + 3) / 4);
+  const int64_t need = (int64_t)(h->NC + 1) * nblocks;
+  if (need > (int64_t)GP_NOALLOC_SCRATCH_DOUBLES)
+    return gp_fail(h, GP_E_ARG,
+                   "gp_invariants_noalloc: a %lldx%lld grid needs %lld scratch doubles, more than the %lld the library "
+                   "holds (GP_NOALLOC_SCRATCH_DOUBLES)",
+                   (long long)mx, (long long)my, (long long)need, (long long)GP_NOALLOC_SCRATCH_DOUBLES);
+  int dev = -1;
+  HIPCHK(h, hipGetDevice(&dev));
+  if (dev != h->device)
+    return gp_fail(h, GP_E_ARG, "gp_invariants_noalloc: the handle lives on device %d, device %d is current",
+                   h->device, dev);
+  double *sp = nullptr;
+  int rc = scratch_lookup(dev, &sp, h->err);
+  if (rc) return rc;
+  /* the caller's state is in the reference's plain layout: plane = rows_x * rows_y, row pitch rows_y, no offset */
+  return h->ops->invariants_on(h, phi, mu, h->rx * h->ry, h->ry, 0, sp, out,
+                               stream ? (hipStream_t)stream : h->stream);
 }
 
 int gp_step(gp_handle *h, int64_t nsteps, gp_stats *stats) {
