@@ -37,6 +37,7 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
+MODES = ("fp32c", "fp32", "fp64")
 METRIC = "cell-updates/s & HBM-roofline % (FP64 vs FP32); adjoint/forward time ratio"
 
 
@@ -111,7 +112,7 @@ def run_mode(case, precision: int, steps: int, warmup: int, late_at: int = 0):
     from microsim_amd.solver import GPSolver
     N, NC = case.N, case.NC
     cells = int(case.params.mesh_x) * int(case.params.mesh_y)
-    es = 8 if precision == 0 else 4
+    es = 8 if precision == 0 else 4  # algorithmic bytes: FP32 state for both FP32 modes
     b_alg = 2 * (N + NC) * es
 
     def window(s, done):
@@ -248,7 +249,7 @@ def traffic_per_launch(kernel: str, precision: int):
     path = os.path.join(REPO, "profiles", "traffic_tern4096.txt")
     if not os.path.exists(path):
         return None
-    key = "%s %s bytes_per_launch" % (kernel, "fp64" if precision == 0 else "fp32")
+    key = "%s %s bytes_per_launch" % (kernel, {0: "fp64", 1: "fp32", 2: "fp32c"}.get(precision, "fp32"))
     for line in open(path):
         if line.startswith(key):
             return float(line.split("=")[1])
@@ -270,12 +271,14 @@ def precision_errors(case, steps: int, checkpoints=(100, 500, 1000, 2000, 5000, 
     import numpy as np
     t0 = time.perf_counter()
     runs = {"fp64": GPSolver(case.params, _abi.GP_FP64), "fp32": GPSolver(case.params, _abi.GP_FP32),
+            "fp32c": GPSolver(case.params, _abi.GP_FP32C),
             "fp64_rounded_init": GPSolver(case.params, _abi.GP_FP64)}
     res = {"steps": steps, "mesh": [int(case.params.mesh_x), int(case.params.mesh_y)], "at": []}
     sl = (slice(None), slice(3, -3), slice(3, -3))
     try:
         runs["fp64"].upload(case.phi, case.mu)
         runs["fp32"].upload(case.phi, case.mu)
+        runs["fp32c"].upload(case.phi, case.mu)
         runs["fp64_rounded_init"].upload(case.phi.astype(np.float32).astype(np.float64),
                                           case.mu.astype(np.float32).astype(np.float64))
         done = 0
@@ -286,7 +289,7 @@ def precision_errors(case, steps: int, checkpoints=(100, 500, 1000, 2000, 5000, 
             ref = [np.ascontiguousarray(a[sl]) for a in runs["fp64"].download()]
             cref = host.composition(case.params, ref[0], ref[1])
             row = {"step": cp}
-            for name in ("fp32", "fp64_rounded_init"):
+            for name in ("fp32c", "fp32", "fp64_rounded_init"):
                 got = [np.ascontiguousarray(a[sl]) for a in runs[name].download()]
                 cgot = host.composition(case.params, got[0], got[1])
                 row[name] = {}
@@ -415,12 +418,13 @@ def main():
               "(multi-GPU sharding is a design note, not a measured path)" % a.gpus, file=sys.stderr)
         sys.exit(2)
     from microsim_amd import _abi
+    PRECISIONS = {"fp64": _abi.GP_FP64, "fp32": _abi.GP_FP32, "fp32c": _abi.GP_FP32C}
     case, _ = make_case(a.mesh, a.seeds)
     copy = copy_bandwidth()
     log("copy kernel", json.dumps(copy))
     res = {}
     for mode in a.modes.split(","):
-        prec = _abi.GP_FP32 if mode == "fp32" else _abi.GP_FP64
+        prec = PRECISIONS[mode]
         res[mode] = run_mode(case, prec, a.steps, a.warmup, a.late_at)
         log(mode, json.dumps(res[mode]))
     first = a.modes.split(",")[0]
@@ -428,7 +432,7 @@ def main():
     out = {
         "metric": METRIC, "value": r0["value"], "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": r0["ms_per_step"], "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32" if first == "fp32" else "f64", "data": "synthetic",
+        "vs_baseline": None, "dtype": "f64" if first == "fp64" else "f32", "data": "synthetic",
         "config": {"workload": "GP ternary 3-phase %dx%d, %d seeds, %s mode" % (a.mesh, a.mesh, a.seeds, first),
                    "mesh": [a.mesh, a.mesh], "phases": 3, "components": 3, "seeds": a.seeds,
                    "parallelism": "single-gpu"},
@@ -439,6 +443,8 @@ def main():
     }
     if "fp32" in res and "fp64" in res:
         out["fp32_over_fp64"] = res["fp32"]["value"] / res["fp64"]["value"]
+    if "fp32c" in res and "fp64" in res:
+        out["fp32c_over_fp64"] = res["fp32c"]["value"] / res["fp64"]["value"]
     if a.adjoint_steps > 0:
         out["adjoint"] = adjoint_ratio(case.params, case.phi, case.mu, a.adjoint_steps)
         out["adjoint"]["workload"] = "this workload, FP64, random linear loss of the final state"

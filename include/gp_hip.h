@@ -59,6 +59,10 @@ extern "C" {
 #define GP_FP64 0            /* FP64 storage and arithmetic, same operation order as the reference */
 #define GP_FP32 1            /* FP32 phi/mu storage and stencil arithmetic (bulk mu increments in direct form),
                                 FP64 statistics and set-up */
+#define GP_FP32C 2           /* compensated FP32: phi/mu stored as FP32 plus an 8-bit extension of each value
+                                (about 31 significant bits); bulk cells step in FP32 arithmetic with the increment
+                                added to the extended value, interface cells step in FP64 arithmetic from the
+                                extended values; FP64 statistics and set-up. The cell-classified step only. */
 
 /* closed-form solve of d(mu) = (dc/dmu)^-1 dc (GPs/solverloop/calculate_divergence_concentration.h:93-125) */
 #define GP_SOLVE_LU 0
@@ -166,7 +170,7 @@ int gp_get_stats(gp_handle *h, gp_stats *stats);
  * capture, serving loops, harnesses that forbid hipMalloc.
  *   phi, mu  device arrays in the reference's halo-inclusive layout (as gp_upload takes on the host: each component
  *            rows_x*rows_y elements, index x*rows_y + y, rows = mesh + 6), of the handle's element type (double for
- *            GP_FP64, float for GP_FP32); the halo cells must hold the boundary images (gp_download gives such arrays)
+ *            GP_FP64 and for the decoded values of GP_FP32C, float for GP_FP32); the halo cells must hold the boundary images (gp_download gives such arrays)
  *   out      device array of NC + 1 doubles: solute[0..NC-1], then energy
  *   stream   hipStream_t to enqueue on, or NULL for the handle's stream
  * The per-block sums go to a device array the library holds for the life of the process, one per GPU, of

@@ -23,6 +23,7 @@ GP_E_NODEV = -6
 
 GP_FP64 = 0
 GP_FP32 = 1
+GP_FP32C = 2
 
 GP_SOLVE_LU = 0
 GP_SOLVE_BINARY = 1

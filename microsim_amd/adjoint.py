@@ -119,5 +119,5 @@ def trajectory(params: _abi.GPParams, theta: torch.Tensor, phi0: torch.Tensor, m
     their halo entries are overwritten from the interior by the boundary conditions and get zero gradient.
     every: checkpoint interval of the reverse sweep (0: about sqrt(nsteps)); device memory for the adjoint is
     (nsteps / every + every + 3) copies of the state (the checkpoints in the handle's precision).
-    precision: _abi.GP_FP64 (default) or _abi.GP_FP32."""
+    precision: _abi.GP_FP64 (default), _abi.GP_FP32 or _abi.GP_FP32C."""
     return _Trajectory.apply(theta, phi0, mu0, params, int(nsteps), int(every), int(precision))

@@ -46,10 +46,10 @@ def main(argv=None) -> int:
     ap.add_argument("--iters", type=int, default=15)
     ap.add_argument("--lr", type=float, default=0.05)
     ap.add_argument("--lr-decay", type=float, default=0.8, help="step size factor per iteration")
-    ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32")
+    ap.add_argument("--precision", choices=("fp32c", "fp32", "fp64"), default="fp32")
     ap.add_argument("--offset", type=float, default=0.2, help="relative offset of the starting parameters")
     a = ap.parse_args(argv)
-    prec = _abi.GP_FP32 if a.precision == "fp32" else _abi.GP_FP64
+    prec = {"fp32c": _abi.GP_FP32C, "fp32": _abi.GP_FP32, "fp64": _abi.GP_FP64}[a.precision]
     case = _case(a.mesh, a.seeds)
     names = adjoint.theta_names(case.N, case.NC + 1)
     idx = [names.index(n) for n in SELECT]
@@ -71,7 +71,10 @@ def main(argv=None) -> int:
     opt = torch.optim.Adam([z], lr=a.lr)
     sched = torch.optim.lr_scheduler.ExponentialLR(opt, gamma=a.lr_decay)
     base = torch.tensor(th_start)
-    state_bytes = (case.N + case.NC) * case.phi.shape[1] * case.phi.shape[2] * (4 if prec == _abi.GP_FP32 else 8)
+    cells = case.phi.shape[1] * case.phi.shape[2]
+    # bytes of one stored state: FP32 planes (plus phi's FP32 low parts and mu's 8-bit extensions in fp32c), or FP64
+    state_bytes = {_abi.GP_FP64: 8 * (case.N + case.NC), _abi.GP_FP32: 4 * (case.N + case.NC),
+                   _abi.GP_FP32C: 8 * case.N + 5 * case.NC}[prec] * cells
     ncheck = (a.steps + every - 1) // every
     adj_bytes = (ncheck + every) * state_bytes + 3 * (case.N + case.NC) * case.phi.shape[1] * case.phi.shape[2] * 8
     hist = []

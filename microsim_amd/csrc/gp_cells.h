@@ -77,13 +77,15 @@ constexpr bool kLeanBulk = sizeof(R) < sizeof(double);
  * only lanes with `store` write, the mu statistics take every lane (the caller resets the lanes without a cell
  * once per tile) and the phi extrema (e_b on every cell) are left to the caller, once per tile. Otherwise the
  * caller calls it on the bulk lanes only, and it folds the full statistics. */
-template <int N, int NC, typename R, bool P2, bool STATS, bool UNIFORM, int NSA>
+template <int N, int NC, typename R, bool P2, bool STATS, bool UNIFORM, int NSA, bool C = false>
 __device__ __forceinline__ void cbulk_cell(const Model<R> &m, int b, int64_t x, int64_t c, bool img,
                                            const CellTile &t, const R *m0,
                                            const R *m1, const R *m2, const R *mud, const R *muu, R divjat0,
                                            R *__restrict__ phi_o, R *__restrict__ mu_o,
                                            int8_t *__restrict__ codes_out, const Maps &maps, int64_t fs, int64_t ps,
-                                           int64_t mx, bool kept, bool store, bool live, R (&st)[NSA]) {
+                                           int64_t mx, bool kept, bool store, bool live, R (&st)[NSA],
+                                           const int8_t *q1 = nullptr, float *__restrict__ plo_o = nullptr,
+                                           int8_t *__restrict__ mlo_o = nullptr) {
   using Vw = View<R, P2>;
   R pe[N];
 #pragma unroll
@@ -196,6 +198,17 @@ __device__ __forceinline__ void cbulk_cell(const Model<R> &m, int b, int64_t x, 
     }
   }
   }
+  int8_t qn[NC];
+  if constexpr (C) {
+    /* compensated mode: the increment is added to the decoded value (hi plus its extension) and the sum is
+     * re-encoded, so the bits of the increment below the FP32 part are kept */
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+      float hn;
+      c32_encode(c32_decode((float)m1[k], q1[k]) + (double)deltamu[k], hn, qn[k]);
+      munew[k] = (R)hn;
+    }
+  }
   if (store) {
     const int8_t code = (int8_t)b;
     /* kept: the output buffers hold the state before the current one and its codes; a cell that was purely
@@ -203,14 +216,27 @@ __device__ __forceinline__ void cbulk_cell(const Model<R> &m, int b, int64_t x, 
     if (!kept) {
 #pragma unroll
       for (int a = 0; a < N; a++) phi_o[a * fs + c] = pe[a];
+      if constexpr (C) {
+#pragma unroll
+        for (int a = 0; a < N; a++) plo_o[a * fs + c] = 0.0f;
+      }
       codes_out[c] = code;
     }
 #pragma unroll
     for (int k = 0; k < NC; k++) mu_o[k * fs + c] = munew[k];
+    if constexpr (C) {
+#pragma unroll
+      for (int k = 0; k < NC; k++) mlo_o[k * fs + c] = qn[k];
+    }
     if (img) {
       write_images(codes_out, 1, &code, 0, maps, x, t.y, 0, ps);
       write_images(phi_o, N, pe, 0, maps, x, t.y, fs, ps);
       write_images(mu_o, NC, munew, 1, maps, x, t.y, fs, ps);
+      if constexpr (C) {
+        const float z[N] = {};
+        write_images(plo_o, N, z, 0, maps, x, t.y, fs, ps);
+        write_images(mlo_o, NC, qn, 1, maps, x, t.y, fs, ps);
+      }
     }
   }
   if (STATS && UNIFORM && live) {
@@ -295,14 +321,16 @@ __device__ __forceinline__ void dring_cell(int64_t i, const int8_t *__restrict__
   if (cand) clist[base + __popcll(mask & ((1ull << lane) - 1ull))] = (int32_t)c;
 }
 
-template <int N, int NC, typename R, bool P2, bool STATS>
+template <int N, int NC, typename R, bool P2, bool STATS, bool C = false>
 __global__ __launch_bounds__(256) void k_cbulk(const Model<R> *__restrict__ mp, const R *__restrict__ mu,
                                                const int8_t *__restrict__ codes_in, R *__restrict__ phi_o,
                                                R *__restrict__ mu_o, int8_t *__restrict__ codes_out, Maps maps,
                                                double *__restrict__ partials, int64_t pstride, int64_t fs, int64_t ps,
                                                int64_t mx, int64_t my, int seg_len, int nstrips, int nwaves,
                                                int keep, int32_t *__restrict__ ilist, int32_t *__restrict__ clist,
-                                               int *counts, int *counts_next, int tile_blocks) {
+                                               int *counts, int *counts_next, int tile_blocks,
+                                               const int8_t *__restrict__ mlo, float *__restrict__ plo_o,
+                                               int8_t *__restrict__ mlo_o) {
   using Vw = View<R, P2>;
   constexpr int NS = 3 * N + 3 * NC + 1;
   constexpr int NSA = STATS ? NS : 1;
@@ -335,6 +363,7 @@ __global__ __launch_bounds__(256) void k_cbulk(const Model<R> *__restrict__ mp, 
    * in between covers a newer load (the counter of outstanding memory operations is in order). */
   constexpr int RG = kCellRing<R>, D = kCellBulkDepth<R>;
   R ring[RG][NC];
+  int8_t qring[RG][NC]; /* compensated mode: the mu extensions of the ring's rows */
   uint8_t kc[RG]; /* output codes as bytes (zero-extending loads, no sign extension after the load) */
   const uint8_t *codes_ou = (const uint8_t *)codes_out;
   if (w < nwaves) {
@@ -352,6 +381,10 @@ __global__ __launch_bounds__(256) void k_cbulk(const Model<R> *__restrict__ mp, 
       const int64_t xr = t.xa + d <= t.xb + 1 ? t.xa + d : t.xb + 1;
 #pragma unroll
       for (int k = 0; k < NC; k++) ring[(d + RG) % RG][k] = mu[k * fs + xr * ps + c0];
+      if constexpr (C) {
+#pragma unroll
+        for (int k = 0; k < NC; k++) qring[(d + RG) % RG][k] = mlo[k * fs + xr * ps + c0];
+      }
     }
 #pragma unroll
     for (int d = 0; d < D; d++) kc[d] = codes_ou[(t.xa + d <= t.xb ? t.xa + d : t.xb) * ps + c0];
@@ -402,9 +435,10 @@ __global__ __launch_bounds__(256) void k_cbulk(const Model<R> *__restrict__ mp, 
             muu[k] = lane_up(m1[k]);
           }
           if (UNI) {
-            cbulk_cell<N, NC, R, P2, STATS, true>(m, tb, x, c, img, t, m0, m1, m2, mud, muu, divjat0, phi_o, mu_o,
-                                                  codes_out, maps, fs, ps, mx, keep && kc[j] == (uint8_t)tb,
-                                                  live && t.out_lane, live, st);
+            cbulk_cell<N, NC, R, P2, STATS, true, NSA, C>(m, tb, x, c, img, t, m0, m1, m2, mud, muu, divjat0, phi_o,
+                                                          mu_o, codes_out, maps, fs, ps, mx,
+                                                          keep && kc[j] == (uint8_t)tb, live && t.out_lane, live, st,
+                                                          qring[j], plo_o, mlo_o);
           } else if (live) {
             const int bl = t.out_lane ? bulk_of(cs, x - xa + 2, lane) : -1;
             {
@@ -424,14 +458,19 @@ __global__ __launch_bounds__(256) void k_cbulk(const Model<R> *__restrict__ mp, 
               ccnt += __popcll(mc);
             }
             if (bl >= 0)
-              cbulk_cell<N, NC, R, P2, STATS, false>(m, bl, x, c, img, t, m0, m1, m2, mud, muu, divjat0, phi_o, mu_o,
-                                                     codes_out, maps, fs, ps, mx, keep && kc[j] == (uint8_t)bl, true,
-                                                     true, st);
+              cbulk_cell<N, NC, R, P2, STATS, false, NSA, C>(m, bl, x, c, img, t, m0, m1, m2, mud, muu, divjat0,
+                                                             phi_o, mu_o, codes_out, maps, fs, ps, mx,
+                                                             keep && kc[j] == (uint8_t)bl, true, true, st, qring[j],
+                                                             plo_o, mlo_o);
           }
           /* refill: row x + D + 1 into the slot of row x - 1, the output codes of row x + D */
           const int xn = min(xoff + (D + 1) * pso, xlast);
 #pragma unroll
           for (int k = 0; k < NC; k++) ring[(j + RG - 1) % RG][k] = mu[k * fs + (int64_t)(xn + c0i)];
+          if constexpr (C) {
+#pragma unroll
+            for (int k = 0; k < NC; k++) qring[(j + RG - 1) % RG][k] = mlo[k * fs + (int64_t)(xn + c0i)];
+          }
           kc[(j + D) % RG] = codes_ou[(int64_t)(xn - pso + c0i)];
           xoff += pso;
         }

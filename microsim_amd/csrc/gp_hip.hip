@@ -119,8 +119,10 @@ static void stats_reset_host(gp_handle *h, double *s) {
   s[3 * N + 3 * NC] = 0.0;
 }
 
-/* halo-inclusive host planes (rows_x*rows_y, y fastest) -> device planes, with the BC maps applied */
-static int upload_planes(gp_handle *h, void *dst, const double *src, int ncomp, int field) {
+/* halo-inclusive host planes (rows_x*rows_y, y fastest) -> device planes, with the BC maps applied; in GP_FP32C
+ * each value is encoded into its FP32 part (dst) and its second part (dst_lo: the FP32 low part of phi, field 0,
+ * or the 8-bit extension of mu, field 1) */
+static int upload_planes(gp_handle *h, void *dst, void *dst_lo, const double *src, int ncomp, int field) {
   const int64_t rx = h->rx, ry = h->ry, n = rx * ry;
   std::vector<double> tmp(src, src + (size_t)ncomp * n);
   const int64_t hx[6] = {0, 1, 2, rx - 3, rx - 2, rx - 1}, hy[6] = {0, 1, 2, ry - 3, ry - 2, ry - 1};
@@ -133,31 +135,47 @@ static int upload_planes(gp_handle *h, void *dst, const double *src, int ncomp, 
   }
   const size_t es = esize(h);
   std::vector<char> stage((size_t)ncomp * h->fs * es, 0);
+  const size_t les = field == 0 ? sizeof(float) : 1;
+  std::vector<char> stage_lo(h->precision == GP_FP32C ? (size_t)ncomp * h->fs * les : 0, 0);
   for (int c = 0; c < ncomp; c++)
     for (int64_t x = 0; x < rx; x++)
       for (int64_t y = 0; y < ry; y++) {
         const double v = tmp[(size_t)c * n + x * ry + y];
         const size_t o = ((size_t)c * h->fs + x * h->ps + y + kYOff);
         if (h->precision == GP_FP64) ((double *)stage.data())[o] = v;
+        else if (h->precision == GP_FP32C && field == 0) df_encode(v, ((float *)stage.data())[o], ((float *)stage_lo.data())[o]);
+        else if (h->precision == GP_FP32C) c32_encode(v, ((float *)stage.data())[o], ((int8_t *)stage_lo.data())[o]);
         else ((float *)stage.data())[o] = (float)v;
       }
   HIPCHK(h, hipMemcpyAsync(dst, stage.data(), stage.size(), hipMemcpyHostToDevice, h->stream));
+  if (h->precision == GP_FP32C)
+    HIPCHK(h, hipMemcpyAsync(dst_lo, stage_lo.data(), stage_lo.size(), hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return GP_OK;
 }
 
-static int download_planes(gp_handle *h, double *dst, const void *src, int ncomp) {
+/* device planes -> halo-inclusive host planes. es: element size of the planes (8: FP64, 4: FP32); src_lo: the
+ * GP_FP32C second parts of FP32 planes (decoded into the values: FP32 low parts of phi, field 0, or 8-bit
+ * extensions of mu, field 1), or NULL */
+static int download_planes(gp_handle *h, double *dst, const void *src, const void *src_lo, int field, int ncomp,
+                           size_t es) {
   const int64_t rx = h->rx, ry = h->ry, n = rx * ry;
-  const size_t es = esize(h);
   std::vector<char> stage((size_t)ncomp * h->fs * es);
+  const size_t les = field == 0 ? sizeof(float) : 1;
+  std::vector<char> stage_lo(src_lo ? (size_t)ncomp * h->fs * les : 0);
   HIPCHK(h, hipMemcpyAsync(stage.data(), src, stage.size(), hipMemcpyDeviceToHost, h->stream));
+  if (src_lo) HIPCHK(h, hipMemcpyAsync(stage_lo.data(), src_lo, stage_lo.size(), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   for (int c = 0; c < ncomp; c++)
     for (int64_t x = 0; x < rx; x++)
       for (int64_t y = 0; y < ry; y++) {
         const size_t o = ((size_t)c * h->fs + x * h->ps + y + kYOff);
-        dst[(size_t)c * n + x * ry + y] =
-            h->precision == GP_FP64 ? ((double *)stage.data())[o] : (double)((float *)stage.data())[o];
+        double v;
+        if (es == sizeof(double)) v = ((double *)stage.data())[o];
+        else if (src_lo && field == 0) v = df_decode(((float *)stage.data())[o], ((float *)stage_lo.data())[o]);
+        else if (src_lo) v = c32_decode(((float *)stage.data())[o], ((int8_t *)stage_lo.data())[o]);
+        else v = (double)((float *)stage.data())[o];
+        dst[(size_t)c * n + x * ry + y] = v;
       }
   return GP_OK;
 }
@@ -186,16 +204,23 @@ static int validate(const gp_params *p, char *err) {
 /* ------------------------------------------------------------------------------------------------ adjoint */
 
 /* the current state into / from a state buffer laid out phi planes then mu planes */
+/* (a GP_FP32C state buffer holds the extensions after the planes) */
 static int state_save(gp_handle *h, double *dst) {
   const size_t es = esize(h), pb = (size_t)h->N * h->fs * es, mb = (size_t)h->NC * h->fs * es;
   HIPCHK(h, hipMemcpyAsync(dst, h->phi[h->cur], pb, hipMemcpyDeviceToDevice, h->stream));
   HIPCHK(h, hipMemcpyAsync((char *)dst + pb, h->mu[h->cur], mb, hipMemcpyDeviceToDevice, h->stream));
+  if (h->precision == GP_FP32C)
+    HIPCHK(h, hipMemcpyAsync((char *)dst + pb + mb, h->lo[h->cur], c32_lo_bytes(h), hipMemcpyDeviceToDevice,
+                             h->stream));
   return GP_OK;
 }
 static int state_load(gp_handle *h, const double *src) {
   const size_t es = esize(h), pb = (size_t)h->N * h->fs * es, mb = (size_t)h->NC * h->fs * es;
   HIPCHK(h, hipMemcpyAsync(h->phi[h->cur], src, pb, hipMemcpyDeviceToDevice, h->stream));
   HIPCHK(h, hipMemcpyAsync(h->mu[h->cur], (const char *)src + pb, mb, hipMemcpyDeviceToDevice, h->stream));
+  if (h->precision == GP_FP32C)
+    HIPCHK(h, hipMemcpyAsync(h->lo[h->cur], (const char *)src + pb + mb, c32_lo_bytes(h), hipMemcpyDeviceToDevice,
+                             h->stream));
   h->flags_valid = false;
   h->codes_valid = false;
   h->prev_valid = false;
@@ -241,7 +266,8 @@ int gp_create(const gp_params *params, int precision, gp_handle **out) {
   *out = nullptr;
   int rc = validate(params, g_create_error);
   if (rc) return rc;
-  if (precision != GP_FP64 && precision != GP_FP32) return gp_fail(nullptr, GP_E_ARG, "unknown precision %d", precision);
+  if (precision != GP_FP64 && precision != GP_FP32 && precision != GP_FP32C)
+    return gp_fail(nullptr, GP_E_ARG, "unknown precision %d", precision);
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return gp_fail(nullptr, GP_E_NODEV, "no HIP device");
   gp_handle *h = new gp_handle();
@@ -292,6 +318,12 @@ int gp_create(const gp_params *params, int precision, gp_handle **out) {
   /* the cell lists and k_cbulk's row offsets are 32-bit: grids beyond that bound (cells_fit, about 46000^2 cells)
    * take the tiled march, which gives the same results */
   h->use_cells = h->use_march && !(path && strcmp(path, "march") == 0) && cells_fit(h->rx, h->ps);
+  if (precision == GP_FP32C && !h->use_cells) {
+    delete h;
+    return gp_fail(nullptr, GP_E_ARG,
+                   "the compensated FP32 mode runs the cell-classified step only (GP_PATH selects another path, or the "
+                   "grid has 2^31 cells or more)");
+  }
   h->prev_valid = false;
   h->fcur = 0;
   h->flags_valid = false;
@@ -302,7 +334,11 @@ int gp_create(const gp_params *params, int precision, gp_handle **out) {
   bool ok = e == hipSuccess;
   ok = ok && hipMalloc(&h->phi[0], pn) == hipSuccess && hipMalloc(&h->phi[1], pn) == hipSuccess;
   ok = ok && hipMalloc(&h->mu[0], mn) == hipSuccess && hipMalloc(&h->mu[1], mn) == hipSuccess;
-  ok = ok && hipMalloc(&h->dphi, pn) == hipSuccess;
+  ok = ok && hipMalloc(&h->dphi, (size_t)h->N * h->fs * dphi_esize(h)) == hipSuccess;
+  if (precision == GP_FP32C)
+    ok = ok && hipMalloc((void **)&h->lo[0], c32_lo_bytes(h)) == hipSuccess &&
+         hipMalloc((void **)&h->lo[1], c32_lo_bytes(h)) == hipSuccess &&
+         hipMalloc(&h->model64, sizeof(Model<double>)) == hipSuccess;
   ok = ok && hipMalloc((void **)&h->partials, sizeof(double) * kNStatMax * h->nparts_max) == hipSuccess;
   ok = ok && hipMalloc((void **)&h->stats_dev, sizeof(double) * kNStatMax) == hipSuccess;
   ok = ok && hipMalloc((void **)&h->inv_dev, sizeof(double) * (GP_MAXNC + 1)) == hipSuccess;
@@ -322,7 +358,7 @@ int gp_create(const gp_params *params, int precision, gp_handle **out) {
   }
   ok = hipMemsetAsync(h->phi[0], 0, pn, h->stream) == hipSuccess && hipMemsetAsync(h->phi[1], 0, pn, h->stream) == hipSuccess &&
        hipMemsetAsync(h->mu[0], 0, mn, h->stream) == hipSuccess && hipMemsetAsync(h->mu[1], 0, mn, h->stream) == hipSuccess &&
-       hipMemsetAsync(h->dphi, 0, pn, h->stream) == hipSuccess &&
+       hipMemsetAsync(h->dphi, 0, (size_t)h->N * h->fs * dphi_esize(h), h->stream) == hipSuccess &&
        hipMemsetAsync(h->lcount, 0, 4 * sizeof(int), h->stream) == hipSuccess;
   h->lsel = 0;
   if (precision == GP_FP64) {
@@ -336,6 +372,14 @@ int gp_create(const gp_params *params, int precision, gp_handle **out) {
     make_model(params, &m);
     h->pow2 = m.pow2 != 0;
     ok = ok && hipMemcpyAsync(h->model, &m, sizeof m, hipMemcpyHostToDevice, h->stream) == hipSuccess;
+    if (precision == GP_FP32C) {
+      Model<double> m64;
+      make_model(params, &m64);
+      ok = ok && hipMemsetAsync(h->lo[0], 0, c32_lo_bytes(h), h->stream) == hipSuccess &&
+           hipMemsetAsync(h->lo[1], 0, c32_lo_bytes(h), h->stream) == hipSuccess &&
+           hipMemcpyAsync(h->model64, &m64, sizeof m64, hipMemcpyHostToDevice, h->stream) == hipSuccess;
+      ok = ok && hipStreamSynchronize(h->stream) == hipSuccess;
+    }
     ok = ok && hipStreamSynchronize(h->stream) == hipSuccess;
   }
   if (!ok) {
@@ -362,8 +406,9 @@ void gp_destroy(gp_handle *h) {
   }
   h->prof.clear();
   delete h->adj;
-  void *bufs[] = {h->phi[0], h->phi[1], h->mu[0], h->mu[1], h->dphi, h->partials, h->stats_dev, h->model,
-                  h->flags[0], h->flags[1], h->inv_dev, h->codes[0], h->codes[1], h->clist, h->ilist, h->lcount};
+  void *bufs[] = {h->phi[0], h->phi[1], h->mu[0],    h->mu[1],    h->dphi,   h->partials, h->stats_dev,
+                  h->model,  h->flags[0], h->flags[1], h->inv_dev, h->codes[0], h->codes[1], h->clist,
+                  h->ilist,  h->lcount, h->lo[0],    h->lo[1],    h->model64, h->wide[0],  h->wide[1]};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -372,14 +417,16 @@ void gp_destroy(gp_handle *h) {
 
 int gp_upload(gp_handle *h, const double *phi, const double *mu) {
   if (!h || !phi || !mu) return h ? gp_fail(h, GP_E_ARG, "null argument") : GP_E_ARG;
-  int rc = upload_planes(h, h->phi[h->cur], phi, h->N, 0);
+  const bool c = h->precision == GP_FP32C;
+  int rc = upload_planes(h, h->phi[h->cur], c ? (void *)c32_plo(h, h->cur) : nullptr, phi, h->N, 0);
   if (rc) return rc;
-  rc = upload_planes(h, h->mu[h->cur], mu, h->NC, 1);
+  rc = upload_planes(h, h->mu[h->cur], c ? (void *)c32_mlo(h, h->cur) : nullptr, mu, h->NC, 1);
   if (rc) return rc;
   /* the other copy starts identical so its halo rows are defined */
   HIPCHK(h, hipMemcpyAsync(h->phi[1 - h->cur], h->phi[h->cur], (size_t)h->N * h->fs * esize(h), hipMemcpyDeviceToDevice, h->stream));
   HIPCHK(h, hipMemcpyAsync(h->mu[1 - h->cur], h->mu[h->cur], (size_t)h->NC * h->fs * esize(h), hipMemcpyDeviceToDevice, h->stream));
-  HIPCHK(h, hipMemsetAsync(h->dphi, 0, (size_t)h->N * h->fs * esize(h), h->stream));
+  if (c) HIPCHK(h, hipMemcpyAsync(h->lo[1 - h->cur], h->lo[h->cur], c32_lo_bytes(h), hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(h, hipMemsetAsync(h->dphi, 0, (size_t)h->N * h->fs * dphi_esize(h), h->stream));
   double s[kNStatMax] = {0};
   stats_reset_host(h, s);
   HIPCHK(h, hipMemcpyAsync(h->stats_dev, s, sizeof s, hipMemcpyHostToDevice, h->stream));
@@ -394,14 +441,16 @@ int gp_upload(gp_handle *h, const double *phi, const double *mu) {
 int gp_download(gp_handle *h, double *phi, double *mu, double *dphi) {
   if (!h) return GP_E_ARG;
   int rc;
-  if (phi && (rc = download_planes(h, phi, h->phi[h->cur], h->N))) return rc;
-  if (mu && (rc = download_planes(h, mu, h->mu[h->cur], h->NC))) return rc;
+  const bool c = h->precision == GP_FP32C;
+  const void *plo = c ? (const void *)c32_plo(h, h->cur) : nullptr, *mlo = c ? (const void *)c32_mlo(h, h->cur) : nullptr;
+  if (phi && (rc = download_planes(h, phi, h->phi[h->cur], plo, 0, h->N, esize(h)))) return rc;
+  if (mu && (rc = download_planes(h, mu, h->mu[h->cur], mlo, 1, h->NC, esize(h)))) return rc;
   if (dphi) {
     if (h->prev_valid) {
       /* single-pass steps do not store delta-phi: recompute the last step's from the previous state */
       if ((rc = h->ops->step(h, MODE_DPHI_PREV))) return rc;
     }
-    if ((rc = download_planes(h, dphi, h->dphi, h->N))) return rc;
+    if ((rc = download_planes(h, dphi, h->dphi, nullptr, 0, h->N, dphi_esize(h)))) return rc;
   }
   return GP_OK;
 }
@@ -552,7 +601,7 @@ int gp_adjoint_forward(gp_handle *h, int64_t nsteps, int64_t every) {
   const int64_t nck = (nsteps + every - 1) / every;
   for (int64_t k = 0; k < nck; k++) {
     double *b = nullptr;
-    if (hipMalloc((void **)&b, state_elems(h) * esize(h)) != hipSuccess)
+    if (hipMalloc((void **)&b, state_bytes(h)) != hipSuccess)
       return gp_fail(h, GP_E_OOM, "adjoint: checkpoint %lld does not fit in device memory", (long long)k);
     A->ckpt.push_back(b);
     int rc = state_save(h, b);
@@ -580,10 +629,10 @@ int gp_adjoint_backward(gp_handle *h, const double *gphi_T, const double *gmu_T,
               hipMalloc((void **)&A->counts, 2 * sizeof(int)) == hipSuccess;
     for (int64_t j = 0; ok && j < std::min(A->every, A->nsteps); j++) {
       double *b = nullptr;
-      ok = hipMalloc((void **)&b, se * esize(h)) == hipSuccess;
+      ok = hipMalloc((void **)&b, state_bytes(h)) == hipSuccess;
       if (ok) A->seg.push_back(b);
     }
-    if (ok && h->precision == GP_FP32) {
+    if (ok && h->precision != GP_FP64) {
       Model<double> m;
       make_model(&h->p, &m);
       ok = hipMalloc((void **)&A->st64, se * sizeof(double)) == hipSuccess &&

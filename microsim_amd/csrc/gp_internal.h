@@ -54,6 +54,13 @@ struct gp_handle {
   int device;
   void *phi[2], *mu[2], *dphi;
   void *model;              /* Model<R> in device memory */
+  /* GP_FP32C: the second parts of the state of each of the two buffers: the FP32 low parts of phi (N planes,
+   * c32_plo) then the 8-bit extensions of mu (NC planes of one byte per element, c32_mlo), at the element offsets
+   * of the FP32 planes; the FP64 model the interface cells step with; FP64 copies of the decoded state (allocated
+   * on first use: smoothing, delta-phi downloads, invariants). dphi holds FP64 planes in this mode. */
+  int8_t *lo[2];
+  void *model64;
+  double *wide[2];
   double *partials;         /* [statistic][nparts_max], statistic-major */
   double *stats_dev;        /* cumulative stats (kNStatMax) */
   double *inv_dev;          /* solute[NC], energy of the current state */
@@ -126,6 +133,16 @@ enum { MODE_SMOOTH = 0, MODE_TWOPASS = 1, MODE_MARCH = 2, MODE_DPHI_PREV = 3, MO
 
 inline size_t esize(const gp_handle *h) { return h->precision == GP_FP64 ? sizeof(double) : sizeof(float); }
 inline size_t state_elems(const gp_handle *h) { return (size_t)(h->N + h->NC) * h->fs; }
+/* bytes of the second parts of one GP_FP32C state (phi low parts, mu extensions) */
+inline size_t c32_lo_bytes(const gp_handle *h) { return (size_t)h->fs * (h->N * sizeof(float) + h->NC); }
+inline float *c32_plo(const gp_handle *h, int b) { return (float *)h->lo[b]; }
+inline int8_t *c32_mlo(const gp_handle *h, int b) { return h->lo[b] + (size_t)h->N * h->fs * sizeof(float); }
+/* bytes of one stored state: the planes, plus the second parts in GP_FP32C */
+inline size_t state_bytes(const gp_handle *h) {
+  return state_elems(h) * esize(h) + (h->precision == GP_FP32C ? c32_lo_bytes(h) : 0);
+}
+/* element size of the delta-phi planes (FP64 in GP_FP32C, where the interface cells step in FP64) */
+inline size_t dphi_esize(const gp_handle *h) { return h->precision == GP_FP32 ? sizeof(float) : sizeof(double); }
 
 /* ------------------------------------------------------------------------------------------------ adjoint */
 

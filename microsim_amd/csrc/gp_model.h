@@ -24,6 +24,7 @@
 #pragma once
 
 #include <stdint.h>
+#include <string.h>
 
 #if defined(__HIPCC__) || defined(__HIP__)
 #include <hip/hip_runtime.h>
@@ -858,6 +859,56 @@ GP_HD void mu_update(const Model<R> &m, const V &v, bool smooth, const R *dph, c
     mu_new[k] = munew[k];
     dmu[k] = deltamu[k];
   }
+}
+
+/* ------------------------------------------------------------------ compensated FP32 storage (GP_FP32C)
+ * mu: a value v is stored as hi = fl32(v) (round to nearest) and an 8-bit signed extension q counting units of
+ * ulp32(hi) / 128: v ~= hi + q * 2^(E - 157), E the biased exponent of hi (ulp32(hi) = 2^(E - 150) for a normal
+ * hi). |v - hi| <= ulp32(hi) / 2 gives |q| <= 64. Zero, subnormal and non-finite hi carry q = 0. The pair holds
+ * about 31 significant bits, relative to v.
+ * phi: the model forms 1 - phi (h, dh/dphi, the anti-trapping s(phi)), which near phi = 1 needs absolute precision
+ * far below that of 31 bits relative to 1, so phi is stored as an unevaluated sum of two FP32 values,
+ * v = hi + lo with hi = fl32(v) and lo = fl32(v - hi) (about 48 bits; 1 - phi keeps 24 bits of its own).
+ * Every conversion is an exact operation or one correctly rounded one, in FP64, so host and device agree bit for
+ * bit. */
+GP_HD uint32_t f32_bits(float f) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __float_as_uint(f);
+#else
+  uint32_t u;
+  memcpy(&u, &f, sizeof u);
+  return u;
+#endif
+}
+/* 2^k for -1022 <= k <= 1023, exactly */
+GP_HD double pow2d(int k) {
+  const uint64_t b = (uint64_t)(k + 1023) << 52;
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __longlong_as_double((long long)b);
+#else
+  double d;
+  memcpy(&d, &b, sizeof d);
+  return d;
+#endif
+}
+GP_HD double c32_decode(float hi, int8_t q) {
+  if (q == 0) return (double)hi;
+  const int E = (int)((f32_bits(hi) >> 23) & 0xffu);
+  return (double)hi + (double)q * pow2d(E - 157);
+}
+GP_HD void c32_encode(double v, float &hi, int8_t &q) {
+  hi = (float)v;
+  const int E = (int)((f32_bits(hi) >> 23) & 0xffu);
+  q = 0;
+  if (E == 0 || E == 255) return;
+  double s = rint((v - (double)hi) * pow2d(157 - E));
+  s = s > 127.0 ? 127.0 : (s < -127.0 ? -127.0 : s);
+  q = (int8_t)s;
+}
+GP_HD double df_decode(float hi, float lo) { return (double)hi + (double)lo; }
+GP_HD void df_encode(double v, float &hi, float &lo) {
+  hi = (float)v;
+  lo = (float)(v - (double)hi);
 }
 
 /* Host-side construction of the device parameter block from gp_params (same expressions as the reference). */

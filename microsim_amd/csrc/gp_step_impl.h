@@ -8,6 +8,7 @@
 #include "gp_cells.h"
 #include "gp_march.h"
 #include "gp_adjoint.h"
+#include "gp_c32.h"
 
 namespace gpk {
 
@@ -246,17 +247,23 @@ static __global__ __launch_bounds__(kStatThreads) void k_fold_sum(const double *
  * (at least (NC + 1) * invariant_blocks(mesh) doubles): the handle's own state (its plane layout) or a caller's in the
  * reference's plain halo-inclusive layout (fs = rows_x * rows_y, ps = rows_y, yoff = 0) */
 template <int N, int NC, typename R>
-static int launch_invariants_on(gp_handle *h, const void *phi, const void *mu, int64_t fs, int64_t ps, int64_t yoff,
-                                double *partials, double *out, hipStream_t stream) {
+static int launch_invariants_model(gp_handle *h, const void *model, const void *phi, const void *mu, int64_t fs,
+                                   int64_t ps, int64_t yoff, double *partials, double *out, hipStream_t stream) {
   const int64_t mx = h->p.mesh_x, my = h->p.mesh_y;
   dim3 blk(kBX, kBY);
   dim3 g2((unsigned)((my + kBX - 1) / kBX), (unsigned)((mx + kBY - 1) / kBY));
-  hipLaunchKernelGGL((k_invariants<N, NC, R>), g2, blk, 0, stream, (const Model<R> *)h->model, (const R *)phi,
+  hipLaunchKernelGGL((k_invariants<N, NC, R>), g2, blk, 0, stream, (const Model<R> *)model, (const R *)phi,
                      (const R *)mu, partials, fs, ps, yoff, mx, my);
   hipLaunchKernelGGL(k_fold_sum, dim3(NC + 1), dim3(kStatThreads), 0, stream, (const double *)partials,
                      (int64_t)g2.x * g2.y, out);
   HIPCHK(h, hipGetLastError());
   return GP_OK;
+}
+
+template <int N, int NC, typename R>
+static int launch_invariants_on(gp_handle *h, const void *phi, const void *mu, int64_t fs, int64_t ps, int64_t yoff,
+                                double *partials, double *out, hipStream_t stream) {
+  return launch_invariants_model<N, NC, R>(h, h->model, phi, mu, fs, ps, yoff, partials, out, stream);
 }
 
 /* solute[NC] and energy of the handle's current state into h->inv_dev */
@@ -332,11 +339,13 @@ static int launch_step_p(gp_handle *h, int mode) {
         if (stats)
           hipLaunchKernelGGL((k_cbulk<N, NC, R, P2, true>), dim3(cblk + rblk), dim3(256), 0, h->stream, m, mu, codes,
                              phi_o, mu_o, codes_o, h->maps, h->partials, cstride, h->fs, h->ps, mx, my, h->cell_seg,
-                             h->march_nstrips, ct, (int)h->prev_valid, h->ilist, h->clist, lc, lnext, (int)cblk);
+                             h->march_nstrips, ct, (int)h->prev_valid, h->ilist, h->clist, lc, lnext, (int)cblk,
+                             (const int8_t *)nullptr, (float *)nullptr, (int8_t *)nullptr);
         else
           hipLaunchKernelGGL((k_cbulk<N, NC, R, P2, false>), dim3(cblk + rblk), dim3(256), 0, h->stream, m, mu, codes,
                              phi_o, mu_o, codes_o, h->maps, h->partials, cstride, h->fs, h->ps, mx, my, h->cell_seg,
-                             h->march_nstrips, ct, (int)h->prev_valid, h->ilist, h->clist, lc, lnext, (int)cblk);
+                             h->march_nstrips, ct, (int)h->prev_valid, h->ilist, h->clist, lc, lnext, (int)cblk,
+                             (const int8_t *)nullptr, (float *)nullptr, (int8_t *)nullptr);
       }
       {
         /* delta-phi of the listed candidates, full wavefronts */
@@ -511,6 +520,184 @@ static int adj_widen_any(gp_handle *h) {
   return h->pow2 ? adj_widen_t<N, NC, true>(h) : adj_widen_t<N, NC, false>(h);
 }
 
+/* ------------------------------------------------------------------------------------------------ GP_FP32C */
+
+/* h->wide[i] (FP64 copy of a decoded state), allocated on first use */
+static int c32_wide_buffer(gp_handle *h, int i) {
+  if (!h->wide[i] && hipMalloc((void **)&h->wide[i], state_elems(h) * sizeof(double)) != hipSuccess) {
+    h->wide[i] = nullptr;
+    return gp_fail(h, GP_E_OOM, "FP64 state copy does not fit in device memory");
+  }
+  return GP_OK;
+}
+
+/* decode state buffer b (phi planes, then mu planes) into dst (FP64 planes in the same layout) */
+static int c32_widen_state(gp_handle *h, int b, double *dst) {
+  const int64_t np = (int64_t)h->N * h->fs, nm = (int64_t)h->NC * h->fs;
+  hipLaunchKernelGGL(k_df_widen, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, h->stream,
+                     (const float *)h->phi[b], c32_plo(h, b), dst, np);
+  hipLaunchKernelGGL(k_c32_widen, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, h->stream,
+                     (const float *)h->mu[b], c32_mlo(h, b), dst + np, nm);
+  HIPCHK(h, hipGetLastError());
+  return GP_OK;
+}
+
+/* encode FP64 planes src into state buffer b */
+static int c32_narrow_state(gp_handle *h, const double *src, int b) {
+  const int64_t np = (int64_t)h->N * h->fs, nm = (int64_t)h->NC * h->fs;
+  hipLaunchKernelGGL(k_df_narrow, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, h->stream, src,
+                     (float *)h->phi[b], c32_plo(h, b), np);
+  hipLaunchKernelGGL(k_c32_narrow, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, h->stream, src + np,
+                     (float *)h->mu[b], c32_mlo(h, b), nm);
+  HIPCHK(h, hipGetLastError());
+  return GP_OK;
+}
+
+/* One step (or a delta-phi / smoothing request) of the compensated mode. The steps take the cell-classified
+ * kernels with the extensions; the smoothing steps and the delta-phi requests run the FP64 two-pass kernels on
+ * the decoded state (they are rare: the pre-loop smoothing, downloads of delta-phi). */
+template <int N, int NC, bool P2>
+static int launch_step_c(gp_handle *h, int mode) {
+  const int64_t mx = h->p.mesh_x, my = h->p.mesh_y, fs = h->fs;
+  const Model<double> *md = (const Model<double> *)h->model64;
+  double *dphi = (double *)h->dphi;
+  dim3 blk(kBX, kBY);
+  dim3 g1((unsigned)((my + 2 + kBX - 1) / kBX), (unsigned)((mx + 2 + kBY - 1) / kBY));
+  dim3 g2((unsigned)((my + kBX - 1) / kBX), (unsigned)((mx + kBY - 1) / kBY));
+  int rc;
+  if (mode == MODE_DPHI_CUR || mode == MODE_DPHI_PREV) {
+    const int b = mode == MODE_DPHI_CUR ? h->cur : 1 - h->cur;
+    if ((rc = c32_wide_buffer(h, 0)) || (rc = c32_widen_state(h, b, h->wide[0]))) return rc;
+    hipLaunchKernelGGL((k_phi<N, NC, double, false, P2>), g1, blk, 0, h->stream, md, (const double *)h->wide[0],
+                       (const double *)(h->wide[0] + N * fs), dphi, fs, h->ps, mx, my);
+    HIPCHK(h, hipGetLastError());
+    return GP_OK;
+  }
+  if (mode == MODE_SMOOTH) {
+    if ((rc = c32_wide_buffer(h, 0)) || (rc = c32_wide_buffer(h, 1))) return rc;
+    if ((rc = c32_widen_state(h, h->cur, h->wide[0]))) return rc;
+    HIPCHK(h, hipMemcpyAsync(h->wide[1], h->wide[0], state_elems(h) * sizeof(double), hipMemcpyDeviceToDevice,
+                             h->stream));
+    const double *wp = h->wide[0], *wm = h->wide[0] + N * fs;
+    {
+      ProfScope ps(h, GP_KERNEL_PHI);
+      hipLaunchKernelGGL((k_phi<N, NC, double, true, P2>), g1, blk, 0, h->stream, md, wp, wm, dphi, fs, h->ps, mx, my);
+    }
+    {
+      ProfScope ps(h, GP_KERNEL_MU);
+      hipLaunchKernelGGL((k_mu<N, NC, double, true, P2>), g2, blk, 0, h->stream, md, wp, wm, (const double *)dphi,
+                         h->wide[1], h->wide[1] + N * fs, h->maps, h->partials, fs, h->ps, mx, my);
+    }
+    HIPCHK(h, hipGetLastError());
+    if ((rc = c32_narrow_state(h, h->wide[1], 1 - h->cur))) return rc;
+    h->cur = 1 - h->cur;
+    h->codes_valid = false;
+    h->flags_valid = false;
+    return GP_OK;
+  }
+  if (mode != MODE_MARCH && mode != MODE_MARCH_STATS)
+    return gp_fail(h, GP_E_ARG, "the compensated FP32 mode runs the cell-classified step only");
+  const float *phi = (const float *)h->phi[h->cur], *mu = (const float *)h->mu[h->cur];
+  float *phi_o = (float *)h->phi[1 - h->cur], *mu_o = (float *)h->mu[1 - h->cur];
+  const float *plo = c32_plo(h, h->cur);
+  const int8_t *mlo = c32_mlo(h, h->cur);
+  float *plo_o = c32_plo(h, 1 - h->cur);
+  int8_t *mlo_o = c32_mlo(h, 1 - h->cur);
+  const Model<float> *m = (const Model<float> *)h->model;
+  const int8_t *codes = h->codes[h->ccur];
+  int8_t *codes_o = h->codes[1 - h->ccur];
+  const bool stats = mode == MODE_MARCH_STATS;
+  if (!h->codes_valid)
+    hipLaunchKernelGGL((k_c32_codes<N>), dim3((unsigned)((h->ry + 63) / 64), (unsigned)((h->rx + 3) / 4)), dim3(256),
+                       0, h->stream, phi, plo, h->codes[h->ccur], fs, h->ps, h->rx, h->ry);
+  const int ct = (int)h->cell_nwaves;
+  const unsigned cblk = (unsigned)((ct + 3) / 4);
+  const int64_t cstride = (int64_t)ct + 4 * kListBlocks;
+  int *lc = h->lcount + 2 * h->lsel, *lnext = h->lcount + 2 * (1 - h->lsel);
+  h->lsel = 1 - h->lsel;
+  const int64_t nring = 2 * (my + 2) + 2 * mx;
+  const unsigned rblk = (unsigned)((nring + 255) / 256);
+  {
+    ProfScope ps(h, GP_KERNEL_CBULK);
+    if (stats)
+      hipLaunchKernelGGL((k_cbulk<N, NC, float, P2, true, true>), dim3(cblk + rblk), dim3(256), 0, h->stream, m, mu,
+                         codes, phi_o, mu_o, codes_o, h->maps, h->partials, cstride, fs, h->ps, mx, my, h->cell_seg,
+                         h->march_nstrips, ct, (int)h->prev_valid, h->ilist, h->clist, lc, lnext, (int)cblk, mlo, plo_o,
+                         mlo_o);
+    else
+      hipLaunchKernelGGL((k_cbulk<N, NC, float, P2, false, true>), dim3(cblk + rblk), dim3(256), 0, h->stream, m, mu,
+                         codes, phi_o, mu_o, codes_o, h->maps, h->partials, cstride, fs, h->ps, mx, my, h->cell_seg,
+                         h->march_nstrips, ct, (int)h->prev_valid, h->ilist, h->clist, lc, lnext, (int)cblk, mlo, plo_o,
+                         mlo_o);
+  }
+  {
+    ProfScope ps(h, GP_KERNEL_DPHI);
+    hipLaunchKernelGGL((k_dlist_c<N, NC, P2>), dim3(kListBlocks), dim3(256), 0, h->stream, md, phi, plo, mu, mlo, dphi,
+                       fs, h->ps, (const int32_t *)h->clist, (const int *)lc + 1);
+  }
+  {
+    ProfScope ps(h, GP_KERNEL_CIFACE);
+    if (stats)
+      hipLaunchKernelGGL((k_ilist_c<N, NC, P2, true>), dim3(kListBlocks), dim3(256), 0, h->stream, md, phi, plo, mu,
+                         mlo, (const double *)dphi, codes, phi_o, plo_o, mu_o, mlo_o, codes_o, h->maps,
+                         h->partials + ct, cstride, fs, h->ps, mx, my, (const int32_t *)h->ilist, (const int *)lc);
+    else
+      hipLaunchKernelGGL((k_ilist_c<N, NC, P2, false>), dim3(kListBlocks), dim3(256), 0, h->stream, md, phi, plo, mu,
+                         mlo, (const double *)dphi, codes, phi_o, plo_o, mu_o, mlo_o, codes_o, h->maps,
+                         h->partials + ct, cstride, fs, h->ps, mx, my, (const int32_t *)h->ilist, (const int *)lc);
+  }
+  if (stats) {
+    ProfScope ps(h, GP_KERNEL_STATS);
+    hipLaunchKernelGGL((k_stats<N, NC>), dim3(3 * N + 3 * NC + 1), dim3(kStatThreads), 0, h->stream, h->partials,
+                       cstride, cstride, h->stats_dev);
+  }
+  HIPCHK(h, hipGetLastError());
+  h->cur = 1 - h->cur;
+  h->ccur = 1 - h->ccur;
+  h->flags_valid = false;
+  h->codes_valid = true;
+  return GP_OK;
+}
+
+template <int N, int NC>
+static int launch_step_c_any(gp_handle *h, int mode) {
+  return h->pow2 ? launch_step_c<N, NC, true>(h, mode) : launch_step_c<N, NC, false>(h, mode);
+}
+
+/* solute and energy of the current (decoded) state: FP64 kernels on its FP64 copy */
+template <int N, int NC>
+static int launch_invariants_c(gp_handle *h) {
+  int rc;
+  if ((rc = c32_wide_buffer(h, 0)) || (rc = c32_widen_state(h, h->cur, h->wide[0]))) return rc;
+  return launch_invariants_model<N, NC, double>(h, h->model64, h->wide[0], h->wide[0] + (size_t)N * h->fs, h->fs,
+                                                h->ps, kYOff, h->partials, h->inv_dev, h->stream);
+}
+/* the same of a caller's FP64 state (gp_invariants_noalloc) */
+template <int N, int NC>
+static int launch_invariants_on_c(gp_handle *h, const void *phi, const void *mu, int64_t fs, int64_t ps, int64_t yoff,
+                                  double *partials, double *out, hipStream_t stream) {
+  return launch_invariants_model<N, NC, double>(h, h->model64, phi, mu, fs, ps, yoff, partials, out, stream);
+}
+
+/* the adjoint's FP64 evaluation point of a compensated state: the decoded state and its FP64 delta-phi */
+template <int N, int NC, bool P2>
+static int adj_widen_c(gp_handle *h) {
+  Adjoint *A = h->adj;
+  int rc;
+  if ((rc = c32_widen_state(h, h->cur, A->st64))) return rc;
+  const int64_t mx = h->p.mesh_x, my = h->p.mesh_y, np = (int64_t)N * h->fs;
+  dim3 g1((unsigned)((my + 2 + kBX - 1) / kBX), (unsigned)((mx + 2 + kBY - 1) / kBY));
+  hipLaunchKernelGGL((k_phi<N, NC, double, false, P2>), g1, dim3(kBX, kBY), 0, h->stream,
+                     (const Model<double> *)A->model64, (const double *)A->st64, (const double *)(A->st64 + np),
+                     A->dphi64, h->fs, h->ps, mx, my);
+  HIPCHK(h, hipGetLastError());
+  return GP_OK;
+}
+template <int N, int NC>
+static int adj_widen_c_any(gp_handle *h) {
+  return h->pow2 ? adj_widen_c<N, NC, true>(h) : adj_widen_c<N, NC, false>(h);
+}
+
 /* the kernel set of (N, NC) in the given precision */
 template <int N, int NC>
 const GpOps *make_ops(int precision) {
@@ -518,7 +705,9 @@ const GpOps *make_ops(int precision) {
                             adj_widen_any<N, NC>, launch_invariants_on<N, NC, double>};
   static const GpOps f32 = {launch_step<N, NC, float>, launch_invariants<N, NC, float>, adj_step_any<N, NC>,
                             adj_widen_any<N, NC>, launch_invariants_on<N, NC, float>};
-  return precision == GP_FP64 ? &f64 : &f32;
+  static const GpOps c32 = {launch_step_c_any<N, NC>, launch_invariants_c<N, NC>, adj_step_any<N, NC>,
+                            adj_widen_c_any<N, NC>, launch_invariants_on_c<N, NC>};
+  return precision == GP_FP64 ? &f64 : precision == GP_FP32C ? &c32 : &f32;
 }
 
 }  // namespace gpk

@@ -11,7 +11,8 @@
  * SAVET (plus the DATA/shift.dat line) and the progress block every TRACK_PROGRESS. The loop body
  * (GPs/microsim_gp.c:103-106) is gp_step; steps between two output events are issued as one batch.
  *
- * Environment: GP_PRECISION=fp32 selects the FP32 production mode (default fp64, bit-compatible).
+ * Environment: GP_PRECISION=fp32c selects the compensated FP32 mode, fp32 the plain FP32 mode (default fp64,
+ * bit-compatible).
  * Differences from the reference, by construction: the progress statistics are taken over interior cells
  * (the reference's sums also visit halo cells holding stale values), and unsupported inputs (TEMPGRADY,
  * SHIFT, noise, Dirichlet/free sides) stop with an error instead of running.
@@ -66,6 +67,7 @@ int main(int argc, char *argv[]) {
   int precision = GP_FP64;
   const char *pe = getenv("GP_PRECISION");
   if (pe && (strcmp(pe, "fp32") == 0 || strcmp(pe, "FP32") == 0)) precision = GP_FP32;
+  if (pe && (strcmp(pe, "fp32c") == 0 || strcmp(pe, "FP32C") == 0)) precision = GP_FP32C;
   gp_handle *h = NULL;
   if (gp_create(p, precision, &h)) die("gp_create", gp_last_error(NULL));
   if (gp_upload(h, phi, mu)) die("gp_upload", gp_last_error(h));

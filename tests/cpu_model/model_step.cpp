@@ -12,6 +12,7 @@
 // model_vjp runs the adjoint of the same steps with the reverse-mode per-cell functions of
 // microsim_amd/csrc/gp_reverse.h (the ones the device adjoint runs), on every cell and without the device's bulk
 // shortcuts: <J^T w, v> from it must equal <w, J v> from model_jvp.
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -228,6 +229,122 @@ static int run_vjp(const gp_params *p, const double *phi0, const double *mu0, co
   return 0;
 }
 
+/* The compensated FP32 step (GP_FP32C) in the device's decomposition: phi as pairs of FP32 values (df_encode /
+ * df_decode of gp_model.h), mu as FP32 parts plus 8-bit extensions (c32_encode / c32_decode); phase codes from the
+ * decoded phi; delta-phi in FP64 from the
+ * decoded state at the candidates of [2, mesh+3]^2 (5-point codes not uniform) and 0 elsewhere; bulk cells (radius-2
+ * code diamond purely phase b) take the FP32 closed form Kbulk[b] . lap(mu) from the FP32 parts, added to the
+ * decoded mu and re-encoded, with phi = e_b; every other interior cell takes jat_x / jat_y / mu_update in FP64 from
+ * the decoded state, encoded on store; halos are copies of their source cells (parts and extensions alike).
+ * (phi, mu) in: the FP64 state (boundary maps applied first, as gp_upload does); out: the decoded final state. */
+template <typename T>
+static void bc_planes(const gp_params *p, T *v, int ncomp, int f) {
+  const int64_t rx = p->mesh_x + 6, ry = p->mesh_y + 6, n = rx * ry;
+  const int64_t hx[6] = {0, 1, 2, rx - 3, rx - 2, rx - 1}, hy[6] = {0, 1, 2, ry - 3, ry - 2, ry - 1};
+  for (int c = 0; c < ncomp; c++) {
+    T *w = v + c * n;
+    for (int s = 0; s < 6; s++)
+      for (int64_t y = 0; y < ry; y++) w[hx[s] * ry + y] = w[(int64_t)p->halo_src_x[f][s] * ry + y];
+    for (int s = 0; s < 6; s++)
+      for (int64_t x = 0; x < rx; x++) w[x * ry + hy[s]] = w[x * ry + p->halo_src_y[f][s]];
+  }
+}
+
+template <int N, int NC, bool P2>
+static int run_c32(const gp_params *p, double *phi, double *mu, int64_t nsteps) {
+  Model<double> md;
+  Model<float> mf;
+  if (make_model(p, &md) || make_model(p, &mf)) return -1;
+  if (P2 && !md.pow2) return -3;
+  const int64_t mx = p->mesh_x, my = p->mesh_y, rx = mx + 6, ry = my + 6, n = rx * ry;
+  apply_bc(p, phi, mu, N, NC);
+  std::vector<float> ph(N * n), pq(N * n), mh(NC * n);
+  std::vector<int8_t> mq(NC * n);
+  for (int64_t i = 0; i < N * n; i++) df_encode(phi[i], ph[i], pq[i]);
+  for (int64_t i = 0; i < NC * n; i++) c32_encode(mu[i], mh[i], mq[i]);
+  std::vector<double> P(N * n), M(NC * n), D(N * n);
+  std::vector<int8_t> code(n);
+  for (int64_t t = 0; t < nsteps; t++) {
+    for (int64_t i = 0; i < N * n; i++) P[i] = df_decode(ph[i], pq[i]);
+    for (int64_t i = 0; i < NC * n; i++) M[i] = c32_decode(mh[i], mq[i]);
+    for (int64_t i = 0; i < n; i++) {
+      double v[N];
+      for (int a = 0; a < N; a++) v[a] = P[a * n + i];
+      int cd = -1, nz = 0;
+      for (int a = 0; a < N; a++) {
+        if (v[a] == 1.0) cd = a;
+        nz += v[a] != 0.0;
+      }
+      code[i] = (int8_t)(nz == 1 ? cd : -1);
+    }
+    std::fill(D.begin(), D.end(), 0.0);
+    View<double, P2> vw{P.data(), M.data(), D.data(), n, ry, 0};
+    for (int64_t x = 2; x <= mx + 3; x++)
+      for (int64_t y = 2; y <= my + 3; y++) {
+        const int64_t c = x * ry + y;
+        const int8_t k0 = code[c];
+        const bool cand = k0 < 0 || code[c - ry] != k0 || code[c + ry] != k0 || code[c - 1] != k0 || code[c + 1] != k0;
+        if (!cand) continue;
+        double d[N];
+        phi_delta<N, NC>(md, vw.shift((int)x, (int)y), false, d);
+        for (int a = 0; a < N; a++) D[a * n + c] = d[a];
+      }
+    std::vector<float> nph(ph), npq(pq), nmh(mh);
+    std::vector<int8_t> nmq(mq);
+    for (int64_t x = 3; x <= mx + 2; x++)
+      for (int64_t y = 3; y <= my + 2; y++) {
+        const int64_t c = x * ry + y;
+        int b = code[c];
+        if (b >= 0) {
+          const int o[12][2] = {{1, 0}, {-1, 0}, {2, 0}, {-2, 0}, {0, 1}, {0, -1},
+                                {0, 2}, {0, -2}, {1, 1}, {1, -1}, {-1, 1}, {-1, -1}};
+          for (auto &e : o)
+            if (code[c + e[0] * ry + e[1]] != b) b = -1;
+        }
+        if (b >= 0) {
+          float lap[NC];
+          for (int l = 0; l < NC; l++) {
+            const float *w = mh.data() + l * n;
+            const float gx = (w[c + ry] - w[c]) - (w[c] - w[c - ry]);
+            const float gy = (w[c + 1] - w[c]) - (w[c] - w[c - 1]);
+            lap[l] = std::fmaf(gx, mf.lap_x, gy * mf.lap_y);
+          }
+          for (int k = 0; k < NC; k++) {
+            float s = 0.f;
+            for (int l = 0; l < NC; l++) s = l == 0 ? mf.Kbulk[b][k][l] * lap[0] : std::fmaf(mf.Kbulk[b][k][l], lap[l], s);
+            c32_encode(c32_decode(mh[k * n + c], mq[k * n + c]) + (double)s, nmh[k * n + c], nmq[k * n + c]);
+          }
+          for (int a = 0; a < N; a++) {
+            nph[a * n + c] = a == b ? 1.0f : 0.0f;
+            npq[a * n + c] = 0.0f;
+          }
+          continue;
+        }
+        auto cv = vw.shift((int)x, (int)y);
+        double jx0[NC], jxb[NC], jy0[NC], jyl[NC], d[N], pn[N], mn[NC], dm[NC];
+        jat_x<N, NC>(md, cv, 0, 0, jx0);
+        jat_x<N, NC>(md, cv, -1, 0, jxb);
+        jat_y<N, NC>(md, cv, 0, 0, y, jy0);
+        jat_y<N, NC>(md, cv, 0, -1, y - 1, jyl);
+        for (int a = 0; a < N; a++) d[a] = D[a * n + c];
+        mu_update<N, NC>(md, cv, false, d, jx0, jxb, jy0, jyl, pn, mn, dm);
+        for (int a = 0; a < N; a++) df_encode(pn[a], nph[a * n + c], npq[a * n + c]);
+        for (int k = 0; k < NC; k++) c32_encode(mn[k], nmh[k * n + c], nmq[k * n + c]);
+      }
+    bc_planes(p, nph.data(), N, 0);
+    bc_planes(p, npq.data(), N, 0);
+    bc_planes(p, nmh.data(), NC, 1);
+    bc_planes(p, nmq.data(), NC, 1);
+    ph.swap(nph);
+    pq.swap(npq);
+    mh.swap(nmh);
+    mq.swap(nmq);
+  }
+  for (int64_t i = 0; i < N * n; i++) phi[i] = df_decode(ph[i], pq[i]);
+  for (int64_t i = 0; i < NC * n; i++) mu[i] = c32_decode(mh[i], mq[i]);
+  return 0;
+}
+
 #define GP_CASES(CASE) CASE(2, 1) CASE(3, 2) CASE(3, 1) CASE(2, 2) CASE(4, 2) CASE(4, 1) CASE(3, 3)
 
 /* pow2: 1 forces the power-of-two spacing path (error -3 when dx, dy are not powers of two), 0 the division path */
@@ -344,4 +461,30 @@ extern "C" int cell_check(const gp_params *p, const double *phi, const double *m
   GP_CASES(CASE)
 #undef CASE
   return -2;
+}
+
+/* nsteps compensated-FP32 steps (run_c32) from the FP64 state (phi, mu), replaced by the decoded final state */
+extern "C" int model_step_c32(const gp_params *p, double *phi, double *mu, int64_t nsteps, int pow2) {
+  const int N = p->nphases, NC = p->ncomp - 1;
+#define CASE(a, b) \
+  if (N == a && NC == b) return pow2 ? run_c32<a, b, true>(p, phi, mu, nsteps) : run_c32<a, b, false>(p, phi, mu, nsteps);
+  GP_CASES(CASE)
+#undef CASE
+  return -2;
+}
+
+/* c32_encode / c32_decode of n values (tests of the encoding) */
+extern "C" void c32_roundtrip(const double *v, int64_t n, float *hi, int8_t *q, double *out) {
+  for (int64_t i = 0; i < n; i++) {
+    c32_encode(v[i], hi[i], q[i]);
+    out[i] = c32_decode(hi[i], q[i]);
+  }
+}
+
+/* df_encode / df_decode of n values (tests of the two-part phi form) */
+extern "C" void df_roundtrip(const double *v, int64_t n, float *hi, float *lo, double *out) {
+  for (int64_t i = 0; i < n; i++) {
+    df_encode(v[i], hi[i], lo[i]);
+    out[i] = df_decode(hi[i], lo[i]);
+  }
 }

@@ -63,6 +63,11 @@ def model_lib():
                                   _abi._dp, _abi._dp, C.c_int64, C.c_int]
         lib.model_vjp.argtypes = [C.POINTER(_abi.GPParams), _abi._dp, _abi._dp, _abi._dp, _abi._dp, _abi._dp,
                                   _abi._dp, _abi._dp, C.c_int64, C.c_int]
+        lib.model_step_c32.argtypes = [C.POINTER(_abi.GPParams), _abi._dp, _abi._dp, C.c_int64, C.c_int]
+        lib.c32_roundtrip.argtypes = [_abi._dp, C.c_int64, C.c_void_p, C.c_void_p, _abi._dp]
+        lib.c32_roundtrip.restype = None
+        lib.df_roundtrip.argtypes = [_abi._dp, C.c_int64, C.c_void_p, C.c_void_p, _abi._dp]
+        lib.df_roundtrip.restype = None
         _model = lib
     return _model
 
@@ -80,6 +85,22 @@ def model_vjp(params, phi, mu, wphi, wmu, nsteps):
                                _abi.dptr(m0), _abi.dptr(th), nsteps, 0)
     assert rc == 0
     return g0, m0, th
+
+
+def pow2_spacing(params) -> bool:
+    """dx and dy are powers of two (the kernels then multiply by the reciprocal spacings, bit for bit the same)."""
+    import math
+    return math.frexp(params.dx)[0] == 0.5 and math.frexp(params.dy)[0] == 0.5
+
+
+def model_step_c32(params, phi, mu, nsteps):
+    """(phi_T, mu_T) after nsteps steps of the compensated FP32 mode in a host build of the device arithmetic
+    (tests/cpu_model/model_step.cpp run_c32), decoded to FP64."""
+    phi = np.ascontiguousarray(phi, dtype=np.float64).copy()
+    mu = np.ascontiguousarray(mu, dtype=np.float64).copy()
+    rc = model_lib().model_step_c32(C.byref(params), _abi.dptr(phi), _abi.dptr(mu), nsteps, int(pow2_spacing(params)))
+    assert rc == 0, rc
+    return phi, mu
 
 
 def model_jvp(params, phi, mu, vphi, vmu, vtheta, nsteps):
