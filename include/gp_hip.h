@@ -106,6 +106,27 @@ typedef struct gp_params {
   double C[GP_MAXP];                       /* C(T) */
   double Beq[GP_MAXP * GP_MAXNC];          /* B(Teq) */
   double dBbdT[GP_MAXP * GP_MAXNC];        /* dB/dT */
+  /* Non-isothermal runs and the moving frame. ISOTHERMAL = 0 turns on TEMPGRADY in the serial driver
+   * (GPs/functions/reading_input_parameters.h:183-188): T depends on y only, is set from the Tempgrady key after
+   * every step (GPs/microsim_gp.c:108-113, GPs/functions/Temperature_gradient.h:4-29) and enters c(mu), Mu(c)
+   * through Beq + dBbdT (T - Teq) and the free energy through B(T), C(T) (GPs/functions/functionF_01.h:5-104).
+   * Shift = 1 moves the frame along y every 100 steps (GPs/microsim_gp.c:115-126, GPs/functions/shift.h). */
+  int32_t tempgrady;                       /* 1: ISOTHERMAL = 0 (temperature gradient along y) */
+  int32_t shift;                           /* Shift */
+  int64_t shiftj;                          /* Shiftj: the interface row the frame keeps */
+  double tg_base_temp, tg_deltaT, tg_distance, tg_velocity; /* Tempgrady = {base_temp, DeltaT, Distance, -, velocity} */
+  double tg_offset;                        /* Tempgrady gradient_OFFSET, after the restart adjustment
+                                              (GPs/microsim_gp.c:68-75) */
+  int32_t halo_src_T_y[6];                 /* source y of the halo columns for T (boundary[.][2]); a column that
+                                              copies nothing maps to itself */
+  int32_t reserved1[2];
+  double ceq_sol[GP_MAXP * GP_MAXNC];      /* ceq[a][a][k]: B(T), C(T) of phase a < liquid */
+  double ceq_liq[GP_MAXP * GP_MAXNC];      /* ceq[a][L][k] */
+  double delta_c[GP_MAXP * GP_MAXNC];      /* DELTA_C[a][k] */
+  double delta_T_sol[GP_MAXP];             /* DELTA_T[a][a] */
+  double delta_T_liq[GP_MAXP];             /* DELTA_T[a][L] */
+  double dcbdT_phase[GP_MAXP * GP_MAXNC];  /* dcbdT_phase of the dc/dT source (the serial driver leaves it 0) */
+  double mu_fill[GP_MAXNC];                /* Mu(cfill[L][L], Tfill, L): mu of the liquid rows a shift adds */
 } gp_params;
 
 /* Diagnostics. max/min are cumulative from the last gp_upload with the reference's initial values
@@ -148,7 +169,7 @@ int gp_upload_fields(gp_handle *h, const gp_cell *cells, int64_t ncells);
  * interior update reads (x,y in [2, mesh+3]) and 0 elsewhere. */
 int gp_download(gp_handle *h, double *phi, double *mu, double *dphi);
 
-/* Same into the reference's AoS array (phia, compi, deltaphi of every cell). */
+/* Same into the reference's AoS array (phia, compi, deltaphi and temperature of every cell). */
 int gp_download_fields(gp_handle *h, gp_cell *cells, int64_t ncells);
 
 /* nsteps pre-conditioning steps (the NSMOOTH-1 loop): no driving force, no anti-trapping current. */
@@ -163,6 +184,13 @@ int gp_step(gp_handle *h, int64_t nsteps, gp_stats *stats);
 
 /* Diagnostics of the current state without stepping (max/min, solute, energy; rel_change of the last step). */
 int gp_get_stats(gp_handle *h, gp_stats *stats);
+
+/* Frame of a non-isothermal / moving-frame run: shift_offset (may be NULL) receives the cells the frame has moved
+ * since gp_upload (shift_OFFSET of GPs/microsim_gp.c:122; DATA/shift.dat adds the restart's shift_position),
+ * T_rows (may be NULL) the temperature of every allocated y row (rows_y values; the reference's
+ * gridinfo[].temperature, which depends on y only) as it stands after the last step. Isothermal runs report
+ * offset 0 and T everywhere. */
+int gp_get_frame(gp_handle *h, int64_t *shift_offset, double *T_rows);
 
 /* Solute and energy (the gp_stats fields solute[NC] and energy, as gp_get_stats computes them) of a caller's
  * device-resident state, with no device allocation and no synchronisation.

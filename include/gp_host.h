@@ -69,7 +69,8 @@ typedef struct gp_input {
  * ("Unrecongized parameter : ..."), unless quiet. Returns GP_OK or GP_E_ARG/GP_E_IO with err filled. */
 int gp_read_input(const char *input_path, gp_input *in, int quiet, char *err, size_t errlen);
 
-/* Check the parameters are in the supported set (2-D, isothermal, F=1, Neumann/periodic, no noise, no SHIFT). */
+/* Check the parameters are in the supported set (2-D, F=1, Neumann/periodic sides, no noise; ISOTHERMAL = 0 needs
+ * Tempgrady; Shiftj inside the allocated rows). */
 int gp_validate_input(const gp_input *in, char *err, size_t errlen);
 
 /* init_propertymatrices(T) on the host: cmu, B(T), C(T), Beq, dBbdT into in->p. */
@@ -87,9 +88,17 @@ int gp_fill_domain(const gp_input *in, const char *filling_path, double *phi, do
 /* Copy halos from the interior with the boundary maps (apply_boundary_conditions(0)). */
 void gp_apply_bc_host(const gp_params *p, double *phi, double *mu);
 
-/* DATA/<out>_<t>.vtk in the serial layout; binary unless in->ascii. */
-int gp_write_vtk(const gp_input *in, const char *path, const double *phi, const double *mu);
+/* DATA/<out>_<t>.vtk in the serial layout; binary unless in->ascii. T_rows (the temperature of every allocated
+ * row, gp_get_frame or gp_temperature_rows) is read when ISOTHERMAL = 0: it fills the Temperature block and the
+ * compositions; it may be NULL otherwise. The reader takes phi and mu; a Temperature block is left unread, since
+ * the non-isothermal loop sets T from the gradient before the first step. */
+int gp_write_vtk(const gp_input *in, const char *path, const double *phi, const double *mu, const double *T_rows);
 int gp_read_vtk(const gp_input *in, const char *path, double *phi, double *mu);
+
+/* Temperature of every allocated row (rows_y values) as the serial driver sets it: before the loop (t = 0,
+ * with the boundary copies of T) or after step t, with the moving frame shifted by shift_offset rows
+ * (GPs/microsim_gp.c:77-84, 108-111). Isothermal runs: T everywhere. */
+void gp_temperature_rows(const gp_params *p, int64_t t, int64_t shift_offset, double *T_rows);
 
 /* <stem>.out and <stem>.bd echo files. */
 int gp_write_echo(const gp_input *in, const char *input_path);

@@ -55,6 +55,13 @@ class GPParams(C.Structure):
         ("diffusivity", C.c_double * (P * NCM * NCM)), ("A", C.c_double * (P * NCM * NCM)),
         ("cmu", C.c_double * (P * NCM * NCM)), ("B", C.c_double * (P * NCM)), ("C", C.c_double * P),
         ("Beq", C.c_double * (P * NCM)), ("dBbdT", C.c_double * (P * NCM)),
+        ("tempgrady", C.c_int32), ("shift", C.c_int32), ("shiftj", C.c_int64),
+        ("tg_base_temp", C.c_double), ("tg_deltaT", C.c_double), ("tg_distance", C.c_double),
+        ("tg_velocity", C.c_double), ("tg_offset", C.c_double),
+        ("halo_src_T_y", C.c_int32 * 6), ("reserved1", C.c_int32 * 2),
+        ("ceq_sol", C.c_double * (P * NCM)), ("ceq_liq", C.c_double * (P * NCM)), ("delta_c", C.c_double * (P * NCM)),
+        ("delta_T_sol", C.c_double * P), ("delta_T_liq", C.c_double * P),
+        ("dcbdT_phase", C.c_double * (P * NCM)), ("mu_fill", C.c_double * NCM),
     ]
 
 
@@ -115,7 +122,9 @@ def _declare_host(lib):
     lib.gp_fill_domain.argtypes = [C.POINTER(GPInput), C.c_char_p, _dp, _dp, _dp, C.c_int, C.c_char_p, C.c_size_t]
     lib.gp_apply_bc_host.argtypes = [C.POINTER(GPParams), _dp, _dp]
     lib.gp_apply_bc_host.restype = None
-    lib.gp_write_vtk.argtypes = [C.POINTER(GPInput), C.c_char_p, _dp, _dp]
+    lib.gp_write_vtk.argtypes = [C.POINTER(GPInput), C.c_char_p, _dp, _dp, _dp]
+    lib.gp_temperature_rows.argtypes = [C.POINTER(GPParams), C.c_int64, C.c_int64, _dp]
+    lib.gp_temperature_rows.restype = None
     lib.gp_read_vtk.argtypes = [C.POINTER(GPInput), C.c_char_p, _dp, _dp]
     lib.gp_write_echo.argtypes = [C.POINTER(GPInput), C.c_char_p]
     lib.gp_composition.argtypes = [C.POINTER(GPParams), _dp, _dp, C.c_int]
@@ -153,6 +162,7 @@ def _declare_hip(lib):
     lib.gp_adjoint_forward.argtypes = [h, C.c_int64, C.c_int64]
     lib.gp_adjoint_backward.argtypes = [h, _dp, _dp, _dp, _dp, _dp]
     lib.gp_invariants_noalloc.argtypes = [h, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.gp_get_frame.argtypes = [h, C.POINTER(C.c_int64), _dp]
 
 
 _host = None

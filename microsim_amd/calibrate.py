@@ -38,7 +38,8 @@ def _case(mesh: int, seeds: int):
     return host.load_case(os.path.join(d, "Input.in"), os.path.join(d, "Filling.in"))
 
 
-def main(argv=None) -> int:
+def main(argv=None, summary_out: dict = None) -> int:
+    """Run the calibration; the summary is printed and, when summary_out is given, stored in it."""
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--mesh", type=int, default=2048)
     ap.add_argument("--seeds", type=int, default=256)
@@ -102,7 +103,7 @@ def main(argv=None) -> int:
         del loss, phiT
     fw = np.mean([h["forward_s"] for h in hist[:-1]]) if len(hist) > 1 else float("nan")
     bw = np.mean([h["backward_s"] for h in hist[:-1]]) if len(hist) > 1 else float("nan")
-    print(json.dumps({"summary": {
+    summary = {
         "workload": "ternary 3-phase %dx%d, %d seeds, %d steps, checkpoints every %d, %s" % (
             a.mesh, a.mesh, a.seeds, a.steps, every, a.precision),
         "parameters": list(SELECT), "start_offset": a.offset, "iterations": a.iters, "lr": a.lr,
@@ -113,7 +114,10 @@ def main(argv=None) -> int:
         "mean_forward_s": round(float(fw), 3), "mean_backward_s": round(float(bw), 3),
         "backward_over_forward_wall": round(float(bw / fw), 2) if fw > 0 else None,
         "target_forward_s": round(t_target, 3),
-        "adjoint_state_bytes": int(adj_bytes)}}), flush=True)
+        "adjoint_state_bytes": int(adj_bytes)}
+    print(json.dumps({"summary": summary}), flush=True)
+    if summary_out is not None:
+        summary_out.update(summary)
     return 0
 
 

@@ -89,6 +89,8 @@ class GPInput:
     writecomposition: int = 0
     shift: int = 0
     shiftj: int = 30
+    # ISOTHERMAL = 0 with Tempgrady = {base_temp, DeltaT, Distance, gradient_OFFSET, velocity}; None: isothermal
+    tempgrady: Optional[Tuple[float, float, float, float, float]] = None
     extra: List[Tuple[str, str]] = field(default_factory=list)
 
     def text(self) -> str:
@@ -125,7 +127,11 @@ class GPInput:
             add("BOUNDARY", _tuple([side] + list(b) + [0, 0]))
         for side in ("phi", "mu", "c", "T"):
             add("BOUNDARY_VALUE", _tuple([side, 0, 0, 0, 0, 0, 0]))
-        add("ISOTHERMAL", 1)
+        if self.tempgrady is None:
+            add("ISOTHERMAL", 1)
+        else:
+            add("ISOTHERMAL", 0)
+            add("Tempgrady", _tuple(self.tempgrady))
         solve = self.solve
         if solve == "auto":
             solve = {2: "BINARY", 3: "TERNARY"}.get(K, "LU")
@@ -288,6 +294,36 @@ def write_vtk(path: str, phi: np.ndarray, mu: np.ndarray, phase_names: Sequence[
 def _le(v: float) -> str:
     """C's ``%le`` formatting."""
     return "%e" % v
+
+
+def read_vtk_fields(path: str, mesh_x: int, mesh_y: int) -> Dict[str, np.ndarray]:
+    """Every SCALARS block of a binary or ASCII VTK written by the serial driver, by name, as [MX, MY] arrays."""
+    with open(path, "rb") as f:
+        data = f.read()
+    binary = data.split(b"\n")[2].strip() == b"BINARY"
+    n = mesh_x * mesh_y
+    out: Dict[str, np.ndarray] = {}
+    pos = data.index(b"SCALARS")
+    while True:
+        e = data.index(b"\n", pos)
+        name = data[pos:e].split()[1].decode()
+        pos = data.index(b"\n", e + 1) + 1   # past LOOKUP_TABLE
+        if binary:
+            out[name] = np.frombuffer(data, dtype=">f8", count=n, offset=pos).astype(np.float64).reshape(mesh_x, mesh_y)
+            pos += 8 * n
+        else:
+            vals = []
+            while len(vals) < n:
+                e = data.index(b"\n", pos)
+                t = data[pos:e].strip()
+                pos = e + 1
+                if t:
+                    vals.append(float(t))
+            out[name] = np.array(vals).reshape(mesh_x, mesh_y)
+        nxt = data.find(b"SCALARS", pos)
+        if nxt < 0:
+            return out
+        pos = nxt
 
 
 def read_vtk(path: str, nphases: int, ncomp: int, mesh_x: int, mesh_y: int, extra_fields: int = 0

@@ -253,6 +253,94 @@ static int adj_fold(gp_handle *h) {
   return GP_OK;
 }
 
+/* ------------------------------------------------------------------------------------------------ frame */
+
+/* temperature of allocated row 0 and the gradient per row, as the serial driver forms them before the loop (t = 0,
+ * GPs/microsim_gp.c:77-80) and after step t (:108-111) */
+static double frame_bottom(const gp_handle *h, int64_t t, double *grad) {
+  const gp_params *p = &h->p;
+  const double G = (p->tg_deltaT) * p->dy / (p->tg_distance);
+  double base;
+  if (t == 0) base = (p->tg_offset / p->dy) - (double)h->shift_off;
+  else base = (p->tg_offset / p->dy) - (double)h->shift_off + ((p->tg_velocity / p->dy) * ((double)t * p->dt));
+  *grad = G;
+  return p->tg_base_temp - base * G;
+}
+
+/* T of every allocated row before the first step: the t = 0 gradient, then the boundary copies of T that run
+ * before the loop (GPs/microsim_gp.c:81-84) */
+static int frame_reset(gp_handle *h) {
+  h->shift_off = 0;
+  if (!h->p.tempgrady) return GP_OK;
+  double G;
+  const double tb = frame_bottom(h, 0, &G);
+  std::vector<double> T((size_t)h->ry);
+  for (int64_t y = 0; y < h->ry; y++) T[y] = tb + (double)y * G;
+  for (int k = 0; k < 6; k++) T[h->tp.hy[k]] = T[h->tp.srcT[k]];
+  HIPCHK(h, hipMemcpyAsync(h->trow, T.data(), T.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return GP_OK;
+}
+
+/* after step t: the new temperature gradient, and every 100 steps the moving frame */
+static int frame_after_step(gp_handle *h, int64_t t) {
+  if (h->p.tempgrady) {
+    double G;
+    const double tb = frame_bottom(h, t, &G);
+    hipLaunchKernelGGL(k_trow_set, dim3((unsigned)((h->ry + 255) / 256)), dim3(256), 0, h->stream, h->trow, h->ry, tb,
+                       G);
+    HIPCHK(h, hipGetLastError());
+  }
+  if (!h->p.shift || t % 100 != 0) return GP_OK;
+  const int N = h->N, NC = h->NC, L = N - 1;
+  const size_t es = esize(h);
+  HIPCHK(h, hipMemsetAsync(h->front, 0, sizeof(int), h->stream));
+  const dim3 gf((unsigned)((h->ry - 1 + 63) / 64), (unsigned)((h->p.mesh_x + 3) / 4));
+  if (h->precision == GP_FP64)
+    hipLaunchKernelGGL((k_front<double>), gf, dim3(256), 0, h->stream,
+                       (const double *)h->phi[h->cur] + (size_t)L * h->fs, h->front, h->ps, h->p.mesh_x, h->ry);
+  else
+    hipLaunchKernelGGL((k_front<float>), gf, dim3(256), 0, h->stream,
+                       (const float *)h->phi[h->cur] + (size_t)L * h->fs, h->front, h->ps, h->p.mesh_x, h->ry);
+  HIPCHK(h, hipGetLastError());
+  int pos = 0;
+  HIPCHK(h, hipMemcpyAsync(&pos, h->front, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (pos <= h->p.shiftj) return GP_OK;
+  const int64_t sh = pos - h->p.shiftj;
+  FillValues fp = {}, fm = {};
+  fp.v[L] = 1.0;
+  for (int k = 0; k < NC; k++) fm.v[k] = h->p.mu_fill[k];
+  const dim3 ga((unsigned)((h->ry + 63) / 64), (unsigned)((h->rx + 3) / 4));
+  if (es == sizeof(double)) {
+    hipLaunchKernelGGL((k_shift<double>), ga, dim3(256), 0, h->stream, (const double *)h->phi[h->cur],
+                       (double *)h->phi[1 - h->cur], N, fp, sh, h->fs, h->ps, h->rx, h->ry);
+    hipLaunchKernelGGL((k_shift<double>), ga, dim3(256), 0, h->stream, (const double *)h->mu[h->cur],
+                       (double *)h->mu[1 - h->cur], NC, fm, sh, h->fs, h->ps, h->rx, h->ry);
+  } else {
+    hipLaunchKernelGGL((k_shift<float>), ga, dim3(256), 0, h->stream, (const float *)h->phi[h->cur],
+                       (float *)h->phi[1 - h->cur], N, fp, sh, h->fs, h->ps, h->rx, h->ry);
+    hipLaunchKernelGGL((k_shift<float>), ga, dim3(256), 0, h->stream, (const float *)h->mu[h->cur],
+                       (float *)h->mu[1 - h->cur], NC, fm, sh, h->fs, h->ps, h->rx, h->ry);
+  }
+  HIPCHK(h, hipGetLastError());
+  h->cur = 1 - h->cur;
+  if (h->p.tempgrady) {
+    /* T moves with the rows; the top rows keep their own */
+    std::vector<double> T((size_t)h->ry);
+    HIPCHK(h, hipMemcpyAsync(T.data(), h->trow, T.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (int64_t y = 0; y + sh <= h->ry - 1; y++) T[y] = T[y + sh];
+    HIPCHK(h, hipMemcpyAsync(h->trow, T.data(), T.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+  }
+  h->shift_off += sh;
+  h->codes_valid = false;
+  h->flags_valid = false;
+  h->prev_valid = false;
+  return GP_OK;
+}
+
 /* ------------------------------------------------------------------------------------------------ C ABI */
 
 extern "C" {
@@ -318,6 +406,11 @@ int gp_create(const gp_params *params, int precision, gp_handle **out) {
   /* the cell lists and k_cbulk's row offsets are 32-bit: grids beyond that bound (cells_fit, about 46000^2 cells)
    * take the tiled march, which gives the same results */
   h->use_cells = h->use_march && !(path && strcmp(path, "march") == 0) && cells_fit(h->rx, h->ps);
+  if ((params->tempgrady || params->shift) && precision == GP_FP32C) {
+    delete h;
+    return gp_fail(nullptr, GP_E_ARG, "non-isothermal runs and the moving frame run in GP_FP64 or GP_FP32");
+  }
+  if (params->tempgrady) h->use_march = h->use_cells = false; /* the two-pass kernels carry the row temperatures */
   if (precision == GP_FP32C && !h->use_cells) {
     delete h;
     return gp_fail(nullptr, GP_E_ARG,
@@ -335,6 +428,12 @@ int gp_create(const gp_params *params, int precision, gp_handle **out) {
   ok = ok && hipMalloc(&h->phi[0], pn) == hipSuccess && hipMalloc(&h->phi[1], pn) == hipSuccess;
   ok = ok && hipMalloc(&h->mu[0], mn) == hipSuccess && hipMalloc(&h->mu[1], mn) == hipSuccess;
   ok = ok && hipMalloc(&h->dphi, (size_t)h->N * h->fs * dphi_esize(h)) == hipSuccess;
+  if (params->tempgrady) {
+    const size_t rt = precision == GP_FP64 ? sizeof(RowThermo<double>) : sizeof(RowThermo<float>);
+    ok = ok && hipMalloc((void **)&h->trow, (size_t)h->ry * sizeof(double)) == hipSuccess &&
+         hipMalloc(&h->thp, (size_t)h->ry * rt) == hipSuccess && hipMalloc(&h->thc, (size_t)h->ry * rt) == hipSuccess;
+  }
+  if (params->shift) ok = ok && hipMalloc((void **)&h->front, sizeof(int)) == hipSuccess;
   if (precision == GP_FP32C)
     ok = ok && hipMalloc((void **)&h->lo[0], c32_lo_bytes(h)) == hipSuccess &&
          hipMalloc((void **)&h->lo[1], c32_lo_bytes(h)) == hipSuccess &&
@@ -386,6 +485,29 @@ int gp_create(const gp_params *params, int precision, gp_handle **out) {
     gp_destroy(h);
     return gp_fail(nullptr, GP_E_HIP, "device initialisation failed");
   }
+  if (params->tempgrady) {
+    /* the row coefficients' parameters (gp_frame.h k_thermo) */
+    ThermoParams &tp = h->tp;
+    tp.N = h->N;
+    tp.NC = h->NC;
+    tp.Teq = params->Teq;
+    for (int k = 0; k < 6; k++) {
+      tp.hy[k] = h->maps.hy[k];
+      tp.srcT[k] = params->halo_src_T_y[k];
+    }
+    for (int a = 0; a < GP_MAXP; a++) {
+      for (int i = 0; i < GP_MAXNC; i++) {
+        tp.Beq[a][i] = params->Beq[a * GP_MAXNC + i];
+        tp.dBbdT[a][i] = params->dBbdT[a * GP_MAXNC + i];
+        tp.ceq_sol[a][i] = params->ceq_sol[a * GP_MAXNC + i];
+        tp.ceq_liq[a][i] = params->ceq_liq[a * GP_MAXNC + i];
+        tp.delta_c[a][i] = params->delta_c[a * GP_MAXNC + i];
+        for (int j = 0; j < GP_MAXNC; j++) tp.A[a][i][j] = params->A[(a * GP_MAXNC + i) * GP_MAXNC + j];
+      }
+      tp.delta_T_sol[a] = params->delta_T_sol[a];
+      tp.delta_T_liq[a] = params->delta_T_liq[a];
+    }
+  }
   double s[kNStatMax] = {0};
   stats_reset_host(h, s);
   if (hipMemcpy(h->stats_dev, s, sizeof s, hipMemcpyHostToDevice) != hipSuccess) {
@@ -408,7 +530,8 @@ void gp_destroy(gp_handle *h) {
   delete h->adj;
   void *bufs[] = {h->phi[0], h->phi[1], h->mu[0],    h->mu[1],    h->dphi,   h->partials, h->stats_dev,
                   h->model,  h->flags[0], h->flags[1], h->inv_dev, h->codes[0], h->codes[1], h->clist,
-                  h->ilist,  h->lcount, h->lo[0],    h->lo[1],    h->model64, h->wide[0],  h->wide[1]};
+                  h->ilist,  h->lcount, h->lo[0],    h->lo[1],    h->model64, h->wide[0],  h->wide[1],
+                  h->trow,   h->thp,    h->thc,      h->front};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -435,7 +558,7 @@ int gp_upload(gp_handle *h, const double *phi, const double *mu) {
   h->prev_valid = false;
   h->flags_valid = false;
   h->codes_valid = false;
-  return GP_OK;
+  return frame_reset(h);
 }
 
 int gp_download(gp_handle *h, double *phi, double *mu, double *dphi) {
@@ -471,10 +594,11 @@ int gp_download_fields(gp_handle *h, gp_cell *cells, int64_t ncells) {
   if (!h || !cells) return h ? gp_fail(h, GP_E_ARG, "null argument") : GP_E_ARG;
   const int64_t n = h->rx * h->ry;
   if (ncells != n) return gp_fail(h, GP_E_ARG, "ncells %lld != rows_x*rows_y %lld", (long long)ncells, (long long)n);
-  std::vector<double> phi((size_t)h->N * n), mu((size_t)h->NC * n), dphi((size_t)h->N * n);
+  std::vector<double> phi((size_t)h->N * n), mu((size_t)h->NC * n), dphi((size_t)h->N * n), T((size_t)h->ry);
   int rc = gp_download(h, phi.data(), mu.data(), dphi.data());
-  if (rc) return rc;
+  if (rc || (rc = gp_get_frame(h, nullptr, T.data()))) return rc;
   for (int64_t i = 0; i < n; i++) {
+    cells[i].temperature = T[(size_t)(i % h->ry)];
     for (int a = 0; a < h->N; a++) {
       cells[i].phia[a] = phi[(size_t)a * n + i];
       if (cells[i].deltaphi) cells[i].deltaphi[a] = dphi[(size_t)a * n + i];
@@ -526,6 +650,20 @@ int gp_get_stats(gp_handle *h, gp_stats *st) {
   return GP_OK;
 }
 
+int gp_get_frame(gp_handle *h, int64_t *shift_offset, double *T_rows) {
+  if (!h) return GP_E_ARG;
+  if (shift_offset) *shift_offset = h->shift_off;
+  if (T_rows) {
+    if (h->p.tempgrady) {
+      HIPCHK(h, hipMemcpyAsync(T_rows, h->trow, (size_t)h->ry * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+      HIPCHK(h, hipStreamSynchronize(h->stream));
+    } else {
+      for (int64_t y = 0; y < h->ry; y++) T_rows[y] = h->p.T;
+    }
+  }
+  return GP_OK;
+}
+
 int gp_invariants_noalloc(gp_handle *h, const void *phi, const void *mu, double *out, void *stream) {
   if (!h || !phi || !mu || !out) return h ? gp_fail(h, GP_E_ARG, "null argument") : GP_E_ARG;
   const int64_t mx = h->p.mesh_x, my = h->p.mesh_y;
@@ -555,11 +693,13 @@ int gp_step(gp_handle *h, int64_t nsteps, gp_stats *stats) {
   step_fn f = h->ops->step;
   {
     ProfScope ps(h, GP_KERNEL_CALL);
+    const bool framed = h->p.tempgrady || h->p.shift;
     for (int64_t t = 0; t < nsteps; t++) {
       int rc = f(h, h->use_march ? (stats ? MODE_MARCH_STATS : MODE_MARCH) : MODE_TWOPASS);
       if (rc) return rc;
       h->steps++;
       h->prev_valid = h->use_march;
+      if (framed && (rc = frame_after_step(h, h->steps))) return rc;
     }
   }
   if (stats) {
@@ -590,6 +730,8 @@ int gp_theta_values(const gp_handle *h, double *theta) {
 
 int gp_adjoint_forward(gp_handle *h, int64_t nsteps, int64_t every) {
   if (!h || nsteps < 0) return h ? gp_fail(h, GP_E_ARG, "bad nsteps") : GP_E_ARG;
+  if (h->p.tempgrady || h->p.shift)
+    return gp_fail(h, GP_E_ARG, "the adjoint covers isothermal runs in a fixed frame");
   if (!h->use_march) return gp_fail(h, GP_E_ARG, "the adjoint records single-pass steps (GP_PATH=twopass is set)");
   if (h->fs > 0x7fffffff) return gp_fail(h, GP_E_ARG, "the adjoint indexes cells with 32 bits: grid too large");
   delete h->adj;

@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "gp_common.h"
+#include "gp_frame.h"
 #include "gp_hip.h"
 #include "gp_model.h"
 
@@ -61,6 +62,14 @@ struct gp_handle {
   int8_t *lo[2];
   void *model64;
   double *wide[2];
+  /* non-isothermal runs (tempgrady) and the moving frame (shift), gp_frame.h: T of every allocated row after the
+   * last step, the row coefficients of the current step's two sweeps (RowThermo<R>), the frame's interface row
+   * (device) and the cells the frame has moved since gp_upload */
+  double *trow;
+  void *thp, *thc;
+  int *front;
+  int64_t shift_off;
+  gpk::ThermoParams tp;
   double *partials;         /* [statistic][nparts_max], statistic-major */
   double *stats_dev;        /* cumulative stats (kNStatMax) */
   double *inv_dev;          /* solute[NC], energy of the current state */

@@ -67,22 +67,43 @@ struct Model {
    * lap_y = 1/dy^2 */
   R Kbulk[P][NCM][NCM];
   R lap_x, lap_y;
+  /* non-isothermal runs: the dc/dT source of the interface mu update (calculate_divergence_concentration.h:82-85),
+   * sum_a dcbdT[a][k] h_a times dt (-G v), with the serial driver's dcbdT_phase and temperature_gradientY.GRADIENT
+   * (never assigned there: 0) and the Tempgrady velocity */
+  R dcbdT[P][NCM];
+  R tg_G, tg_vel;
+};
+
+/* Temperature-dependent coefficients of one row of cells (non-isothermal runs, where T depends on y only):
+ * Beq + dBbdT (T - Teq), the offset of c(mu) and Mu(c) (functionF_01.h:19-37), and B(T), C(T), the free energy's
+ * (function_B / function_C, functionF_01.h:62-104, set by init_propertymatrices(T) per cell). */
+template <typename R>
+struct RowThermo {
+  R bmu[P][NCM];
+  R b[P][NCM];
+  R c[P];
+  R T;
 };
 
 /* Stencil accessor over SoA planes: field a at (x+i, y+j) relative to the centre index c.
  * POW2: dx and dy are powers of two, so v/dx == v*(1/dx) exactly (the quotient and the product are the
  * same real number, rounded once) and the divisions by the spacings become multiplications. */
-template <typename R, bool POW2 = false>
+template <typename R, bool POW2 = false, bool TG = false>
 struct View {
   static constexpr bool kPow2 = POW2;
+  /* TG: non-isothermal, th[y] holds the coefficients of allocated row y and y is the centre's row */
+  static constexpr bool kTG = TG;
   const R *phi, *mu, *dphi;
   int64_t fs, ps, c;
+  const RowThermo<R> *th = nullptr;
+  int64_t y = 0;
   GP_HD R ph(int a, int i, int j) const { return phi[a * fs + c + i * ps + j]; }
   GP_HD R m(int k, int i, int j) const { return mu[k * fs + c + i * ps + j]; }
   GP_HD R dp(int a, int i, int j) const { return dphi[a * fs + c + i * ps + j]; }
   GP_HD View shift(int i, int j) const {
     View v = *this;
     v.c = c + i * ps + j;
+    v.y = y + j;
     return v;
   }
 };
@@ -104,6 +125,7 @@ struct RegStore {
 template <int N, int NC, typename R, bool POW2>
 struct RegView {
   static constexpr bool kPow2 = POW2;
+  static constexpr bool kTG = false;
   const RegStore<N, NC, R> *s;
   int si, sj;
   GP_HD R ph(int a, int i, int j) const { return s->p[a][si + i + 1][sj + j + 1]; }
@@ -189,19 +211,35 @@ GP_HD R dhphi(const R *phi, int b, int a) {
   return s;
 }
 
+/* c(mu) and Mu(c) with the offsets bmu[a][k] = Beq + dBbdT (T - Teq) of the cell's temperature */
+template <int NC, typename R>
+GP_HD R c_mu_b(const Model<R> &m, const R (*bmu)[NCM], const R *mu, int a, int i) {
+  R s = R(0.0);
+  for (int k = 0; k < NC; k++) s += m.cmu[a][i][k] * (mu[k] - bmu[a][k]);
+  return s;
+}
 template <int NC, typename R>
 GP_HD R c_mu(const Model<R> &m, const R *mu, int a, int i) {
-  R s = R(0.0);
-  for (int k = 0; k < NC; k++) s += m.cmu[a][i][k] * (mu[k] - m.Bmu[a][k]);
-  return s;
+  return c_mu_b<NC>(m, m.Bmu, mu, a, i);
 }
 
 template <int NC, typename R>
-GP_HD R Mu(const Model<R> &m, const R *c, int a, int i) {
-  R s = R(2.0) * m.A[a][i][i] * c[i] + m.Bmu[a][i];
+GP_HD R Mu_b(const Model<R> &m, const R (*bmu)[NCM], const R *c, int a, int i) {
+  R s = R(2.0) * m.A[a][i][i] * c[i] + bmu[a][i];
   for (int j = 0; j < NC; j++)
     if (i != j) s += m.A[a][i][j] * c[j];
   return s;
+}
+template <int NC, typename R>
+GP_HD R Mu(const Model<R> &m, const R *c, int a, int i) {
+  return Mu_b<NC>(m, m.Bmu, c, a, i);
+}
+
+/* the offsets of the row at y offset j from a view's centre: the row's own (non-isothermal views) or the model's */
+template <class V, typename R>
+GP_HD auto row_bmu(const Model<R> &m, const V &v, int j) -> const R (*)[NCM] {
+  if constexpr (V::kTG) return v.th[v.y + j].bmu;
+  else return m.Bmu;
 }
 
 /* Linear part of Mu: the change of mu_i for a change dc of the composition in phase a (Mu is affine in c). */
@@ -223,30 +261,39 @@ template <typename R>
 constexpr bool kDirectBulkMu = sizeof(R) < sizeof(double);
 
 template <int NC, typename R>
-GP_HD R free_energy(const Model<R> &m, const R *c, int a) {
+GP_HD R free_energy_b(const Model<R> &m, const R (*B)[NCM], const R *C, const R *c, int a) {
   R s = R(0.0);
   for (int i = 0; i < NC; i++) {
     for (int j = 0; j < NC; j++)
       if (i <= j) s += m.A[a][i][j] * c[i] * c[j];
-    s += m.B[a][i] * c[i];
+    s += B[a][i] * c[i];
   }
-  s += m.C[a];
+  s += C[a];
   return s;
 }
+template <int NC, typename R>
+GP_HD R free_energy(const Model<R> &m, const R *c, int a) {
+  return free_energy_b<NC>(m, m.B, m.C, c, a);
+}
 
-/* psi_b(mu) = f_b(c_b(mu)) - mu.c_b(mu) for every phase (the phase-independent part of dpsi) */
+/* psi_b(mu) = f_b(c_b(mu)) - mu.c_b(mu) for every phase (the phase-independent part of dpsi), with the given
+ * temperature-dependent coefficients */
 template <int N, int NC, typename R>
-GP_HD void psi_all(const Model<R> &m, const R *mu, R *psi) {
+GP_HD void psi_all_b(const Model<R> &m, const R (*bmu)[NCM], const R (*B)[NCM], const R *C, const R *mu, R *psi) {
   for (int b = 0; b < N; b++) {
     R c[NC];
     R p = R(0.0);
     for (int k = 0; k < NC; k++) {
-      c[k] = c_mu<NC>(m, mu, b, k);
+      c[k] = c_mu_b<NC>(m, bmu, mu, b, k);
       p -= mu[k] * c[k];
     }
-    p += free_energy<NC>(m, c, b);
+    p += free_energy_b<NC>(m, B, C, c, b);
     psi[b] = p;
   }
+}
+template <int N, int NC, typename R>
+GP_HD void psi_all(const Model<R> &m, const R *mu, R *psi) {
+  psi_all_b<N, NC>(m, m.Bmu, m.B, m.C, mu, psi);
 }
 
 template <int N, typename R>
@@ -531,7 +578,14 @@ GP_HD void phi_delta(const Model<R> &m, const V &v, bool smooth, R *out) {
     cyc[a] = cy(m, v, a, 0, 0);
   }
   R psi[N];
-  if (!smooth) psi_all<N, NC>(m, mu, psi);
+  if (!smooth) {
+    if constexpr (V::kTG) {
+      const RowThermo<R> &t = v.th[v.y];
+      psi_all_b<N, NC>(m, t.bmu, t.b, t.c, mu, psi);
+    } else {
+      psi_all<N, NC>(m, mu, psi);
+    }
+  }
   R sum_lambda = R(0.0);
   int active = 0;
   for (int a = 0; a < N; a++) {
@@ -598,6 +652,7 @@ GP_HD void jat_x(const Model<R> &m, const V &v0, int i, int j, R *jx) {
   R gyl = R(0.5) * (cy(m, v, L, 0, 0) + cy(m, v, L, 1, 0));
   R gxL = gx(m, v, L, 0, 0);
   R nxl = qsqrt(gyl * gyl + gxL * gxL);
+  const auto bm = row_bmu(m, v, 0); /* both face cells are in the centre's row */
   for (int a = 0; a < N - 1; a++) {
     R gpy = R(0.5) * (cy(m, v, a, 0, 0) + cy(m, v, a, 1, 0));
     R gxa = gx(m, v, a, 0, 0);
@@ -610,8 +665,8 @@ GP_HD void jat_x(const Model<R> &m, const V &v0, int i, int j, R *jx) {
     }
     R dc = v.dp(a, 0, 0), df = v.dp(a, 1, 0);
     for (int k = 0; k < NC; k++) {
-      R cl_f = c_mu<NC>(m, muf, L, k), cl_c = c_mu<NC>(m, muc, L, k);
-      R cs_f = c_mu<NC>(m, muf, a, k), cs_c = c_mu<NC>(m, muc, a, k);
+      R cl_f = c_mu_b<NC>(m, bm, muf, L, k), cl_c = c_mu_b<NC>(m, bm, muc, L, k);
+      R cs_f = c_mu_b<NC>(m, bm, muf, a, k), cs_c = c_mu_b<NC>(m, bm, muc, a, k);
       R sp = gxa * gxL;
       sp += gpy * gyl;
       if (nxl > R(0.0)) sp = qdiv(sp, (nxl * ng));
@@ -645,6 +700,7 @@ GP_HD void jat_y(const Model<R> &m, const V &v0, int i, int j, int64_t yglob, R 
   R gxl = R(0.5) * (cx(m, v, L, 0, 0) + cx(m, v, L, 0, 1));
   R gyL = gy(m, v, L, 0, 0);
   R nyl = qsqrt(gxl * gxl + gyL * gyL);
+  const auto bmc = row_bmu(m, v, 0), bmr = row_bmu(m, v, 1); /* the face's lower and upper rows */
   for (int a = 0; a < N - 1; a++) {
     R gpx = R(0.5) * (cx(m, v, a, 0, 0) + cx(m, v, a, 0, 1));
     R gya = gy(m, v, a, 0, 0);
@@ -657,8 +713,8 @@ GP_HD void jat_y(const Model<R> &m, const V &v0, int i, int j, int64_t yglob, R 
     }
     R dc = v.dp(a, 0, 0), dr = v.dp(a, 0, 1);
     for (int k = 0; k < NC; k++) {
-      R cl_r = c_mu<NC>(m, mur, L, k), cl_c = c_mu<NC>(m, muc, L, k);
-      R cs_r = c_mu<NC>(m, mur, a, k), cs_c = c_mu<NC>(m, muc, a, k);
+      R cl_r = c_mu_b<NC>(m, bmr, mur, L, k), cl_c = c_mu_b<NC>(m, bmc, muc, L, k);
+      R cs_r = c_mu_b<NC>(m, bmr, mur, a, k), cs_c = c_mu_b<NC>(m, bmc, muc, a, k);
       R sp = gya * gyL;
       sp += gpx * gxl;
       if (nyl > R(0.0)) sp = qdiv(sp, (nyl * ng));
@@ -755,11 +811,13 @@ GP_HD void lu_inverse(R (*a)[n], R (*inv)[n]) {
 template <int N, int NC, typename R, class V>
 GP_HD void mu_update(const Model<R> &m, const V &v, bool smooth, const R *dph, const R *jx0, const R *jxb,
                      const R *jy0, const R *jyl, R *phi_new, R *mu_new, R *dmu) {
-  R phi[N], mu[NC], sum[NC], divflux[NC], divjat[NC], deltamu[NC];
+  R phi[N], mu[NC], sum[NC], sum_dcbdT[NC], divflux[NC], divjat[NC], deltamu[NC];
+  const auto bm = row_bmu(m, v, 0);
   for (int a = 0; a < N; a++) phi[a] = v.ph(a, 0, 0);
   for (int k = 0; k < NC; k++) {
     mu[k] = v.m(k, 0, 0);
     sum[k] = R(0.0);
+    sum_dcbdT[k] = R(0.0);
     divjat[k] = R(0.0);
     deltamu[k] = R(0.0);
   }
@@ -767,7 +825,10 @@ GP_HD void mu_update(const Model<R> &m, const V &v, bool smooth, const R *dph, c
     for (int a = 0; a < N; a++) {
       R sdh = R(0.0);
       for (int b = 0; b < N; b++) sdh += dhphi<N>(phi, a, b) * dph[b];
-      for (int k = 0; k < NC; k++) sum[k] += c_mu<NC>(m, mu, a, k) * sdh;
+      for (int k = 0; k < NC; k++) {
+        sum[k] += c_mu_b<NC>(m, bm, mu, a, k) * sdh;
+        if constexpr (V::kTG) sum_dcbdT[k] += m.dcbdT[a][k] * hphi<N>(phi, a);
+      }
     }
   }
   R Dx[NC][NC], gmx[NC], Dxb[NC][NC], gmxb[NC], Dy[NC][NC], gmy[NC], Dyl[NC][NC], gmyl[NC];
@@ -801,11 +862,11 @@ GP_HD void mu_update(const Model<R> &m, const V &v, bool smooth, const R *dph, c
   } else if (bulk >= 0) {
     R c_new[NC];
     for (int k = 0; k < NC; k++) {
-      R c_old = c_mu<NC>(m, mu, bulk, k);
+      R c_old = c_mu_b<NC>(m, bm, mu, bulk, k);
       c_new[k] = smooth ? c_old + m.dt * (divflux[k]) : c_old + m.dt * (divflux[k]) - divjat[k];
     }
     for (int k = 0; k < NC; k++) {
-      R nm = Mu<NC>(m, c_new, bulk, k);
+      R nm = Mu_b<NC>(m, bm, c_new, bulk, k);
       deltamu[k] = nm - munew[k];
       munew[k] = nm;
     }
@@ -817,6 +878,8 @@ GP_HD void mu_update(const Model<R> &m, const V &v, bool smooth, const R *dph, c
     for (int a = 0; a < N; a++) h[a] = hphi<N>(phi, a);
     for (int k = 0; k < NC; k++) {
       deltac[k] = smooth ? m.dt * (divflux[k]) : m.dt * (divflux[k]) - divjat[k] - sum[k];
+      if constexpr (V::kTG)
+        if (!smooth) deltac[k] = deltac[k] - sum_dcbdT[k] * m.dt * (-m.tg_G * m.tg_vel);
       for (int l = 0; l < NC; l++)
         for (int a = 0; a < N; a++) dcdmu[k][l] += m.cmu[a][k][l] * h[a];
     }
@@ -981,6 +1044,10 @@ inline int make_model(const gp_params *p, Model<R> *m) {
             for (int q = 0; q < NC; q++) s += m->DC[a][k][l][q] * (a == b ? R(1.0) : R(0.0));
         m->Mpure[b][k][l] = s;
       }
+  for (int a = 0; a < P; a++)
+    for (int k = 0; k < NCM; k++) m->dcbdT[a][k] = (R)p->dcbdT_phase[a * NCM + k];
+  m->tg_G = R(0.0); /* temperature_gradientY.GRADIENT, which the serial driver never assigns */
+  m->tg_vel = (R)p->tg_velocity;
   m->lap_x = (R)(1.0 / (p->dx * p->dx));
   m->lap_y = (R)(1.0 / (p->dy * p->dy));
   for (int b = 0; b < N; b++)

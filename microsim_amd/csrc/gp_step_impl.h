@@ -9,32 +9,34 @@
 #include "gp_march.h"
 #include "gp_adjoint.h"
 #include "gp_c32.h"
+#include "gp_frame.h"
 
 namespace gpk {
 
 /* ------------------------------------------------------------------------------------------------ kernels */
 
-template <int N, int NC, typename R, bool SMOOTH, bool P2>
+/* TG: non-isothermal, th holds the coefficients of every allocated row (gp_frame.h) */
+template <int N, int NC, typename R, bool SMOOTH, bool P2, bool TG = false>
 __global__ __launch_bounds__(kBX *kBY) void k_phi(const Model<R> *__restrict__ mp, const R *__restrict__ phi,
                                                   const R *__restrict__ mu, R *__restrict__ dphi, int64_t fs,
-                                                  int64_t ps, int64_t mx, int64_t my) {
+                                                  int64_t ps, int64_t mx, int64_t my, const RowThermo<R> *th) {
   const int64_t y = 2 + (int64_t)blockIdx.x * kBX + threadIdx.x;
   const int64_t x = 2 + (int64_t)blockIdx.y * kBY + threadIdx.y;
   if (x > mx + 3 || y > my + 3) return;
   const Model<R> &m = *mp;
   const int64_t c = x * ps + y + kYOff;
-  View<R, P2> v{phi, mu, nullptr, fs, ps, c};
+  View<R, P2, TG> v{phi, mu, nullptr, fs, ps, c, th, y};
   R d[N];
   phi_delta<N, NC>(m, v, SMOOTH, d);
   for (int a = 0; a < N; a++) dphi[a * fs + c] = d[a];
 }
 
-template <int N, int NC, typename R, bool SMOOTH, bool P2>
+template <int N, int NC, typename R, bool SMOOTH, bool P2, bool TG = false>
 __global__ __launch_bounds__(kBX *kBY) void k_mu(const Model<R> *__restrict__ mp, const R *__restrict__ phi,
                                                  const R *__restrict__ mu, const R *__restrict__ dphi,
                                                  R *__restrict__ phi_o, R *__restrict__ mu_o, Maps maps,
                                                  double *__restrict__ partials, int64_t fs, int64_t ps, int64_t mx,
-                                                 int64_t my) {
+                                                 int64_t my, const RowThermo<R> *th) {
   const int64_t y = 3 + (int64_t)blockIdx.x * kBX + threadIdx.x;
   const int64_t x = 3 + (int64_t)blockIdx.y * kBY + threadIdx.y;
   const bool live = (x <= mx + 2 && y <= my + 2);
@@ -54,7 +56,7 @@ __global__ __launch_bounds__(kBX *kBY) void k_mu(const Model<R> *__restrict__ mp
   if (live) {
     const Model<R> &m = *mp;
     const int64_t c = x * ps + y + kYOff;
-    View<R, P2> v{phi, mu, dphi, fs, ps, c};
+    View<R, P2, TG> v{phi, mu, dphi, fs, ps, c, th, y};
     R jx0[NC], jxb[NC], jy0[NC], jyl[NC], d[N], pn[N], mn[NC], dm[NC];
     if (!SMOOTH) {
       jat_x<N, NC>(m, v, 0, 0, jx0);
@@ -288,14 +290,22 @@ static int launch_step_p(gp_handle *h, int mode) {
   const bool smooth = mode == MODE_SMOOTH;
   if (mode == MODE_DPHI_CUR) {
     /* delta-phi of the current state (the adjoint's recompute) */
-    hipLaunchKernelGGL((k_phi<N, NC, R, false, P2>), g1, blk, 0, h->stream, m, phi, mu, dphi, h->fs, h->ps, mx, my);
+    hipLaunchKernelGGL((k_phi<N, NC, R, false, P2>), g1, blk, 0, h->stream, m, phi, mu, dphi, h->fs, h->ps, mx, my, (const RowThermo<R> *)nullptr);
     HIPCHK(h, hipGetLastError());
     return GP_OK;
   }
+  const bool tg = h->p.tempgrady != 0;
+  const RowThermo<R> *thp = (const RowThermo<R> *)h->thp, *thc = (const RowThermo<R> *)h->thc;
   if (mode == MODE_DPHI_PREV) {
-    /* delta-phi of the last step, recomputed from the previous state (still in the other buffer) */
+    /* delta-phi of the last step, recomputed from the previous state (still in the other buffer; non-isothermal
+     * runs: with the last step's row coefficients) */
     const R *pp = (const R *)h->phi[1 - h->cur], *mp = (const R *)h->mu[1 - h->cur];
-    hipLaunchKernelGGL((k_phi<N, NC, R, false, P2>), g1, blk, 0, h->stream, m, pp, mp, dphi, h->fs, h->ps, mx, my);
+    if (tg)
+      hipLaunchKernelGGL((k_phi<N, NC, R, false, P2, true>), g1, blk, 0, h->stream, m, pp, mp, dphi, h->fs, h->ps,
+                         mx, my, thp);
+    else
+      hipLaunchKernelGGL((k_phi<N, NC, R, false, P2>), g1, blk, 0, h->stream, m, pp, mp, dphi, h->fs, h->ps, mx, my,
+                         (const RowThermo<R> *)nullptr);
     HIPCHK(h, hipGetLastError());
     return GP_OK;
   }
@@ -416,25 +426,56 @@ static int launch_step_p(gp_handle *h, int mode) {
   }
   h->flags_valid = false; /* the two-pass kernels form neither tile flags nor phase codes */
   h->codes_valid = false;
+  if (tg) {
+    /* non-isothermal: this step's row coefficients, then the two sweeps with them */
+    hipLaunchKernelGGL((k_thermo<R>), dim3((unsigned)((h->ry + 255) / 256)), dim3(256), 0, h->stream, h->tp, h->trow,
+                       (RowThermo<R> *)h->thp, (RowThermo<R> *)h->thc, h->ry);
+    {
+      ProfScope ps(h, GP_KERNEL_PHI);
+      if (smooth)
+        hipLaunchKernelGGL((k_phi<N, NC, R, true, P2, true>), g1, blk, 0, h->stream, m, phi, mu, dphi, h->fs, h->ps,
+                           mx, my, thp);
+      else
+        hipLaunchKernelGGL((k_phi<N, NC, R, false, P2, true>), g1, blk, 0, h->stream, m, phi, mu, dphi, h->fs, h->ps,
+                           mx, my, thp);
+    }
+    {
+      ProfScope ps(h, GP_KERNEL_MU);
+      if (smooth)
+        hipLaunchKernelGGL((k_mu<N, NC, R, true, P2, true>), g2, blk, 0, h->stream, m, phi, mu, dphi, phi_o, mu_o,
+                           h->maps, h->partials, h->fs, h->ps, mx, my, thc);
+      else
+        hipLaunchKernelGGL((k_mu<N, NC, R, false, P2, true>), g2, blk, 0, h->stream, m, phi, mu, dphi, phi_o, mu_o,
+                           h->maps, h->partials, h->fs, h->ps, mx, my, thc);
+    }
+    if (!smooth) {
+      ProfScope ps(h, GP_KERNEL_STATS);
+      hipLaunchKernelGGL((k_stats<N, NC>), dim3(3 * N + 3 * NC + 1), dim3(kStatThreads), 0, h->stream,
+                         h->partials, h->nblocks_mu, h->nblocks_mu, h->stats_dev);
+    }
+    HIPCHK(h, hipGetLastError());
+    h->cur = 1 - h->cur;
+    return GP_OK;
+  }
   if (smooth) {
     {
       ProfScope ps(h, GP_KERNEL_PHI);
-      hipLaunchKernelGGL((k_phi<N, NC, R, true, P2>), g1, blk, 0, h->stream, m, phi, mu, dphi, h->fs, h->ps, mx, my);
+      hipLaunchKernelGGL((k_phi<N, NC, R, true, P2>), g1, blk, 0, h->stream, m, phi, mu, dphi, h->fs, h->ps, mx, my, (const RowThermo<R> *)nullptr);
     }
     {
       ProfScope ps(h, GP_KERNEL_MU);
       hipLaunchKernelGGL((k_mu<N, NC, R, true, P2>), g2, blk, 0, h->stream, m, phi, mu, dphi, phi_o, mu_o, h->maps,
-                         h->partials, h->fs, h->ps, mx, my);
+                         h->partials, h->fs, h->ps, mx, my, (const RowThermo<R> *)nullptr);
     }
   } else {
     {
       ProfScope ps(h, GP_KERNEL_PHI);
-      hipLaunchKernelGGL((k_phi<N, NC, R, false, P2>), g1, blk, 0, h->stream, m, phi, mu, dphi, h->fs, h->ps, mx, my);
+      hipLaunchKernelGGL((k_phi<N, NC, R, false, P2>), g1, blk, 0, h->stream, m, phi, mu, dphi, h->fs, h->ps, mx, my, (const RowThermo<R> *)nullptr);
     }
     {
       ProfScope ps(h, GP_KERNEL_MU);
       hipLaunchKernelGGL((k_mu<N, NC, R, false, P2>), g2, blk, 0, h->stream, m, phi, mu, dphi, phi_o, mu_o, h->maps,
-                         h->partials, h->fs, h->ps, mx, my);
+                         h->partials, h->fs, h->ps, mx, my, (const RowThermo<R> *)nullptr);
     }
     {
       ProfScope ps(h, GP_KERNEL_STATS);
@@ -506,7 +547,7 @@ static int adj_widen_t(gp_handle *h) {
   dim3 g1((unsigned)((my + 2 + kBX - 1) / kBX), (unsigned)((mx + 2 + kBY - 1) / kBY));
   hipLaunchKernelGGL((k_phi<N, NC, double, false, P2>), g1, dim3(kBX, kBY), 0, h->stream,
                      (const Model<double> *)A->model64, (const double *)A->st64, (const double *)(A->st64 + np),
-                     A->dphi64, h->fs, h->ps, mx, my);
+                     A->dphi64, h->fs, h->ps, mx, my, (const RowThermo<double> *)nullptr);
   HIPCHK(h, hipGetLastError());
   return GP_OK;
 }
@@ -569,7 +610,7 @@ static int launch_step_c(gp_handle *h, int mode) {
     const int b = mode == MODE_DPHI_CUR ? h->cur : 1 - h->cur;
     if ((rc = c32_wide_buffer(h, 0)) || (rc = c32_widen_state(h, b, h->wide[0]))) return rc;
     hipLaunchKernelGGL((k_phi<N, NC, double, false, P2>), g1, blk, 0, h->stream, md, (const double *)h->wide[0],
-                       (const double *)(h->wide[0] + N * fs), dphi, fs, h->ps, mx, my);
+                       (const double *)(h->wide[0] + N * fs), dphi, fs, h->ps, mx, my, (const RowThermo<double> *)nullptr);
     HIPCHK(h, hipGetLastError());
     return GP_OK;
   }
@@ -581,12 +622,12 @@ static int launch_step_c(gp_handle *h, int mode) {
     const double *wp = h->wide[0], *wm = h->wide[0] + N * fs;
     {
       ProfScope ps(h, GP_KERNEL_PHI);
-      hipLaunchKernelGGL((k_phi<N, NC, double, true, P2>), g1, blk, 0, h->stream, md, wp, wm, dphi, fs, h->ps, mx, my);
+      hipLaunchKernelGGL((k_phi<N, NC, double, true, P2>), g1, blk, 0, h->stream, md, wp, wm, dphi, fs, h->ps, mx, my, (const RowThermo<double> *)nullptr);
     }
     {
       ProfScope ps(h, GP_KERNEL_MU);
       hipLaunchKernelGGL((k_mu<N, NC, double, true, P2>), g2, blk, 0, h->stream, md, wp, wm, (const double *)dphi,
-                         h->wide[1], h->wide[1] + N * fs, h->maps, h->partials, fs, h->ps, mx, my);
+                         h->wide[1], h->wide[1] + N * fs, h->maps, h->partials, fs, h->ps, mx, my, (const RowThermo<double> *)nullptr);
     }
     HIPCHK(h, hipGetLastError());
     if ((rc = c32_narrow_state(h, h->wide[1], 1 - h->cur))) return rc;
@@ -689,7 +730,7 @@ static int adj_widen_c(gp_handle *h) {
   dim3 g1((unsigned)((my + 2 + kBX - 1) / kBX), (unsigned)((mx + 2 + kBY - 1) / kBY));
   hipLaunchKernelGGL((k_phi<N, NC, double, false, P2>), g1, dim3(kBX, kBY), 0, h->stream,
                      (const Model<double> *)A->model64, (const double *)A->st64, (const double *)(A->st64 + np),
-                     A->dphi64, h->fs, h->ps, mx, my);
+                     A->dphi64, h->fs, h->ps, mx, my, (const RowThermo<double> *)nullptr);
   HIPCHK(h, hipGetLastError());
   return GP_OK;
 }

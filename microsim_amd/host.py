@@ -94,11 +94,24 @@ def state_case(input_path: str, phi_interior: np.ndarray, mu_interior: np.ndarra
     return Case(inp, phi, mu)
 
 
-def write_vtk(case_inp: _abi.GPInput, path: str, phi: np.ndarray, mu: np.ndarray) -> None:
+def temperature_rows(params: _abi.GPParams, t: int = 0, shift_offset: int = 0) -> np.ndarray:
+    """Temperature of every allocated row as the serial driver sets it before the loop (t = 0) or after step t,
+    with the moving frame shifted by shift_offset rows (T everywhere for an isothermal run)."""
+    T = np.empty(int(params.mesh_y) + 6)
+    _abi.host_lib().gp_temperature_rows(C.byref(params), int(t), int(shift_offset), _abi.dptr(T))
+    return T
+
+
+def write_vtk(case_inp: _abi.GPInput, path: str, phi: np.ndarray, mu: np.ndarray, T_rows: np.ndarray = None) -> None:
+    """The snapshot file; T_rows (per allocated row) is required when ISOTHERMAL = 0."""
     lib = _abi.host_lib()
     phi = np.ascontiguousarray(phi, dtype=np.float64)
     mu = np.ascontiguousarray(mu, dtype=np.float64)
-    rc = lib.gp_write_vtk(C.byref(case_inp), path.encode(), _abi.dptr(phi), _abi.dptr(mu))
+    if case_inp.p.tempgrady and T_rows is None:
+        raise ValueError("a non-isothermal snapshot needs the row temperatures (T_rows)")
+    T = None if T_rows is None else np.ascontiguousarray(T_rows, dtype=np.float64)
+    rc = lib.gp_write_vtk(C.byref(case_inp), path.encode(), _abi.dptr(phi), _abi.dptr(mu),
+                          None if T is None else _abi.dptr(T))
     if rc != 0:
         raise IOError("cannot write %s" % path)
 

@@ -23,7 +23,7 @@ enum {
   SEEN_DIM = 1u << 0, SEEN_MX = 1u << 1, SEEN_MY = 1u << 2, SEEN_DX = 1u << 3, SEEN_DY = 1u << 4,
   SEEN_DT = 1u << 5, SEEN_N = 1u << 6, SEEN_K = 1u << 7, SEEN_EPS = 1u << 8, SEEN_TAU = 1u << 9,
   SEEN_GAMMA = 1u << 10, SEEN_TAUAB = 1u << 11, SEEN_DAB = 1u << 12, SEEN_GABC = 1u << 13, SEEN_T = 1u << 14,
-  SEEN_TEQ = 1u << 15, SEEN_ISO = 1u << 16
+  SEEN_TEQ = 1u << 15, SEEN_ISO = 1u << 16, SEEN_TG = 1u << 17
 };
 
 static void seterr(char *err, size_t n, const char *fmt, ...) {
@@ -341,8 +341,23 @@ static int parse_pass1(FILE *fr, gp_input *in, int quiet, char *err, size_t errl
     }
     else if (IS("OBSTACLE")) { /* superseded by Function_W (initialize_functions_solverloop.h:21-32) */ }
     else if (IS("Function_W") && N > 0 && K - 1 > 0) fn_w = atoi(v);
-    else if (IS("ISOTHERMAL")) { in->isothermal = atoi(v); in->seen |= SEEN_ISO; }
-    else if (IS("Tempgrady")) { /* only used when ISOTHERMAL = 0 */ }
+    else if (IS("ISOTHERMAL")) {
+      in->isothermal = atoi(v);
+      in->seen |= SEEN_ISO;
+      if (!in->isothermal) p->tempgrady = 1; /* TEMPGRADY follows ISOTHERMAL = 0 and stays set */
+    }
+    else if (IS("Tempgrady") && p->tempgrady) {
+      /* {base_temp, DeltaT, Distance, gradient_OFFSET, velocity}, read only once TEMPGRADY is set */
+      char tok[8][100];
+      int nt = split_tuple(v, tok, 8);
+      if (nt < 5) { seterr(err, errlen, "Tempgrady needs {base_temp, DeltaT, Distance, gradient_OFFSET, velocity}"); return GP_E_ARG; }
+      p->tg_base_temp = atof(tok[0]);
+      p->tg_deltaT = atof(tok[1]);
+      p->tg_distance = atof(tok[2]);
+      p->tg_offset = atof(tok[3]);
+      p->tg_velocity = atof(tok[4]);
+      in->seen |= SEEN_TG;
+    }
     else if (IS("Shift")) in->shift = atoi(v);
     else if (IS("Shiftj")) { if (in->shift) in->shiftj = atol(v); }
     else if (IS("BINARY")) { in->binary = atoi(v); in->ternary = 0; in->dilute = 0; }
@@ -476,6 +491,17 @@ void gp_property_matrices(gp_input *in) {
     }
   }
   for (int k = 0; k < NC; k++) p->dBbdT[L * NCM + k] = 0.0;
+  /* what B(T), C(T) of a non-isothermal run need per row (gp_frame.h); dcbdT_phase stays 0 as in the serial driver
+   * (initialize_variables.h:22, never assigned afterwards) */
+  for (int a = 0; a < N - 1; a++) {
+    for (int k = 0; k < NC; k++) {
+      p->ceq_sol[a * NCM + k] = in->ceq[a][a][k];
+      p->ceq_liq[a * NCM + k] = in->ceq[a][L][k];
+      p->delta_c[a * NCM + k] = DELTA_C[a][k];
+    }
+    p->delta_T_sol[a] = DELTA_T[a][a];
+    p->delta_T_liq[a] = DELTA_T[a][L];
+  }
   /* function_B / function_C (functionF_01.h:62-104) at Teq and T */
   for (int pass = 0; pass < 2; pass++) {
     double Tv = pass == 0 ? p->Teq : p->T;
@@ -520,14 +546,15 @@ int gp_validate_input(const gp_input *in, char *err, size_t errlen) {
   }
   if (in->dimension != 2) { seterr(err, errlen, "only DIMENSION = 2 is supported"); return GP_E_ARG; }
   if (p->mesh_x < 4 || p->mesh_y < 4) { seterr(err, errlen, "MESH_X/MESH_Y must be >= 4"); return GP_E_ARG; }
-  if (!in->isothermal) { seterr(err, errlen, "ISOTHERMAL = 0 (TEMPGRADY) is not supported"); return GP_E_ARG; }
+  if (p->tempgrady && !(in->seen & SEEN_TG)) { seterr(err, errlen, "ISOTHERMAL = 0 needs Tempgrady after it"); return GP_E_ARG; }
+  if (p->tempgrady && (p->tg_distance == 0.0 || p->dy == 0.0)) { seterr(err, errlen, "Tempgrady Distance must be nonzero"); return GP_E_ARG; }
   if (in->fn_f != 1) { seterr(err, errlen, "only Function_F = 1 (parabolic) is supported"); return GP_E_ARG; }
   if (p->fn_w != 1 && p->fn_w != 2) { seterr(err, errlen, "Function_W must be 1 or 2"); return GP_E_ARG; }
   if (p->fn_aniso != 0 && p->fn_aniso != 1) { seterr(err, errlen, "Function_anisotropy must be 0 or 1"); return GP_E_ARG; }
   if (p->fn_aniso == 1 && p->fold != 4) { seterr(err, errlen, "Function_anisotropy = 1 needs Anisotropy_type = 4"); return GP_E_ARG; }
   if (p->fn_aniso == 1 && !(in->seen & SEEN_DAB)) { seterr(err, errlen, "Function_anisotropy = 1 needs dab"); return GP_E_ARG; }
   if (in->noise_phasefield) { seterr(err, errlen, "Noise_phasefield is not supported (rand() stream)"); return GP_E_ARG; }
-  if (in->shift) { seterr(err, errlen, "Shift = 1 (moving frame) is not supported"); return GP_E_ARG; }
+  if (in->shift && (in->shiftj < 1 || in->shiftj >= rowsy(in) - 1)) { seterr(err, errlen, "Shiftj must lie inside the allocated rows"); return GP_E_ARG; }
   if (p->nphases >= 3 && !(in->seen & SEEN_GABC)) { seterr(err, errlen, "NUMPHASES >= 3 needs Gamma_abc"); return GP_E_ARG; }
   if (p->solve == GP_SOLVE_BINARY && p->ncomp != 2) { seterr(err, errlen, "BINARY = 1 needs NUMCOMPONENTS = 2"); return GP_E_ARG; }
   if (p->solve == GP_SOLVE_TERNARY && p->ncomp != 3) { seterr(err, errlen, "TERNARY = 1 needs NUMCOMPONENTS = 3"); return GP_E_ARG; }
@@ -555,6 +582,8 @@ int gp_validate_input(const gp_input *in, char *err, size_t errlen) {
   if (p->nphases > P || p->ncomp - 1 > NCM) { seterr(err, errlen, "too many phases/components"); return GP_E_ARG; }
   return GP_OK;
 }
+
+static double host_Mu(const gp_params *p, const double *c, double T, int a, int i);
 
 int gp_read_input(const char *input_path, gp_input *in, int quiet, char *err, size_t errlen) {
   memset(in, 0, sizeof(*in));
@@ -590,12 +619,28 @@ int gp_read_input(const char *input_path, gp_input *in, int quiet, char *err, si
       p->halo_src_y[f][3 + k] = in->bc_proxy[2][f][k];
     }
   }
+  /* T's boundary copies along y (boundary[.][2]); a side that copies nothing leaves its halo column as it is */
+  {
+    const int64_t ry = rowsy(in);
+    const int32_t hy[6] = {0, 1, 2, (int32_t)ry - 3, (int32_t)ry - 2, (int32_t)ry - 1};
+    for (int k = 0; k < 3; k++) {
+      p->halo_src_T_y[2 - k] = in->bc_proxy_set[3][2] ? in->bc_proxy[3][2][k] : hy[2 - k];
+      p->halo_src_T_y[3 + k] = in->bc_proxy_set[2][2] ? in->bc_proxy[2][2][k] : hy[3 + k];
+    }
+  }
+  p->shift = in->shift;
+  p->shiftj = in->shift ? in->shiftj : 0;
   in->max_length = (int)strlen(in->phases[0]);
   for (int a = 1; a < p->nphases; a++)
     if ((int)strlen(in->phases[a]) > in->max_length) in->max_length = (int)strlen(in->phases[a]);
   for (int k = 0; k < p->ncomp - 1; k++)
     if ((int)strlen(in->components[k]) > in->max_length) in->max_length = (int)strlen(in->components[k]);
   gp_property_matrices(in);
+  /* mu of the liquid rows the moving frame adds: Mu(cfill[L][L], Tfill, L) (GPs/functions/shift.h:24-38) */
+  {
+    const int L = p->nphases - 1;
+    for (int k = 0; k < p->ncomp - 1; k++) p->mu_fill[k] = host_Mu(p, in->cfill[L][L], in->Tfill, L, k);
+  }
   return GP_OK;
 }
 
@@ -628,10 +673,36 @@ static double host_hphi(const gp_params *p, const double *phi, int a) {
   return s + s1;
 }
 
-double gp_composition(const gp_params *p, const double *phi_cell, const double *mu_cell, int k) {
+static double composition_at(const gp_params *p, const double *phi_cell, const double *mu_cell, int k, double T) {
   double c = 0.0;
-  for (int b = 0; b < p->nphases; b++) c += host_c_mu(p, mu_cell, p->T, b, k) * host_hphi(p, phi_cell, b);
+  for (int b = 0; b < p->nphases; b++) c += host_c_mu(p, mu_cell, T, b, k) * host_hphi(p, phi_cell, b);
   return c;
+}
+
+double gp_composition(const gp_params *p, const double *phi_cell, const double *mu_cell, int k) {
+  return composition_at(p, phi_cell, mu_cell, k, p->T);
+}
+
+/* T(y) = temp_bottom + y GRADIENT with temp_bottom = base_temp - BASE_POS GRADIENT, BASE_POS = gradient_OFFSET/dy
+ * - shift_OFFSET (+ velocity/dy t dt after step t), GRADIENT = DeltaT dy / Distance (GPs/microsim_gp.c:77-80,
+ * 108-111; apply_temperature_gradientY, GPs/functions/Temperature_gradient.h:4-29); at t = 0 the boundary copies
+ * of T follow (GPs/microsim_gp.c:84) */
+void gp_temperature_rows(const gp_params *p, int64_t t, int64_t shift_offset, double *T_rows) {
+  const int64_t ry = gp_rows_y(p);
+  if (!p->tempgrady) {
+    for (int64_t y = 0; y < ry; y++) T_rows[y] = p->T;
+    return;
+  }
+  double base;
+  if (t == 0) base = (p->tg_offset / p->dy) - (double)shift_offset;
+  else base = (p->tg_offset / p->dy) - (double)shift_offset + ((p->tg_velocity / p->dy) * ((double)t * p->dt));
+  const double G = (p->tg_deltaT) * p->dy / (p->tg_distance);
+  const double bottom = p->tg_base_temp - base * G;
+  for (int64_t y = 0; y < ry; y++) T_rows[y] = bottom + (double)y * G;
+  if (t == 0) {
+    const int64_t hy[6] = {0, 1, 2, ry - 3, ry - 2, ry - 1};
+    for (int k = 0; k < 6; k++) T_rows[hy[k]] = T_rows[p->halo_src_T_y[k]];
+  }
 }
 
 void gp_composition_grid(const gp_params *p, const double *phi, const double *mu, double *c, int64_t ncells) {
@@ -837,10 +908,12 @@ static void put_be(FILE *fp, double v) {
   fwrite(o, 1, 8, fp);
 }
 
-int gp_write_vtk(const gp_input *in, const char *path, const double *phi, const double *mu) {
+int gp_write_vtk(const gp_input *in, const char *path, const double *phi, const double *mu, const double *T_rows) {
   const gp_params *p = &in->p;
   const int N = p->nphases, NC = p->ncomp - 1;
   const int64_t ry = gp_rows_y(p), n = gp_rows_x(p) * ry;
+  const int thermal = p->tempgrady != 0;
+  if (thermal && !T_rows) return GP_E_ARG;
   FILE *fp = fopen(path, in->ascii ? "w" : "wb");
   if (!fp) return GP_E_IO;
   fprintf(fp, "# vtk DataFile Version 3.0\n");
@@ -852,10 +925,13 @@ int gp_write_vtk(const gp_input *in, const char *path, const double *phi, const 
   fprintf(fp, "SPACING %le %le %le\n", p->dx, p->dy, in->dz);
   fprintf(fp, "POINT_DATA %ld\n", (long)p->mesh_x * (long)p->mesh_y);
   const int64_t x0 = 3, x1 = p->mesh_x + 2, y0 = 3, y1 = p->mesh_y + 2;
-  for (int f = 0; f < N + NC + (in->writecomposition ? NC : 0); f++) {
+  const int nc = in->writecomposition ? NC : 0;
+  for (int f = 0; f < N + NC + nc + thermal; f++) {
+    const int temp = f == N + NC + nc;
     if (f < N) fprintf(fp, "SCALARS %s double 1\n", in->phases[f]);
     else if (f < N + NC) fprintf(fp, "SCALARS Mu_%s double 1\n", in->components[f - N]);
-    else fprintf(fp, "SCALARS Composition_%s double 1\n", in->components[f - N - NC]);
+    else if (!temp) fprintf(fp, "SCALARS Composition_%s double 1\n", in->components[f - N - NC]);
+    else fprintf(fp, "SCALARS Temperature double 1\n");
     fprintf(fp, "LOOKUP_TABLE default\n");
     for (int64_t x = x0; x <= x1; x++)
       for (int64_t y = y0; y <= y1; y++) {
@@ -863,16 +939,18 @@ int gp_write_vtk(const gp_input *in, const char *path, const double *phi, const 
         double v;
         if (f < N) v = phi[(int64_t)f * n + idx];
         else if (f < N + NC) v = mu[(int64_t)(f - N) * n + idx];
+        else if (temp) v = T_rows[y];
         else {
           double pc[GP_MAXP], mc[GP_MAXNC];
           for (int a = 0; a < N; a++) pc[a] = phi[(int64_t)a * n + idx];
           for (int k = 0; k < NC; k++) mc[k] = mu[(int64_t)k * n + idx];
-          v = gp_composition(p, pc, mc, f - N - NC);
+          v = composition_at(p, pc, mc, f - N - NC, thermal ? T_rows[y] : p->T);
         }
         if (in->ascii) fprintf(fp, f < N + NC ? "%le\n" : "%le \n", v);
         else put_be(fp, v);
       }
-    if (in->ascii) fprintf(fp, "\n");
+    /* the Temperature block is the last and ends without an empty line (file_writer.h:142-147) */
+    if (in->ascii && !temp) fprintf(fp, "\n");
   }
   fclose(fp);
   return GP_OK;

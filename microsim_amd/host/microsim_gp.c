@@ -11,11 +11,16 @@
  * SAVET (plus the DATA/shift.dat line) and the progress block every TRACK_PROGRESS. The loop body
  * (GPs/microsim_gp.c:103-106) is gp_step; steps between two output events are issued as one batch.
  *
+ * Non-isothermal runs (ISOTHERMAL = 0 with Tempgrady) and the moving frame (Shift = 1) follow the same main: on a
+ * restart the frame position is read back from DATA/shift.dat and the gradient offset moved by the elapsed time and
+ * that position (GPs/microsim_gp.c:56-74); every snapshot carries the Temperature block and its DATA/shift.dat line
+ * the frame offset.
+ *
  * Environment: GP_PRECISION=fp32c selects the compensated FP32 mode, fp32 the plain FP32 mode (default fp64,
- * bit-compatible).
+ * bit-compatible; the non-isothermal and moving-frame runs are FP64 only).
  * Differences from the reference, by construction: the progress statistics are taken over interior cells
- * (the reference's sums also visit halo cells holding stale values), and unsupported inputs (TEMPGRADY,
- * SHIFT, noise, Dirichlet/free sides) stop with an error instead of running.
+ * (the reference's sums also visit halo cells holding stale values), and unsupported inputs (noise,
+ * Dirichlet/free sides) stop with an error instead of running.
  */
 #include <math.h>
 #include <stdio.h>
@@ -49,19 +54,39 @@ int main(int argc, char *argv[]) {
   const int64_t n = gp_rows_x(p) * gp_rows_y(p);
   double *phi = calloc((size_t)N * n, sizeof(double));
   double *mu = calloc((size_t)NC * n, sizeof(double));
-  if (!phi || !mu) die("host", "out of memory");
+  double *T_rows = calloc((size_t)gp_rows_y(p), sizeof(double));
+  if (!phi || !mu || !T_rows) die("host", "out of memory");
 
   char name[1200];
+  int64_t shift_position = 0;
   if (in.starttime == 0 && in.restart == 0) {
     if (gp_fill_domain(&in, argv[2], phi, mu, NULL, 0, err, sizeof err)) die("Filling", err);
   } else {
     snprintf(name, sizeof name, "DATA/%s_%ld.vtk", argv[3], (long)in.starttime);
     if (gp_read_vtk(&in, name, phi, mu)) die("restart", name);
+    if (in.shift) {
+      /* the position of line STARTTIME/SAVET (0-based) of DATA/shift.dat; the last line read when it is shorter */
+      FILE *fp = fopen("DATA/shift.dat", "r");
+      if (!fp) die("restart", "DATA/shift.dat");
+      long tf = 0, pos = 0;
+      for (int64_t it = 0; it <= in.starttime / (in.savet > 0 ? in.savet : 1); it++)
+        if (fscanf(fp, "%ld %ld\n", &tf, &pos) != 2) break;
+      fclose(fp);
+      shift_position = pos;
+    }
+    if (p->tempgrady) {
+      if (in.shift)
+        in.p.tg_offset = (in.p.tg_offset) + floor((in.p.tg_velocity) * (in.starttime * in.p.dt)) -
+                         (double)shift_position * in.p.dy;
+      else
+        in.p.tg_offset = (in.p.tg_offset) + floor((in.p.tg_velocity) * (in.starttime * in.p.dt));
+    }
   }
   gp_apply_bc_host(p, phi, mu);
+  gp_temperature_rows(p, 0, 0, T_rows);
   if (in.restart == 0 || in.starttime == 0) {
     snprintf(name, sizeof name, "DATA/%s_%ld.vtk", argv[3], (long)0);
-    if (gp_write_vtk(&in, name, phi, mu)) die("write", name);
+    if (gp_write_vtk(&in, name, phi, mu, T_rows)) die("write", name);
   }
 
   int precision = GP_FP64;
@@ -87,12 +112,14 @@ int main(int argc, char *argv[]) {
     if (rc) die("gp_step", gp_last_error(h));
     t = next;
     if (t % saveT == 0) {
+      int64_t shift_offset = 0;
       if (gp_download(h, phi, mu, NULL)) die("gp_download", gp_last_error(h));
+      if (gp_get_frame(h, &shift_offset, T_rows)) die("gp_get_frame", gp_last_error(h));
       snprintf(name, sizeof name, "DATA/%s_%ld.vtk", argv[3], (long)(t + in.starttime));
-      if (gp_write_vtk(&in, name, phi, mu)) die("write", name);
+      if (gp_write_vtk(&in, name, phi, mu, T_rows)) die("write", name);
       FILE *fp = fopen("DATA/shift.dat", "a");
       if (fp) {
-        fprintf(fp, "%ld %ld\n", (long)(t + in.starttime), (long)0);
+        fprintf(fp, "%ld %ld\n", (long)(t + in.starttime), (long)(shift_offset + shift_position));
         fclose(fp);
       }
     }
@@ -110,5 +137,6 @@ int main(int argc, char *argv[]) {
   gp_destroy(h);
   free(phi);
   free(mu);
+  free(T_rows);
   return 0;
 }

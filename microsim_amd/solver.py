@@ -84,6 +84,13 @@ class GPSolver:
                                          _abi.dptr(dphi) if with_dphi else None))
         return (phi, mu, dphi) if with_dphi else (phi, mu)
 
+    def frame(self) -> Tuple[int, np.ndarray]:
+        """(rows the moving frame has shifted since upload, temperature of every allocated y row)."""
+        off = C.c_int64()
+        T = np.empty(self.ry)
+        self._check(self.lib.gp_get_frame(self.h, C.byref(off), _abi.dptr(T)))
+        return int(off.value), T
+
     def profile_enable(self, on=True) -> None:
         """True / _abi.GP_PROFILE_LAUNCHES: events around every kernel launch; _abi.GP_PROFILE_CALLS: around every
         whole step / smooth call only (kind "call"); False: off."""

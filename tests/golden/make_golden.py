@@ -100,6 +100,19 @@ def cases():
 
     inp, fills, _ = C.config_ternary(96, 4, seed=5, steps=200, savet=1, min_spacing=30.0, gamma=1.0, driving=None)
     out["tern_c1_mini"] = (inp, fills, None, [0, 1, 10, 200])
+
+    # non-isothermal (ISOTHERMAL = 0: temperature gradient along y, reset every step) with the moving frame: a
+    # planar solid front in an undercooled melt, T = base_temp at the row gradient_OFFSET/dy + velocity t / dy;
+    # the front has passed Shiftj by step 100, so the frame moves there (by two rows)
+    inp = _bin_base(24, 64, bc_phi=(1, 1, 1, 1), fn_anisotropy=0, shift=1, shiftj=16,
+                    tempgrady=(0.97, 0.02, 40.0, 15.0, 0.5))
+    out["bin_tempgrad_shift"] = (inp, [C.Fill("cube", 0, (0, 0, 23, 14))], None, [0, 1, 2, 10, 99, 100, 101, 150,
+                                                                               200, 201, 300])
+
+    # non-isothermal ternary three-phase seeds without the moving frame, T dependence of c(mu), Mu and B(T), C(T)
+    inp, rng = _tern_base(40, 48, 41, bc_phi=(3, 3, 1, 1), tempgrady=(0.985, 0.03, 30.0, 10.0, 0.4))
+    fills = [C.Fill("cylinder", 0, (12, 14, 7.0)), C.Fill("cylinder", 1, (28, 30, 7.0))]
+    out["tern_tempgrad"] = (inp, fills, None, [0, 1, 10, 100])
     return out
 
 
@@ -122,13 +135,19 @@ def run_case(name, inp, fills, restart, steps, dest):
             shutil.copy(os.path.join(work, fn), os.path.join(dest, fn))
         shutil.copy(os.path.join(work, "DATA", "out_0.vtk"), os.path.join(dest, "out_0.vtk"))
         N, K = inp.nphases, inp.ncomp
-        phis, mus = [], []
+        phis, mus, temps = [], [], []
         for s in steps:
-            p, m = C.read_vtk(os.path.join(work, "DATA", "out_%d.vtk" % s), N, K, inp.mesh_x, inp.mesh_y)
+            path = os.path.join(work, "DATA", "out_%d.vtk" % s)
+            p, m = C.read_vtk(path, N, K, inp.mesh_x, inp.mesh_y)
             phis.append(p)
             mus.append(m)
+            if inp.tempgrady is not None:
+                temps.append(C.read_vtk_fields(path, inp.mesh_x, inp.mesh_y)["Temperature"])
+        extra = {"T": np.stack(temps)} if temps else {}
         np.savez_compressed(os.path.join(dest, "ref.npz"), steps=np.array(steps, dtype=np.int64),
-                            phi=np.stack(phis), mu=np.stack(mus))
+                            phi=np.stack(phis), mu=np.stack(mus), **extra)
+        if os.path.exists(os.path.join(work, "DATA", "shift.dat")):
+            shutil.copy(os.path.join(work, "DATA", "shift.dat"), os.path.join(dest, "shift.dat"))
         if restart is not None:
             np.savez_compressed(os.path.join(dest, "restart0.npz"), phi=restart[0], mu=restart[1])
         with open(os.path.join(dest, "stdout.txt"), "w") as f:

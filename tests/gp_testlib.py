@@ -127,8 +127,27 @@ def oracle_smooth(params, phi, mu, nsteps, threads=0):
     assert oracle().oracle_smooth(C.byref(params), _abi.dptr(phi), _abi.dptr(mu), nsteps, threads) == 0
 
 
-def golden_cases():
-    return sorted(os.path.basename(os.path.dirname(p)) for p in glob.glob(os.path.join(GOLDEN, "*", "ref.npz")))
+def _frame_deck(d):
+    """Whether a golden deck is non-isothermal (ISOTHERMAL = 0) or runs the moving frame (Shift = 1)."""
+    keys = {}
+    for line in open(os.path.join(d, "Input.in")):
+        if "=" in line:
+            k, v = line.split("=", 1)
+            keys[k.strip()] = v.strip().rstrip(";").strip()
+    return keys.get("ISOTHERMAL", "1") == "0" or keys.get("Shift", "0") == "1"
+
+
+def golden_cases(frame: bool = False):
+    """Golden decks: the isothermal fixed-frame ones, or with frame=True also the non-isothermal / moving-frame
+    ones (which the FP64 device path, the host stage and the driver cover; the oracle, the FP32 modes and the
+    adjoint do not take them)."""
+    dirs = sorted(os.path.dirname(p) for p in glob.glob(os.path.join(GOLDEN, "*", "ref.npz")))
+    return [os.path.basename(d) for d in dirs if frame or not _frame_deck(d)]
+
+
+def frame_cases():
+    """The non-isothermal / moving-frame golden decks."""
+    return [n for n in golden_cases(frame=True) if n not in golden_cases()]
 
 
 def load_golden(name):
@@ -145,7 +164,10 @@ def load_golden(name):
         configs.write_vtk(restart, r["phi"], r["mu"], ["p%d" % i for i in range(inp.p.nphases)],
                           ["c%d" % i for i in range(inp.p.ncomp)], binary=not inp.ascii)
     case = host.load_case(os.path.join(d, "Input.in"), os.path.join(d, "Filling.in"), restart)
-    return case, {"steps": list(z["steps"]), "phi": z["phi"], "mu": z["mu"]}
+    ref = {"steps": list(z["steps"]), "phi": z["phi"], "mu": z["mu"]}
+    if "T" in z.files:
+        ref["T"] = z["T"]
+    return case, ref
 
 
 def max_rel_err(a, b, floor=1e-300):

@@ -77,13 +77,13 @@ def _copy_case(name):
     return d
 
 
-@pytest.mark.parametrize("name", golden_cases())
+@pytest.mark.parametrize("name", golden_cases(frame=True))
 def test_initial_snapshot_matches_reference_bytes(name):
     case, _ = load_golden(name)
     d = tempfile.mkdtemp(prefix="gp_vtk_")
     try:
         out = os.path.join(d, "out_0.vtk")
-        host.write_vtk(case.inp, out, case.phi, case.mu)
+        host.write_vtk(case.inp, out, case.phi, case.mu, host.temperature_rows(case.inp.p))
         got = open(out, "rb").read()
         want = open(os.path.join(GOLDEN, name, "out_0.vtk"), "rb").read()
         assert got == want, "%s: %d vs %d bytes" % (name, len(got), len(want))
@@ -91,7 +91,7 @@ def test_initial_snapshot_matches_reference_bytes(name):
         shutil.rmtree(d, ignore_errors=True)
 
 
-@pytest.mark.parametrize("name", golden_cases())
+@pytest.mark.parametrize("name", golden_cases(frame=True))
 def test_echo_files_match_reference(name):
     d = _copy_case(name)
     try:
