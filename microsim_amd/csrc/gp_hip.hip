@@ -242,6 +242,59 @@ static int planes_to_host(gp_handle *h, double *dst, const double *src, int ncom
   return GP_OK;
 }
 
+/* GP_ADJOINT_CHECK: the smallest index of a non-finite value of g[0, n) into out[0] (atomicMin; stays at ~0 when
+ * there is none) and the largest |g| among the finite ones, as the bits of a non-negative double, into out[1] */
+static __global__ __launch_bounds__(256) void k_adj_scan(const double *__restrict__ g, int64_t n,
+                                                         unsigned long long *out) {
+  unsigned long long bad = ~0ull, mx = 0ull;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const double v = g[i];
+    if (!isfinite(v)) bad = (unsigned long long)i < bad ? (unsigned long long)i : bad;
+    else {
+      const unsigned long long b = (unsigned long long)__double_as_longlong(fabs(v));
+      mx = b > mx ? b : mx;
+    }
+  }
+  if (bad != ~0ull) atomicMin(&out[0], bad);
+  if (mx) atomicMax(&out[1], mx);
+}
+
+/* after transposing forward step t (state t -> t + 1): stop with GP_E_NAN at the first non-finite adjoint or theta
+ * gradient, naming the cell and the forward state there; every `every` steps print the largest |adjoint| */
+static int adj_check(gp_handle *h, int64_t t, int64_t every, const double *phi64, const double *mu64) {
+  Adjoint *A = h->adj;
+  const int nth = theta_count(h->N, h->NC);
+  if (!A->scan) HIPCHK(h, hipMalloc((void **)&A->scan, 4 * sizeof(unsigned long long)));
+  const unsigned long long init[4] = {~0ull, 0ull, ~0ull, 0ull};
+  HIPCHK(h, hipMemcpyAsync(A->scan, init, sizeof init, hipMemcpyHostToDevice, h->stream));
+  const int64_t se = (int64_t)state_elems(h);
+  hipLaunchKernelGGL(k_adj_scan, dim3(1024), dim3(256), 0, h->stream, (const double *)A->g[1], se, A->scan);
+  hipLaunchKernelGGL(k_adj_scan, dim3(1), dim3(256), 0, h->stream, (const double *)A->gth, (int64_t)nth, A->scan + 2);
+  HIPCHK(h, hipGetLastError());
+  unsigned long long r[4];
+  HIPCHK(h, hipMemcpyAsync(r, A->scan, sizeof r, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  double gmax;
+  memcpy(&gmax, &r[1], sizeof gmax);
+  if (every > 0 && t % every == 0) fprintf(stderr, "gp adjoint check: step %lld max |adjoint| %.6e\n", (long long)t, gmax);
+  if (r[0] == ~0ull && r[2] == ~0ull) return GP_OK;
+  if (r[0] == ~0ull)
+    return gp_fail(h, GP_E_NAN, "adjoint check: theta gradient %llu non-finite after transposing step %lld",
+                   (unsigned long long)r[2], (long long)t);
+  const int64_t comp = (int64_t)(r[0] / (unsigned long long)h->fs), c = (int64_t)(r[0] % (unsigned long long)h->fs);
+  const int64_t x = (c - kYOff) / h->ps, y = (c - kYOff) % h->ps;
+  double ph[P] = {0}, mv[NCM] = {0};
+  for (int a = 0; a < h->N; a++)
+    HIPCHK(h, hipMemcpy(&ph[a], phi64 + (size_t)a * h->fs + c, sizeof(double), hipMemcpyDeviceToHost));
+  for (int k = 0; k < h->NC; k++)
+    HIPCHK(h, hipMemcpy(&mv[k], mu64 + (size_t)k * h->fs + c, sizeof(double), hipMemcpyDeviceToHost));
+  return gp_fail(h, GP_E_NAN,
+                 "adjoint check: non-finite adjoint of %s %lld at cell (x %lld, y %lld) after transposing step %lld "
+                 "(largest finite |adjoint| %.3e); state there: phi %.17g %.17g %.17g %.17g, mu %.17g %.17g %.17g",
+                 comp < h->N ? "phi" : "mu", (long long)(comp < h->N ? comp : comp - h->N), (long long)x,
+                 (long long)y, (long long)t, gmax, ph[0], ph[1], ph[2], ph[3], mv[0], mv[1], mv[2]);
+}
+
 /* fold the halo adjoints of A->g[1] into their source cells */
 static int adj_fold(gp_handle *h) {
   Adjoint *A = h->adj;
@@ -794,6 +847,11 @@ int gp_adjoint_backward(gp_handle *h, const double *gphi_T, const double *gmu_T,
   adj_fn fa = h->ops->adj_step;
   widen_fn fw = h->ops->widen;
   const bool wide = h->precision == GP_FP64;
+  /* GP_ADJOINT_CHECK=<n>: scan the adjoint after every transposed step (one synchronisation each) and print its
+   * largest magnitude every n steps */
+  const char *ck = getenv("GP_ADJOINT_CHECK");
+  const bool check = ck && *ck && strcmp(ck, "0") != 0;
+  const int64_t check_every = check ? atoll(ck) : 0;
   for (int64_t k = (int64_t)A->ckpt.size() - 1; k >= 0; k--) {
     const int64_t n = std::min(A->every, A->nsteps - k * A->every);
     if ((rc = state_load(h, A->ckpt[k]))) return rc;
@@ -812,6 +870,11 @@ int gp_adjoint_backward(gp_handle *h, const double *gphi_T, const double *gmu_T,
         rc = fa(h, A->st64, A->st64 + (size_t)h->N * h->fs, A->dphi64, A->model64);
       }
       if (rc) return rc;
+      if (check) {
+        const double *p64 = wide ? (const double *)h->phi[h->cur] : A->st64;
+        const double *m64 = wide ? (const double *)h->mu[h->cur] : A->st64 + (size_t)h->N * h->fs;
+        if ((rc = adj_check(h, k * A->every + j, check_every, p64, m64))) return rc;
+      }
     }
   }
   if ((rc = adj_fold(h))) return rc;

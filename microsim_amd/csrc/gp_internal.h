@@ -170,10 +170,11 @@ struct Adjoint {
   double *st64 = nullptr, *dphi64 = nullptr;
   void *model64 = nullptr;
   int *counts = nullptr;                      /* their lengths */
+  unsigned long long *scan = nullptr;         /* GP_ADJOINT_CHECK: first non-finite index, max |g| bits, theta flag */
   ~Adjoint() {
     for (double *b : ckpt) (void)hipFree(b);
     for (double *b : seg) (void)hipFree(b);
-    void *bufs[] = {g[0], g[1], gdphi, gth, cbar, qlist, plist, counts, st64, dphi64, model64};
+    void *bufs[] = {g[0], g[1], gdphi, gth, cbar, qlist, plist, counts, st64, dphi64, model64, scan};
     for (void *b : bufs)
       if (b) (void)hipFree(b);
   }

@@ -469,10 +469,12 @@ GP_HD void project_simplex_vjp(const double *phi, const double *divphi, const do
         gphi[b] -= g[b];
         g[b] = 0.0;
       }
-  /* C: d2 = d1 - delta phi_b / sum_phib (or delta / count) */
+  /* C: d2 = d1 - delta phi_b / sum_phib (or delta / count). A value that D overwrote has no adjoint left and
+   * contributes nothing; skipping it also keeps 0 / count = 0 / 0 out when no phase lay strictly inside (0, 1)
+   * before the clamping (count = 0: the forward's delta / count is an infinity that D then replaces by -phi). */
   double gdelta = 0.0, gsum = 0.0;
   for (int b = 0; b < N; b++)
-    if (inC[b]) {
+    if (inC[b] && g[b] != 0.0) {
       if (wsum) {
         gdelta -= g[b] * phi[b] / sum_phib;
         gphi[b] -= g[b] * delta / sum_phib;

@@ -64,6 +64,8 @@ def model_lib():
         lib.model_vjp.argtypes = [C.POINTER(_abi.GPParams), _abi._dp, _abi._dp, _abi._dp, _abi._dp, _abi._dp,
                                   _abi._dp, _abi._dp, C.c_int64, C.c_int]
         lib.model_step_c32.argtypes = [C.POINTER(_abi.GPParams), _abi._dp, _abi._dp, C.c_int64, C.c_int]
+        lib.cell_check.argtypes = [C.POINTER(_abi.GPParams), _abi._dp, _abi._dp, _abi._dp, C.c_int64, C.c_int,
+                                   _abi._dp, _abi._dp, _abi._dp]
         lib.c32_roundtrip.argtypes = [_abi._dp, C.c_int64, C.c_void_p, C.c_void_p, _abi._dp]
         lib.c32_roundtrip.restype = None
         lib.df_roundtrip.argtypes = [_abi._dp, C.c_int64, C.c_void_p, C.c_void_p, _abi._dp]
@@ -85,6 +87,18 @@ def model_vjp(params, phi, mu, wphi, wmu, nsteps):
                                _abi.dptr(m0), _abi.dptr(th), nsteps, 0)
     assert rc == 0
     return g0, m0, th
+
+
+def cell_check(params, phi, mu, dphi, c, stage, w):
+    """(vjp, jvp) of one cell's stage (0: phi_delta, 1: currents and update) at flat index c of a halo-inclusive
+    state: the reverse-mode transposed Jacobian applied to w, and the same products column by column in dual
+    numbers (tests/cpu_model/model_step.cpp cell_check)."""
+    vjp, jvp = np.zeros(512), np.zeros(512)
+    n = model_lib().cell_check(C.byref(params), _abi.dptr(np.ascontiguousarray(phi)), _abi.dptr(np.ascontiguousarray(mu)),
+                               _abi.dptr(np.ascontiguousarray(dphi)), int(c), int(stage),
+                               _abi.dptr(np.ascontiguousarray(w, dtype=np.float64)), _abi.dptr(vjp), _abi.dptr(jvp))
+    assert n > 0
+    return vjp[:n], jvp[:n]
 
 
 def pow2_spacing(params) -> bool:

@@ -146,3 +146,34 @@ def test_reverse_mode_is_the_transposed_tangent(name):
         assert got == pytest.approx(want, rel=1e-11, abs=1e-13 * (np.abs(wphi * tp).sum() + np.abs(wmu * tm).sum()))
     # the halo entries of the initial-state gradient are zero (they are copies of the interior)
     assert not g0[:, :3].any() and not g0[:, -3:].any() and not g0[:, :, :3].any() and not m0[:, :, -3:].any()
+
+
+def test_reverse_projection_when_no_phase_lies_inside():
+    """A cell where the projection clamps a vanishing phase while no active phase lay strictly inside (0, 1) before
+    the clamping (simplex_projection.h: count = 0, so delta / count is an infinity that the phase pushed above 1 then
+    overwrites with -phi). The forward value is finite; the transposed projection must be too and must equal the
+    dual-number tangent. The 3x3 neighbourhood is from a ternary 2048^2 seed run at step 3702, where a phase-1 grain
+    meets the liquid with phase 0 absent."""
+    import os
+    import tempfile
+    from microsim_amd import configs, host
+    inp, fills, _ = configs.config_ternary(58, 1, seed=1234, min_spacing=30.0)
+    d = tempfile.mkdtemp(prefix="gp_proj_")
+    configs.write_case(d, inp, fills)
+    case = host.load_case(os.path.join(d, "Input.in"), os.path.join(d, "Filling.in"))
+    phi, mu = case.phi.copy(), case.mu.copy()
+    phi[0, 32:35, 32:35] = 0.0
+    phi[1, 32:35, 32:35] = [[0.9719499370683231, 1.0, 1.0], [0.9593802331723985, 0.9999994421461349, 1.0],
+                            [0.9391134117412449, 0.9906544827095958, 1.0]]
+    phi[2, 32:35, 32:35] = [[0.028050062931676667, 0.0, 0.0], [0.04061976682760295, 5.578538646156249e-07, 0.0],
+                            [0.06088658825875734, 0.009345517290404339, 0.0]]
+    mu[:, 33, 33] = [0.3959179828929696, 0.5021716343457124]
+    ry = phi.shape[2]
+    dphi = np.zeros_like(phi)
+    for o in range(case.N):
+        w = np.zeros(case.N)
+        w[o] = 1.0
+        vjp, jvp = gp_testlib.cell_check(case.params, phi, mu, dphi, 33 * ry + 33, 0, w)
+        assert np.isfinite(jvp).all()
+        assert np.isfinite(vjp).all(), "output %d: non-finite transposed projection" % o
+        np.testing.assert_allclose(vjp, jvp, rtol=1e-12, atol=1e-14 * max(1.0, np.abs(jvp).max()))
