@@ -19,7 +19,7 @@ def test_calibration_descends(precision):
 
 
 @pytest.mark.gpu
-def test_calibration_config4_scale_descends():
+def test_calibration_2048_20000_steps_descends():
     """The calibration workload at its configured scale: ternary three-phase 2048^2 (256 seeds), 20 000 steps with
     checkpoints every ceil(sqrt(20000)) = 142 steps, compensated FP32 trajectory with FP64 gradient accumulation;
     one descent iteration (a 20 000-step gradient, about 70 s) lowers the loss and the largest parameter error from
