@@ -81,7 +81,11 @@ int64_t gp_rows_x(const gp_params *p);
 int64_t gp_rows_y(const gp_params *p);
 
 /* fill_domain(argv) into halo-inclusive SoA arrays phi[N][rows], mu[NC][rows] (rows = rows_x*rows_y).
- * dphi may be NULL. */
+ * dphi may be NULL. Filling.in keys: the serial driver's FILLCUBE, FILLCYLINDER, FILLSPHERE, FILLELLIPSE
+ * (GPs/functions/fill_domain.h) and the MPI driver's FILLCYLINDERRANDOM, FILLSPHERERANDOM, FILLVORONOI2D,
+ * FILLCUBEPATTERN, FILLCUBERANDOM (GPm/functions/fill_domain.h:146-332). The drand48-driven ones start from the
+ * POSIX initial state as the reference does; the GSL-driven ones seed MT19937 with GP_FILL_SEED when that
+ * environment variable is set, else with time(0) as the reference does. */
 int gp_fill_domain(const gp_input *in, const char *filling_path, double *phi, double *mu, double *dphi,
                    int quiet, char *err, size_t errlen);
 

@@ -15,6 +15,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #define P GP_MAXP
 #define NCM GP_MAXNC
@@ -746,6 +747,307 @@ static void fill_solid_cell(grid_t *g, int64_t x, int64_t y, int b, int inside) 
   }
 }
 
+/* ------------------------------------------------------------------------------------------------------------
+ * The MPI driver's random and pattern fillers (GPm/functions/fill_domain.h:146-332, GPm/functions/filling.h).
+ * They work on the allocated grid (rows_x x rows_y, halo included) with centres and boxes in allocated
+ * coordinates, as the single-rank MPI driver does.
+ * ---------------------------------------------------------------------------------------------------------- */
+
+/* The POSIX drand48 generator: X(n+1) = (0x5DEECE66D X(n) + 0xB) mod 2^48. The reference never seeds it, so its runs
+ * start from the POSIX initial state 0x1234ABCD330E and draw the same sequence every time; drand48 = X / 2^48,
+ * lrand48 = X >> 17. */
+typedef struct { uint64_t x; } rng48_t;
+static void rng48_init(rng48_t *r) { r->x = 0x1234ABCD330EULL; }
+static uint64_t rng48_next(rng48_t *r) {
+  r->x = (0x5DEECE66DULL * r->x + 0xBULL) & 0xFFFFFFFFFFFFULL;
+  return r->x;
+}
+static double rng48_drand(rng48_t *r) { return (double)rng48_next(r) / 281474976710656.0; }
+static long rng48_lrand(rng48_t *r) { return (long)(rng48_next(r) >> 17); }
+
+/* MT19937 as GSL's default generator runs it (Matsumoto and Nishimura 1998, with the Knuth seeding
+ * mt[i] = 1812433253 (mt[i-1] ^ (mt[i-1] >> 30)) + i, seed 0 taken as 4357): uniform = draw / 2^32,
+ * uniform_int(n) = draw / floor((2^32 - 1) / n), redrawn while >= n. The reference seeds it with time(0). */
+typedef struct { uint32_t mt[624]; int mti; } mt_t;
+static void mt_seed(mt_t *s, unsigned long seed) {
+  if (seed == 0) seed = 4357;
+  s->mt[0] = (uint32_t)(seed & 0xffffffffUL);
+  for (int i = 1; i < 624; i++) s->mt[i] = (uint32_t)(1812433253UL * (s->mt[i - 1] ^ (s->mt[i - 1] >> 30)) + (unsigned)i);
+  s->mti = 624;
+}
+static uint32_t mt_next(mt_t *s) {
+  if (s->mti >= 624) {
+    for (int k = 0; k < 624; k++) {
+      const uint32_t y = (s->mt[k] & 0x80000000U) | (s->mt[(k + 1) % 624] & 0x7fffffffU);
+      s->mt[k] = s->mt[(k + 397) % 624] ^ (y >> 1) ^ ((y & 1U) ? 0x9908b0dfU : 0U);
+    }
+    s->mti = 0;
+  }
+  uint32_t y = s->mt[s->mti++];
+  y ^= y >> 11;
+  y ^= (y << 7) & 0x9d2c5680U;
+  y ^= (y << 15) & 0xefc60000U;
+  y ^= y >> 18;
+  return y;
+}
+static double mt_uniform(mt_t *s) { return mt_next(s) / 4294967296.0; }
+static unsigned long mt_uniform_int(mt_t *s, unsigned long n) {
+  const unsigned long scale = 0xffffffffUL / n;
+  unsigned long k;
+  do k = mt_next(s) / scale; while (k >= n);
+  return k;
+}
+
+/* seed of the GSL-driven fillers: GP_FILL_SEED when set (reproducible runs), else time(0) as in the reference */
+static unsigned long fill_seed(void) {
+  const char *e = getenv("GP_FILL_SEED");
+  return e && *e ? strtoul(e, NULL, 10) : (unsigned long)time(NULL);
+}
+
+/* fill_phase_cube (GPm/functions/filling.h:22-71) over the allocated grid for a solid b (the matrix pass is separate) */
+static void fill_cube_solid(grid_t *g, int b, long x0, long x1, long y0, long y1, long z0, long z1) {
+  for (int64_t x = 0; x < g->rx; x++)
+    for (int64_t y = 0; y < g->ry; y++)
+      fill_solid_cell(g, x, y, b, x >= x0 && x <= x1 && y >= y0 && y <= y1 && 0 >= z0 && 0 <= z1);
+}
+
+static void fill_liquid_all(grid_t *g, int quiet) {
+  for (int64_t x = 0; x < g->rx; x++)
+    for (int64_t y = 0; y < g->ry; y++) fill_liquid_cell(g, x, y, quiet);
+}
+
+/* the matrix (last phase) set to 1 on every allocated cell, the other phases left as they are */
+static void fill_matrix_one(grid_t *g) {
+  for (int64_t x = 0; x < g->rx; x++)
+    for (int64_t y = 0; y < g->ry; y++) *PH(g, g->N - 1, x, y) = 1.0;
+}
+
+/* fill_phase_cylinder_random (GPm/functions/filling.h:632-698) with checkOverlap (:7-20) and
+ * fill_phase_cylinder_occupancy (:579-631); z = 0 only */
+static void fill_cylinders_random(grid_t *g, rng48_t *r, int64_t MX, int64_t MY, long phase, long ppt_radius,
+                                  double vf, long shield, double spread, int quiet) {
+  int *occ = (int *)calloc((size_t)g->n, sizeof(int));
+  const double vol = (double)MX * MY * 1;
+  const double vpp = (double)(ppt_radius * ppt_radius) * M_PI;
+  const int nparticles = (int)ceil(vol * vf / vpp);
+  if (!quiet) printf("Domain volume = %lf, ppt_radius = %le, volume_per_particle = %lf, Number of particles = %d\n", vol,
+                     (double)ppt_radius, vpp, nparticles);
+  long idx = 1, count = 0;
+  const long limit = 10000;
+  while (idx <= nparticles) {
+    const long cx = (long)(MX * rng48_drand(r)), cy = (long)(MY * rng48_drand(r));
+    const double mdev = spread * ppt_radius;
+    const double rad = (double)ppt_radius + (1.0 * rng48_drand(r) - 0.5) * mdev;
+    if (++count > limit) {
+      if (!quiet) printf("Random filling attempt count limit = %ld reached. Only %ld particles could be filled.\n", limit, idx - 1);
+      break;
+    }
+    if (((cx - rad) < 0.0) || ((cx + rad) > (MX - 1)) || ((cy - rad) < 0.0) || ((cy + rad) > (MY - 1))) continue;
+    int overlap = 0;
+    if (shield)
+      for (int64_t x = 0; x < g->rx && !overlap; x++)
+        for (int64_t y = 0; y < g->ry; y++)
+          if (occ[x * g->ry + y] == 1 && (x - cx) * (x - cx) + (y - cy) * (y - cy) + 0 <= (shield * rad) * (shield * rad)) {
+            overlap = 1;
+            break;
+          }
+    if (overlap) continue;
+    for (int64_t x = 0; x < g->rx; x++)
+      for (int64_t y = 0; y < g->ry; y++) {
+        const int in = ((x - cx) * (x - cx) + (y - cy) * (y - cy)) <= rad * rad;
+        fill_solid_cell(g, x, y, (int)phase, in);
+        if (in) occ[x * g->ry + y] = 1;
+      }
+    idx++;
+  }
+  fill_liquid_all(g, quiet);
+  free(occ);
+}
+
+/* fill_phase_sphere_random (GPm/functions/filling.h:516-577) with fill_phase_sphere_occupancy (:463-514) on the
+ * plane z = 0. The centre's z is MESH_Z drand48() = 0, so with a positive radius every attempt fails the z-extent
+ * test (0 - rad < 0) and no sphere is placed on a 2-D grid; the attempts still draw four values each, up to the
+ * 1e5 limit, before the matrix pass. */
+static void fill_spheres_random(grid_t *g, rng48_t *r, int64_t MX, int64_t MY, long phase, long ppt_radius,
+                                double vf, long shield, double spread, int quiet) {
+  int *occ = (int *)calloc((size_t)g->n, sizeof(int));
+  const double vol = (double)MX * MY * 1;
+  const double vpp = (double)(ppt_radius * ppt_radius * ppt_radius) * (4.0 / 3.0) * M_PI;
+  const int nparticles = (int)ceil(vol * vf / vpp);
+  if (!quiet) printf("Domain volume = %lf, ppt_radius = %le, volume_per_particle = %lf, Number of particles = %d\n", vol,
+                     (double)ppt_radius, vpp, nparticles);
+  long idx = 1, count = 0;
+  const long limit = 100000;
+  while (idx <= nparticles) {
+    const long cx = (long)(MX * rng48_drand(r)), cy = (long)(MY * rng48_drand(r)), cz = (long)(1 * rng48_drand(r));
+    const double mdev = spread * ppt_radius;
+    const double rad = (double)ppt_radius + (1.0 * rng48_drand(r) - 0.5) * mdev;
+    if (++count > limit) {
+      if (!quiet) printf("Random filling attempt count limit = %ld reached. Only %ld particles could be filled.\n", limit, idx - 1);
+      break;
+    }
+    if (((cx - rad) < 0.0) || ((cx + rad) > (MX - 1)) || ((cy - rad) < 0.0) || ((cy + rad) > (MY - 1)) ||
+        ((cz - rad) < 0.0) || ((cz + rad) > (1 - 1)))
+      continue;
+    int overlap = 0;
+    if (shield)
+      for (int64_t x = 0; x < g->rx && !overlap; x++)
+        for (int64_t y = 0; y < g->ry; y++)
+          if (occ[x * g->ry + y] == 1 &&
+              (x - cx) * (x - cx) + (y - cy) * (y - cy) + (0 - cz) * (0 - cz) <= (shield * rad) * (shield * rad)) {
+            overlap = 1;
+            break;
+          }
+    if (overlap) continue;
+    for (int64_t x = 0; x < g->rx; x++)
+      for (int64_t y = 0; y < g->ry; y++) {
+        const int in = ((x - cx) * (x - cx) + (y - cy) * (y - cy) + (0 - cz) * (0 - cz)) <= rad * rad;
+        fill_solid_cell(g, x, y, (int)phase, in);
+        if (in) occ[x * g->ry + y] = 1;
+      }
+    idx++;
+  }
+  fill_liquid_all(g, quiet);
+  free(occ);
+}
+
+/* fill_phase_voronoi_2D (GPm/functions/filling.h:700-781) over [x0, x1) x [y0, y1). A point that lands inside the
+ * exclusion disc of an earlier one keeps the reference's uninitialised coordinates there; here it is left out. The
+ * nearest-point search keeps the reference's start (location 0, not the first point's phase). */
+static int fill_voronoi(grid_t *g, rng48_t *r, long x0, long x1, long y0, long y1, long npts, double size_min) {
+  const int N = g->N;
+  const long lx = x1 - x0, ly = y1 - y0;
+  if (npts <= 0 || lx <= 0 || ly <= 0 || x0 < 0 || y0 < 0 || x1 > g->rx || y1 > g->ry) return GP_E_ARG;
+  double *px = (double *)malloc(sizeof(double) * (size_t)npts), *py = (double *)malloc(sizeof(double) * (size_t)npts);
+  int *valid = (int *)calloc((size_t)npts, sizeof(int));
+  long *ph = (long *)malloc(sizeof(long) * (size_t)npts);
+  char *flag = (char *)calloc((size_t)g->n, 1);
+  for (long k = 0; k < npts; k++) {
+    const long rx = (long)(x0 + rng48_drand(r) * lx);
+    const long ry = (long)(y0 + rng48_drand(r) * ly);
+    if (flag[rx * g->ry + ry] != 1) {
+      px[k] = (double)rx;
+      py[k] = (double)ry;
+      valid[k] = 1;
+      for (long x = x0; x < x1; x++)
+        for (long y = y0; y < y1; y++)
+          if ((py[k] - y) * (py[k] - y) + (px[k] - x) * (px[k] - x) <= size_min * size_min) flag[x * g->ry + y] = 1;
+    }
+  }
+  for (long k = 0; k < npts; k++) ph[k] = rng48_lrand(r) % (N - 1);
+  for (long x = x0; x < x1; x++)
+    for (long y = y0; y < y1; y++) {
+      double minimum = INFINITY;
+      long location = 0, first = 1;
+      for (long k = 0; k < npts; k++) {
+        if (!valid[k]) continue;
+        const double l = (px[k] - x) * (px[k] - x) + (py[k] - y) * (py[k] - y);
+        if (first) { /* l[0] starts the search with location 0 */
+          minimum = l;
+          first = 0;
+          if (k == 0) continue;
+        }
+        if (l < minimum) {
+          minimum = l;
+          location = ph[k];
+        }
+      }
+      for (int s = 0; s < N - 1; s++) *PH(g, s, x, y) = location == s ? 1.0 : 0.0;
+    }
+  free(px);
+  free(py);
+  free(valid);
+  free(ph);
+  free(flag);
+  return GP_OK;
+}
+
+/* fill_cube_pattern (GPm/functions/filling.h:73-169): a lattice of boxes of size s +- sfrac s at pitch s + gap, each
+ * kept with probability 1 - 1/outof, of a random variant among the first `variants` phases */
+static void fill_cube_pattern(grid_t *g, mt_t *rng, int64_t MX, int64_t MY, long variants, long sx, long sy,
+                              double sfrac, long gap, double gfrac, long outof, int quiet) {
+  const int N = g->N;
+  if (N > 1) fill_matrix_one(g);
+  ldiv_t rx = ldiv((long)MX, sx + gap), ry = ldiv((long)MY, sy + gap);
+  if (rx.quot == 0) rx.quot = 1;
+  if (ry.quot == 0) ry.quot = 1;
+  if (!quiet) printf("Filling %ld particles.\n", rx.quot * ry.quot * 1);
+  long xlo = 0, xhi = 1, ylo = 0, yhi = 1;
+  for (long i = 0; i < rx.quot; i++) {
+    if (MX > 1) {
+      const double r = mt_uniform(rng);
+      const long sgn = 2 * lround(r) - 1;
+      xlo = (long)(i * (gap + sx) + gap + sgn * gfrac * gap - (2 * r - 1) * sfrac * sx);
+      xhi = (long)((i + 1) * (gap + sx) + sgn * gfrac * gap + (2 * r - 1) * sfrac * sx);
+    }
+    for (long j = 0; j < ry.quot; j++) {
+      if (MY > 1) {
+        const double r = mt_uniform(rng);
+        const long sgn = 2 * lround(r) - 1;
+        ylo = (long)(j * (gap + sy) + gap + sgn * gfrac * gap - (2 * r - 1) * sfrac * sy);
+        yhi = (long)((j + 1) * (gap + sy) + sgn * gfrac * gap + (2 * r - 1) * sfrac * sy);
+      }
+      if (outof > 0 && !mt_uniform_int(rng, (unsigned long)outof)) continue;
+      if (variants >= N) variants = N - 1;
+      const long phase = (long)(mt_uniform(rng) * variants);
+      if (!quiet) printf("phase now = %ld\n", phase);
+      fill_cube_solid(g, (int)phase, xlo, xhi, ylo, yhi, 0, 1);
+    }
+  }
+}
+
+/* fill_phase_cube_random_variants (GPm/functions/filling.h:204-295) with check_overlap_sq (:171-202): boxes of size
+ * s +- sfrac s at uniform positions, skipped when the shielded box meets a cell whose matrix fraction is not
+ * positive, until vf of the domain is covered. The reference retries without bound; here the attempts stop after
+ * 1e7 with an error. */
+static int fill_cubes_random(grid_t *g, mt_t *rng, int64_t MX, int64_t MY, long variants, long sx, long sy, long sz,
+                             double sfrac, double vf, long shield, int quiet) {
+  const int N = g->N;
+  if (N > 1) fill_matrix_one(g);
+  const long vol = (long)(MX * MY * 1);
+  const double vpp = (double)(sx * sy * sz);
+  int nparticles = (int)ceil(vol * vf / vpp);
+  if (!quiet) printf("Domain volume = %ld, sx = %ld, sy = %ld, sz = %ld, volume_per_particle = %lf, number of particles = %d\n",
+                     vol, sx, sy, sz, vpp, nparticles);
+  long count = 0, attempts = 0;
+  const long limit = 10000;
+  while (nparticles) {
+    if (++attempts > 10000000L) return GP_E_ARG;
+    double r = mt_uniform(rng);
+    const long sxn = (long)(sx + (2 * r - 1) * sfrac * sx), syn = (long)(sy + (2 * r - 1) * sfrac * sy);
+    const long szn = (long)(sz + (2 * r - 1) * sfrac * sz);
+    const long xn = (long)(MX * mt_uniform(rng)), yn = (long)(MY * mt_uniform(rng));
+    const long zn = (long)(1 * mt_uniform(rng));
+    const long xlo = xn - sxn / 2, xhi = xn + sxn / 2, ylo = yn - syn / 2, yhi = yn + syn / 2;
+    const long zlo = zn - szn / 2, zhi = zn + szn / 2;
+    int overlap = 0;
+    for (long x = xlo - shield; x < xhi + shield && !overlap; x++)
+      for (long y = ylo - shield; y < yhi + shield && !overlap; y++)
+        for (long z = zlo - shield; z < zhi + shield; z++) {
+          const long index = x * (long)g->ry + z * (long)g->ry + y;
+          if (0 <= index && index <= vol) {
+            const double m = g->phi[(int64_t)(N - 1) * g->n + index];
+            if (N > 1 ? !(m > 0) : (m > 0)) {
+              overlap = 1;
+              break;
+            }
+          }
+        }
+    if (overlap) continue;
+    if (variants >= N) variants = N - 1;
+    r = mt_uniform(rng);
+    const long phase = (long)(r * variants);
+    if (!quiet) printf("phase now = %ld\n", phase);
+    fill_cube_solid(g, (int)phase, xlo, xhi, ylo, yhi, zlo, zhi);
+    --nparticles;
+    if (++count > limit) {
+      if (!quiet) printf("Warning: random filling limit = %ld reached. %d particles could not be filled.\n", limit, nparticles);
+      break;
+    }
+  }
+  return GP_OK;
+}
+
 int gp_fill_domain(const gp_input *in, const char *filling_path, double *phi, double *mu, double *dphi, int quiet,
                    char *err, size_t errlen) {
   const gp_params *p = &in->p;
@@ -761,6 +1063,8 @@ int gp_fill_domain(const gp_input *in, const char *filling_path, double *phi, do
   }
   char line[1000], key[128] = "", val[128] = "";
   int nfills = 0;
+  rng48_t r48;
+  rng48_init(&r48);
   while (fgets(line, sizeof line, fr)) {
     sscanf(line, "%100s = %100[^;];", key, val);
     if (key[0] == '#') continue;
@@ -850,6 +1154,58 @@ int gp_fill_domain(const gp_input *in, const char *filling_path, double *phi, do
           }
       }
       nfills++;
+    } else if (strcmp(key, "FILLCYLINDERRANDOM") == 0 || strcmp(key, "FILLSPHERERANDOM") == 0) {
+      const int cyl = strcmp(key, "FILLCYLINDERRANDOM") == 0;
+      if (!quiet) printf(cyl ? "Filling cylinders at random\n" : "Filling spheres at random\n");
+      nt = split_tuple(val, tok, 16);
+      if (nt < 5) { fclose(fr); seterr(err, errlen, "%s needs 5 values", key); return GP_E_ARG; }
+      const long ph = atol(tok[0]), rad = atol(tok[1]);
+      const double vf = atof(tok[2]), spread = atof(tok[4]);
+      long shield = atol(tok[3]);
+      if (shield > 8) shield = 8;
+      else if (shield == 1) shield = 2;
+      if (ph < 0 || ph >= N - 1 || rad <= 0) { fclose(fr); seterr(err, errlen, "%s: phase or radius out of range", key); return GP_E_ARG; }
+      if (nfills) fill_liquid_all(&g, quiet); /* the earlier lines' liquid passes, completed */
+      nfills = 0;
+      if (cyl) fill_cylinders_random(&g, &r48, p->mesh_x, p->mesh_y, ph, rad, vf, shield, spread, quiet);
+      else fill_spheres_random(&g, &r48, p->mesh_x, p->mesh_y, ph, rad, vf, shield, spread, quiet);
+      if (!quiet) printf(cyl ? "End filling cylinders at random\n" : "End filling spheres at random\n");
+    } else if (strcmp(key, "FILLVORONOI2D") == 0) {
+      if (!quiet) printf("Filling Voronoi 2D\n");
+      nt = split_tuple(val, tok, 16);
+      if (nt < 6) { fclose(fr); seterr(err, errlen, "FILLVORONOI2D needs 6 values"); return GP_E_ARG; }
+      if (nfills) fill_liquid_all(&g, quiet);
+      nfills = 0;
+      if (N < 2 || fill_voronoi(&g, &r48, atol(tok[0]) + 3, atol(tok[1]) + 3, atol(tok[2]) + 3, atol(tok[3]) + 3,
+                                atol(tok[4]), atof(tok[5])) != GP_OK) {
+        fclose(fr);
+        seterr(err, errlen, "FILLVORONOI2D: region or point count out of range");
+        return GP_E_ARG;
+      }
+      fill_liquid_all(&g, quiet);
+      if (!quiet) printf("End filling Voronoi 2D");
+    } else if (strcmp(key, "FILLCUBEPATTERN") == 0 || strcmp(key, "FILLCUBERANDOM") == 0) {
+      const int pat = strcmp(key, "FILLCUBEPATTERN") == 0;
+      if (!quiet) printf(pat ? "Filling cube pattern.\n" : "Filling random cubes.\n");
+      nt = split_tuple(val, tok, 16);
+      if (nt < (pat ? 8 : 7)) { fclose(fr); seterr(err, errlen, "%s needs %d values", key, pat ? 8 : 7); return GP_E_ARG; }
+      const long variants = atol(tok[0]), sx = atol(tok[1]), sy = atol(tok[2]), sz = atol(tok[3]);
+      const double sfrac = atof(tok[4]);
+      if (sx < 1 || sy < 1 || sz < 1 || variants < 1) { fclose(fr); seterr(err, errlen, "%s: sizes and variants must be >= 1", key); return GP_E_ARG; }
+      if (nfills) fill_liquid_all(&g, quiet);
+      nfills = 0;
+      mt_t mt;
+      mt_seed(&mt, fill_seed());
+      if (pat) {
+        fill_cube_pattern(&g, &mt, p->mesh_x, p->mesh_y, variants, sx, sy, sfrac, atol(tok[5]), atof(tok[6]),
+                          atol(tok[7]), quiet);
+      } else if (fill_cubes_random(&g, &mt, p->mesh_x, p->mesh_y, variants, sx, sy, sz, sfrac, atof(tok[5]),
+                                   atol(tok[6]), quiet) != GP_OK) {
+        fclose(fr);
+        seterr(err, errlen, "FILLCUBERANDOM: no free place found for the requested volume fraction");
+        return GP_E_ARG;
+      }
+      if (!quiet) printf(pat ? "End filling cube pattern.\n" : "End filling random cubes.\n");
     }
   }
   fclose(fr);
