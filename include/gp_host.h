@@ -12,6 +12,8 @@
  *                          (GPs/functions/fill_domain.h:4-151, GPs/functions/filling.h:6-270)
  *   gp_write_vtk        <- writetofile_serial2D[_binary] (GPs/solverloop/file_writer.h:23-255)
  *   gp_read_vtk         <- readfromfile_serial2D[_binary] (GPs/solverloop/file_writer.h:47-70, 256-416)
+ *   gp_write_h5 / gp_read_h5 <- writetofile_mpi2D_hdf5 / readfromfile_mpi2D_hdf5 (GPm/solverloop/file_writer.h)
+ *   gp_write_xdmf       <- write_xdmf_xml (GPm/write_xdmf.c)
  *   gp_write_echo       <- the <infile>.out / <infile>.bd echo files
  *                          (GPs/functions/reading_input_parameters.h:310-516, read_boundary_conditions.h:195-207)
  */
@@ -62,6 +64,8 @@ typedef struct gp_input {
   uint32_t seen;
   uint8_t seen_A[GP_MAXP], seen_D[GP_MAXP], seen_rot[GP_MAXP][GP_MAXP];
   double eigen_dummy;
+  int32_t writehdf5;             /* WRITEHDF5 (the MPI driver's key): snapshots as DATA/<out>_<t>.h5 */
+  int32_t reserved_h5;
 } gp_input;
 
 /* Parse Input.in (both passes: parameters, then BOUNDARY / BOUNDARY_VALUE lines), compute the property
@@ -107,8 +111,22 @@ void gp_temperature_rows(const gp_params *p, int64_t t, int64_t shift_offset, do
 /* <stem>.out and <stem>.bd echo files. */
 int gp_write_echo(const gp_input *in, const char *input_path);
 
-/* Composition field sum_b c_b(mu) h_b(phi) of one cell (the Composition_<k> output). */
+/* Composition field sum_b c_b(mu) h_b(phi) of one cell (the Composition_<k> output), at the parameters' T or at T. */
 double gp_composition(const gp_params *p, const double *phi_cell, const double *mu_cell, int k);
+double gp_composition_at(const gp_params *p, const double *phi_cell, const double *mu_cell, int k, double T);
+
+/* The MPI driver's HDF5 snapshot DATA/<out>_<t>.h5 (writetofile_mpi2D_hdf5 / write_cells_hdf5_2D_mpi,
+ * GPm/solverloop/file_writer.h:76-95, 534-633): root-group datasets /<phase>, /Mu_<c>, /Composition_<c> and, when
+ * ISOTHERMAL = 0, /T, each [rows_x][rows_y] binary64 over the halo-inclusive grid. Written in the HDF5 file format
+ * (version 0 superblock, symbol-table root group, contiguous datasets) without the HDF5 library. T_rows as for
+ * gp_write_vtk. The reader (readfromfile_mpi2D_hdf5, :97-117 and :637-725) takes the phi and mu datasets of such a
+ * file into halo-inclusive SoA arrays. */
+int gp_write_h5(const gp_input *in, const char *path, const double *phi, const double *mu, const double *T_rows);
+int gp_read_h5(const gp_input *in, const char *path, double *phi, double *mu);
+
+/* The XDMF description of an HDF5 snapshot (write_xdmf_xml, GPm/write_xdmf.c:66-146, 2-D form): h5_name is the
+ * file name the XML points at, relative to the XML file. */
+int gp_write_xdmf(const gp_input *in, const char *path, const char *h5_name);
 
 /* The same for every cell of a halo-inclusive SoA state (phi [N][n], mu [NC][n]) into c [NC][n], n = ncells. */
 void gp_composition_grid(const gp_params *p, const double *phi, const double *mu, double *c, int64_t ncells);

@@ -96,6 +96,7 @@ class GPInput(C.Structure):
         ("bc_proxy", ((C.c_int32 * 3) * 3) * 6), ("bc_proxy_set", (C.c_int32 * 3) * 6),
         ("seen", C.c_uint32), ("seen_A", C.c_uint8 * P), ("seen_D", C.c_uint8 * P),
         ("seen_rot", (C.c_uint8 * P) * P), ("eigen_dummy", C.c_double),
+        ("writehdf5", C.c_int32), ("reserved_h5", C.c_int32),
     ]
 
 
@@ -123,6 +124,11 @@ def _declare_host(lib):
     lib.gp_apply_bc_host.argtypes = [C.POINTER(GPParams), _dp, _dp]
     lib.gp_apply_bc_host.restype = None
     lib.gp_write_vtk.argtypes = [C.POINTER(GPInput), C.c_char_p, _dp, _dp, _dp]
+    lib.gp_write_h5.argtypes = [C.POINTER(GPInput), C.c_char_p, _dp, _dp, _dp]
+    lib.gp_read_h5.argtypes = [C.POINTER(GPInput), C.c_char_p, _dp, _dp]
+    lib.gp_write_xdmf.argtypes = [C.POINTER(GPInput), C.c_char_p, C.c_char_p]
+    lib.gp_composition_at.argtypes = [C.POINTER(GPParams), _dp, _dp, C.c_int, C.c_double]
+    lib.gp_composition_at.restype = C.c_double
     lib.gp_temperature_rows.argtypes = [C.POINTER(GPParams), C.c_int64, C.c_int64, _dp]
     lib.gp_temperature_rows.restype = None
     lib.gp_read_vtk.argtypes = [C.POINTER(GPInput), C.c_char_p, _dp, _dp]

@@ -45,7 +45,8 @@ def hipcc() -> str:
 def build_host():
     os.makedirs(LIB, exist_ok=True)
     _run(["gcc"] + HOST_CFLAGS + ["-shared", "-I" + INC, "-o", os.path.join(LIB, "libgp_host.so"),
-                                  os.path.join(HERE, "host", "gp_host.c"), "-lm"])
+                                  os.path.join(HERE, "host", "gp_host.c"), os.path.join(HERE, "host", "gp_h5.c"),
+                                  "-lm"])
 
 
 def combos():

@@ -359,6 +359,7 @@ static int parse_pass1(FILE *fr, gp_input *in, int quiet, char *err, size_t errl
       p->tg_velocity = atof(tok[4]);
       in->seen |= SEEN_TG;
     }
+    else if (IS("WRITEHDF5")) in->writehdf5 = atoi(v);
     else if (IS("Shift")) in->shift = atoi(v);
     else if (IS("Shiftj")) { if (in->shift) in->shiftj = atol(v); }
     else if (IS("BINARY")) { in->binary = atoi(v); in->ternary = 0; in->dilute = 0; }
@@ -682,6 +683,10 @@ static double composition_at(const gp_params *p, const double *phi_cell, const d
 
 double gp_composition(const gp_params *p, const double *phi_cell, const double *mu_cell, int k) {
   return composition_at(p, phi_cell, mu_cell, k, p->T);
+}
+
+double gp_composition_at(const gp_params *p, const double *phi_cell, const double *mu_cell, int k, double T) {
+  return composition_at(p, phi_cell, mu_cell, k, T);
 }
 
 /* T(y) = temp_bottom + y GRADIENT with temp_bottom = base_temp - BASE_POS GRADIENT, BASE_POS = gradient_OFFSET/dy

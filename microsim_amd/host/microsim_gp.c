@@ -8,7 +8,8 @@
  * Same sequence as the reference main (GPs = Grand_potential_Finite_difference_2D_serial, GPs/microsim_gp.c:41-151):
  * mkdir DATA, read Input.in (+ <stem>.out / <stem>.bd echoes), fill the domain or read the restart snapshot,
  * apply the boundary conditions, write t = 0, NSMOOTH-1 smoothing steps, NTIMESTEPS steps with a snapshot every
- * SAVET (plus the DATA/shift.dat line) and the progress block every TRACK_PROGRESS. The loop body
+ * SAVET (plus the DATA/shift.dat line) and the progress block every TRACK_PROGRESS. WRITEHDF5 = 1 selects the MPI
+ * driver's HDF5 snapshots (GPm/microsim_gp.c:202-210, 368-377) with their XDMF descriptions. The loop body
  * (GPs/microsim_gp.c:103-106) is gp_step; steps between two output events are issued as one batch.
  *
  * Non-isothermal runs (ISOTHERMAL = 0 with Tempgrady) and the moving frame (Shift = 1) follow the same main: on a
@@ -34,6 +35,23 @@
 static void die(const char *what, const char *msg) {
   fprintf(stderr, "microsim_gp: %s: %s\n", what, msg);
   exit(1);
+}
+
+/* DATA/<out>_<t>.vtk, or with WRITEHDF5 = 1 the MPI driver's DATA/<out>_<t>.h5 plus its XDMF description
+ * DATA/<out>_<t>.xmf (the reference writes the latter with its separate write_xdmf tool) */
+static void write_snapshot(const gp_input *in, const char *out, int64_t t, const double *phi, const double *mu,
+                           const double *T_rows) {
+  char name[1200], h5[1100];
+  if (!in->writehdf5) {
+    snprintf(name, sizeof name, "DATA/%s_%ld.vtk", out, (long)t);
+    if (gp_write_vtk(in, name, phi, mu, T_rows)) die("write", name);
+    return;
+  }
+  snprintf(h5, sizeof h5, "%s_%ld.h5", out, (long)t);
+  snprintf(name, sizeof name, "DATA/%s", h5);
+  if (gp_write_h5(in, name, phi, mu, T_rows)) die("write", name);
+  snprintf(name, sizeof name, "DATA/%s_%ld.xmf", out, (long)t);
+  if (gp_write_xdmf(in, name, h5)) die("write", name);
 }
 
 int main(int argc, char *argv[]) {
@@ -62,8 +80,8 @@ int main(int argc, char *argv[]) {
   if (in.starttime == 0 && in.restart == 0) {
     if (gp_fill_domain(&in, argv[2], phi, mu, NULL, 0, err, sizeof err)) die("Filling", err);
   } else {
-    snprintf(name, sizeof name, "DATA/%s_%ld.vtk", argv[3], (long)in.starttime);
-    if (gp_read_vtk(&in, name, phi, mu)) die("restart", name);
+    snprintf(name, sizeof name, "DATA/%s_%ld.%s", argv[3], (long)in.starttime, in.writehdf5 ? "h5" : "vtk");
+    if ((in.writehdf5 ? gp_read_h5(&in, name, phi, mu) : gp_read_vtk(&in, name, phi, mu))) die("restart", name);
     if (in.shift) {
       /* the position of line STARTTIME/SAVET (0-based) of DATA/shift.dat; the last line read when it is shorter */
       FILE *fp = fopen("DATA/shift.dat", "r");
@@ -84,10 +102,7 @@ int main(int argc, char *argv[]) {
   }
   gp_apply_bc_host(p, phi, mu);
   gp_temperature_rows(p, 0, 0, T_rows);
-  if (in.restart == 0 || in.starttime == 0) {
-    snprintf(name, sizeof name, "DATA/%s_%ld.vtk", argv[3], (long)0);
-    if (gp_write_vtk(&in, name, phi, mu, T_rows)) die("write", name);
-  }
+  if (in.restart == 0 || in.starttime == 0) write_snapshot(&in, argv[3], 0, phi, mu, T_rows);
 
   int precision = GP_FP64;
   const char *pe = getenv("GP_PRECISION");
@@ -115,8 +130,7 @@ int main(int argc, char *argv[]) {
       int64_t shift_offset = 0;
       if (gp_download(h, phi, mu, NULL)) die("gp_download", gp_last_error(h));
       if (gp_get_frame(h, &shift_offset, T_rows)) die("gp_get_frame", gp_last_error(h));
-      snprintf(name, sizeof name, "DATA/%s_%ld.vtk", argv[3], (long)(t + in.starttime));
-      if (gp_write_vtk(&in, name, phi, mu, T_rows)) die("write", name);
+      write_snapshot(&in, argv[3], t + in.starttime, phi, mu, T_rows);
       FILE *fp = fopen("DATA/shift.dat", "a");
       if (fp) {
         fprintf(fp, "%ld %ld\n", (long)(t + in.starttime), (long)(shift_offset + shift_position));

@@ -129,3 +129,50 @@ def test_driver_restart_equals_uninterrupted_run(name, precision):
     finally:
         shutil.rmtree(d1, ignore_errors=True)
         shutil.rmtree(d2, ignore_errors=True)
+
+
+def test_driver_hdf5_snapshots_and_restart():
+    """WRITEHDF5 = 1 (the MPI driver's key): the snapshots are DATA/out_<t>.h5 with their DATA/out_<t>.xmf, hold the
+    reference run's state bit for bit, and a restart from the HDF5 snapshot at n continues to the same file at 2n."""
+    from test_h5_cpu import H5
+    print("[gpu] driver hdf5", flush=True)
+    sys.stdout.flush()
+    if not os.path.exists(DRIVER):
+        pytest.fail("microsim_gp is not built: run __graft_entry__.build()")
+    name = "tern_aniso_per"
+    case, ref = load_golden(name)
+    d1, d2 = tempfile.mkdtemp(prefix="gp_h5d1_"), tempfile.mkdtemp(prefix="gp_h5d2_")
+    try:
+        for d in (d1, d2):
+            for f in ("Input.in", "Filling.in"):
+                shutil.copy(os.path.join(GOLDEN, name, f), d)
+            with open(os.path.join(d, "Input.in"), "a") as fh:
+                fh.write("WRITEHDF5 = 1;\n")
+        saves = [s for s in ref["steps"] if s > 0]
+        savet = int(np.gcd.reduce(saves))
+        n = max(saves)
+        _set_keys(os.path.join(d1, "Input.in"), NTIMESTEPS=n, SAVET=savet, TRACK_PROGRESS=n)
+        _run_driver(d1)
+        names = [case.inp.phases[a].value.decode() for a in range(case.N)]
+        comps = [case.inp.components[k].value.decode() for k in range(case.NC)]
+        for i, s in enumerate(ref["steps"]):
+            f = H5(os.path.join(d1, "DATA", "out_%d.h5" % s))
+            data = {nm: f.dataset(oh)[1] for nm, oh in f.links()}
+            for a, nm in enumerate(names):
+                assert np.array_equal(data[nm][3:-3, 3:-3], ref["phi"][i][a]), "step %d %s" % (s, nm)
+            for k, c in enumerate(comps):
+                assert np.array_equal(data["Mu_" + c][3:-3, 3:-3], ref["mu"][i][k]), "step %d Mu_%s" % (s, c)
+            assert os.path.exists(os.path.join(d1, "DATA", "out_%d.xmf" % s))
+        # restart from the HDF5 snapshot at the first save
+        m = saves[0]
+        os.makedirs(os.path.join(d2, "DATA"))
+        shutil.copy(os.path.join(d1, "DATA", "out_%d.h5" % m), os.path.join(d2, "DATA"))
+        _set_keys(os.path.join(d2, "Input.in"), NTIMESTEPS=n - m, SAVET=savet, TRACK_PROGRESS=n, RESTART=1,
+                  STARTTIME=m)
+        _run_driver(d2)
+        a = open(os.path.join(d1, "DATA", "out_%d.h5" % n), "rb").read()
+        b = open(os.path.join(d2, "DATA", "out_%d.h5" % n), "rb").read()
+        assert a == b
+    finally:
+        shutil.rmtree(d1, ignore_errors=True)
+        shutil.rmtree(d2, ignore_errors=True)

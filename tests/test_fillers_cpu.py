@@ -12,6 +12,7 @@ against restatements of Grand_potential_Finite_difference_2D_MPI/functions/filli
 import ctypes as C
 import math
 import os
+import shutil
 import tempfile
 
 import numpy as np
@@ -27,6 +28,14 @@ _libc.nrand48.argtypes = [C.POINTER(C.c_ushort)]
 _libc.nrand48.restype = C.c_long
 
 MESH = 64
+_DIRS = []
+
+
+@pytest.fixture(autouse=True, scope="module")
+def _cleanup():
+    yield
+    for d in _DIRS:
+        shutil.rmtree(d, ignore_errors=True)
 
 
 class Drand48:
@@ -68,6 +77,7 @@ def _case(nphases=3):
     else:
         inp, fills = configs.config_c0(MESH, steps=10, savet=10)
     d = tempfile.mkdtemp(prefix="gp_fill_")
+    _DIRS.append(d)
     configs.write_case(d, inp, fills)
     return d, host.read_input(os.path.join(d, "Input.in"))
 

@@ -1,0 +1,185 @@
+"""The MPI driver's HDF5 snapshots and XDMF descriptions (gp_write_h5 / gp_read_h5 / gp_write_xdmf,
+microsim_amd/host/gp_h5.c).
+
+The reference writes them with the HDF5 library (Grand_potential_Finite_difference_2D_MPI/solverloop/
+file_writer.h:534-633, write_xdmf.c:66-146); neither that library nor the MPI reference builds here, so the files are
+checked by what the format and the reference's writer fix:
+- a reader written here in Python from the HDF5 file-format specification (independent of the C reader) walks the
+  superblock, the root group's B-tree, symbol-table node and local heap, and every dataset's object header, and
+  finds the reference's dataset names, the halo-inclusive [MESH_X+6][MESH_Y+6] shape, IEEE binary64 little-endian
+  and the values of the state (composition and T included);
+- the C reader returns the written phi and mu bit for bit;
+- the XDMF text has the reference's lines with this run's names and sizes.
+"""
+import ctypes as C
+import os
+import struct
+
+import numpy as np
+import pytest
+
+from gp_testlib import frame_cases, golden_cases, load_golden
+from microsim_amd import _abi, host
+
+UNDEF = 0xFFFFFFFFFFFFFFFF
+
+
+class H5:
+    """Reader of the subset of the HDF5 format a symbol-table root group of contiguous datasets uses."""
+
+    def __init__(self, path):
+        self.b = open(path, "rb").read()
+        b = self.b
+        assert b[:8] == b"\x89HDF\r\n\x1a\n", "signature"
+        assert b[8] == 0, "superblock version 0"
+        self.so, self.sl = b[13], b[14]
+        assert (self.so, self.sl) == (8, 8)
+        self.leaf_k, self.node_k = struct.unpack_from("<HH", b, 16)
+        base, freespace, self.eof, driver = struct.unpack_from("<QQQQ", b, 24)
+        assert base == 0 and freespace == UNDEF and driver == UNDEF
+        _, root_oh, cache = struct.unpack_from("<QQI", b, 56)
+        assert cache == 1
+        self.btree, self.heap = struct.unpack_from("<QQ", b, 80)
+        # the root object header carries the same symbol-table message
+        ver, _, nmsg, _, size = struct.unpack_from("<BBHII", b, root_oh)
+        assert ver == 1 and nmsg >= 1
+        mtype, msize = struct.unpack_from("<HH", b, root_oh + 16)
+        assert mtype == 0x11 and struct.unpack_from("<QQ", b, root_oh + 24) == (self.btree, self.heap)
+        assert b[self.heap:self.heap + 4] == b"HEAP"
+        self.heap_size, self.free_head, self.heap_data = struct.unpack_from("<QQQ", b, self.heap + 8)
+
+    def name(self, off):
+        e = self.b.index(b"\0", self.heap_data + off)
+        return self.b[self.heap_data + off:e].decode()
+
+    def links(self):
+        b = self.b
+        assert b[self.btree:self.btree + 4] == b"TREE"
+        ntype, level, used = struct.unpack_from("<BBH", b, self.btree + 4)
+        assert ntype == 0 and level == 0 and used == 1
+        key0, child, key1 = struct.unpack_from("<QQQ", b, self.btree + 24)
+        assert b[child:child + 4] == b"SNOD" and b[child + 4] == 1
+        n = struct.unpack_from("<H", b, child + 6)[0]
+        out = []
+        for i in range(n):
+            off, oh = struct.unpack_from("<QQ", b, child + 8 + 40 * i)
+            out.append((self.name(off), oh))
+        assert self.name(key0) == "" and self.name(key1) == out[-1][0]
+        return out
+
+    def dataset(self, oh):
+        b = self.b
+        ver, _, nmsg, refc, size = struct.unpack_from("<BBHII", b, oh)
+        assert ver == 1 and refc == 1
+        m, info = oh + 16, {}
+        for _ in range(nmsg):
+            mtype, msize = struct.unpack_from("<HH", b, m)
+            assert msize % 8 == 0
+            d = m + 8
+            if mtype == 1:
+                assert b[d] == 1
+                info["shape"] = struct.unpack_from("<%dQ" % b[d + 1], b, d + 8)
+            elif mtype == 3:
+                info["type"] = (b[d] & 0x0F, b[d] >> 4, b[d + 1], b[d + 2], struct.unpack_from("<I", b, d + 4)[0],
+                                struct.unpack_from("<HHBBBBI", b, d + 8))
+            elif mtype == 8:
+                assert b[d] == 3 and b[d + 1] == 1
+                info["addr"], info["size"] = struct.unpack_from("<QQ", b, d + 2)
+            m += 8 + msize
+        assert m - oh - 16 == size
+        shape = info["shape"]
+        data = np.frombuffer(b, dtype="<f8", count=shape[0] * shape[1], offset=info["addr"]).reshape(shape)
+        return info, data
+
+
+def _state(name):
+    case, _ = load_golden(name)
+    T = host.temperature_rows(case.inp.p)
+    rng = np.random.default_rng(3)
+    # a state with every bit pattern in play: the golden initial state plus noise in the interior
+    phi, mu = case.phi.copy(), case.mu.copy()
+    mu[:, 3:-3, 3:-3] += 1e-3 * rng.standard_normal(mu[:, 3:-3, 3:-3].shape)
+    return case, phi, mu, T
+
+
+@pytest.mark.parametrize("name", ["tern_iso_mixed", "quad_k3", "bin_tanh_restart"] + frame_cases())
+def test_h5_layout_and_values(name, tmp_path):
+    case, phi, mu, T = _state(name)
+    inp, p = case.inp, case.inp.p
+    N, NC = int(p.nphases), int(p.ncomp) - 1
+    path = str(tmp_path / "out_0.h5")
+    assert _abi.host_lib().gp_write_h5(C.byref(inp), path.encode(), _abi.dptr(phi), _abi.dptr(mu),
+                                       _abi.dptr(T)) == 0
+    f = H5(path)
+    assert f.eof == os.path.getsize(path)
+    links = f.links()
+    names = [n for n, _ in links]
+    phases = [inp.phases[a].value.decode() for a in range(N)]
+    comps = [inp.components[k].value.decode() for k in range(NC)]
+    want = phases + ["Mu_" + c for c in comps] + ["Composition_" + c for c in comps] + (["T"] if p.tempgrady else [])
+    assert names == sorted(want)
+    comp = host.composition(p, phi, mu) if not p.tempgrady else None
+    rx, ry = int(p.mesh_x) + 6, int(p.mesh_y) + 6
+    for n, oh in links:
+        info, data = f.dataset(oh)
+        assert info["shape"] == (rx, ry)
+        cls, ver, bf0, bf1, size, props = info["type"]
+        assert (cls, ver, size) == (1, 1, 8) and bf0 & 1 == 0 and bf1 == 63 and (bf0 >> 4) & 3 == 2
+        assert props == (0, 64, 52, 11, 0, 52, 1023)
+        assert info["size"] == rx * ry * 8
+        if n in phases:
+            assert np.array_equal(data, phi[phases.index(n)])
+        elif n.startswith("Mu_"):
+            assert np.array_equal(data, mu[comps.index(n[3:])])
+        elif n.startswith("Composition_"):
+            k = comps.index(n[12:])
+            if comp is not None:
+                assert np.array_equal(data, comp[k])
+            else:
+                lib = _abi.host_lib()
+                for x, y in [(3, 3), (rx // 2, ry // 2), (rx - 4, ry - 4)]:
+                    pc = np.ascontiguousarray(phi[:, x, y])
+                    mc = np.ascontiguousarray(mu[:, x, y])
+                    assert data[x, y] == lib.gp_composition_at(C.byref(p), _abi.dptr(pc), _abi.dptr(mc), k, T[y])
+        else:
+            assert np.array_equal(data, np.broadcast_to(T, (rx, ry)))
+    # the C reader gives phi and mu back exactly
+    phi2, mu2 = np.zeros_like(phi), np.zeros_like(mu)
+    assert _abi.host_lib().gp_read_h5(C.byref(inp), path.encode(), _abi.dptr(phi2), _abi.dptr(mu2)) == 0
+    assert np.array_equal(phi2, phi) and np.array_equal(mu2, mu)
+
+
+def test_h5_reader_rejects_a_foreign_grid(tmp_path):
+    case, phi, mu, T = _state("tern_iso_mixed")
+    path = str(tmp_path / "out_0.h5")
+    assert _abi.host_lib().gp_write_h5(C.byref(case.inp), path.encode(), _abi.dptr(phi), _abi.dptr(mu), None) == 0
+    other, _ = load_golden("tern_aniso_per")
+    N, NC = int(other.inp.p.nphases), int(other.inp.p.ncomp) - 1
+    rx, ry = int(other.inp.p.mesh_x) + 6, int(other.inp.p.mesh_y) + 6
+    p2, m2 = np.zeros((N, rx, ry)), np.zeros((NC, rx, ry))
+    assert _abi.host_lib().gp_read_h5(C.byref(other.inp), path.encode(), _abi.dptr(p2), _abi.dptr(m2)) != 0
+    open(path, "r+b").write(b"\0")
+    assert _abi.host_lib().gp_read_h5(C.byref(case.inp), path.encode(), _abi.dptr(p2), _abi.dptr(m2)) != 0
+
+
+@pytest.mark.parametrize("name", ["tern_iso_mixed", "bin_tempgrad_shift"])
+def test_xdmf_text(name, tmp_path):
+    case, _ = load_golden(name)
+    inp, p = case.inp, case.inp.p
+    path = str(tmp_path / "out_100.xmf")
+    assert _abi.host_lib().gp_write_xdmf(C.byref(inp), path.encode(), b"out_100.h5") == 0
+    text = open(path).read().split("\n")
+    MX, MY = int(p.mesh_x) + 6, int(p.mesh_y) + 6
+    assert text[:6] == ['<?xml version="1.0" ?>', '<!DOCTYPE Xdmf SYSTEM "Xdmf.dtd" []>', '<Xdmf Version="2.0">',
+                        " <Domain>", '   <Grid Name="mesh1" GridType="Uniform">',
+                        '     <Topology TopologyType="2DCoRectMesh" NumberOfElements="%d %d"/>' % (MX, MY)]
+    assert text[8] == "       \t0 0" and text[11] == "       \t1.000000 1.000000"
+    N, NC = int(p.nphases), int(p.ncomp) - 1
+    attrs = [l.split('"')[1] for l in text if l.startswith("     <Attribute Name=")]
+    want = [inp.phases[a].value.decode() for a in range(N)] + ["Mu_" + inp.components[k].value.decode() for k in range(NC)]
+    want += ["Composition_" + inp.components[k].value.decode() for k in range(NC)] if inp.writecomposition else []
+    want += ["T"] if p.tempgrady else []
+    assert attrs == want
+    refs = [l.strip() for l in text if l.startswith("        out_100.h5:/")]
+    assert refs == ["out_100.h5:/" + a for a in want]
+    assert text[-4:] == ["   </Grid>", " </Domain>", "</Xdmf>", ""]
