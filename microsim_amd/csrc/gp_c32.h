@@ -136,7 +136,7 @@ __global__ __launch_bounds__(256) void k_dlist_c(const Model<double> *__restrict
     const int64_t c = clist[i];
     RegStore<N, NC, double> rs;
     c32_gather<N, NC>(phi, plo, mu, mlo, c, fs, ps, false, rs);
-    const RegView<N, NC, double, P2> v{&rs, 0, 0};
+    const RegView<N, NC, double, P2, true> v{&rs, 0, 0};
     double d[N];
     phi_delta<N, NC>(*mp, v, false, d);
 #pragma unroll
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(256) void k_ilist_c(const Model<double> *__restrict
         rs.d[a][o] = cand ? v : 0.0;
       }
     }
-    const RegView<N, NC, double, P2> v{&rs, 0, 0};
+    const RegView<N, NC, double, P2, true> v{&rs, 0, 0};
     double jx0[NC], jxb[NC], jy0[NC], jyl[NC], d[N], pn[N], mn[NC], dm[NC];
     jat_x<N, NC>(m, v, 0, 0, jx0);
     jat_x<N, NC>(m, v, -1, 0, jxb);

@@ -86,7 +86,7 @@ __device__ __forceinline__ void cbulk_cell(const Model<R> &m, int b, int64_t x, 
                                            int64_t mx, bool kept, bool store, bool live, R (&st)[NSA],
                                            const int8_t *q1 = nullptr, float *__restrict__ plo_o = nullptr,
                                            int8_t *__restrict__ mlo_o = nullptr) {
-  using Vw = View<R, P2>;
+  using Vw = View<R, P2, false, kFastOf<R>>;
   R pe[N];
 #pragma unroll
   for (int a = 0; a < N; a++) pe[a] = a == b ? R(1.0) : R(0.0);
@@ -331,7 +331,7 @@ __global__ __launch_bounds__(256) void k_cbulk(const Model<R> *__restrict__ mp, 
                                                int *counts, int *counts_next, int tile_blocks,
                                                const int8_t *__restrict__ mlo, float *__restrict__ plo_o,
                                                int8_t *__restrict__ mlo_o) {
-  using Vw = View<R, P2>;
+  using Vw = View<R, P2, false, kFastOf<R>>;
   constexpr int NS = 3 * N + 3 * NC + 1;
   constexpr int NSA = STATS ? NS : 1;
   if ((int)blockIdx.x >= tile_blocks) {
@@ -578,7 +578,7 @@ __global__ __launch_bounds__(256) void k_ilist(const Model<R> *__restrict__ mp, 
         rs.d[a][o] = cand ? v : R(0.0);
       }
     }
-    const RegView<N, NC, R, P2> v{&rs, 0, 0};
+    const RegView<N, NC, R, P2, kFastOf<R>> v{&rs, 0, 0};
     R jx0[NC], jxb[NC], jy0[NC], jyl[NC], d[N], pn[N], mn[NC], dm[NC];
     jat_x<N, NC>(m, v, 0, 0, jx0);
     jat_x<N, NC>(m, v, -1, 0, jxb);

@@ -246,7 +246,7 @@ __global__ __launch_bounds__(256) void k_march(const Model<R> *__restrict__ mp, 
                                                const int *__restrict__ flags_in, int *__restrict__ flags_out,
                                                int64_t fs, int64_t ps, int64_t mx, int64_t my, int seg_len,
                                                int nstrips, int nsegs, int nwaves) {
-  using Vw = View<R, P2>;
+  using Vw = View<R, P2, false, kFastOf<R>>;
   constexpr int NS = 3 * N + 3 * NC + 1;
   const Model<R> &m = *mp;
   const int lane = threadIdx.x & 63;
@@ -691,7 +691,7 @@ __global__ __launch_bounds__(256) void k_bulk(const Model<R> *__restrict__ mp, c
                                               const int *__restrict__ flags_in, int *__restrict__ flags_out,
                                               int64_t fs, int64_t ps, int64_t mx, int seg_len, int nstrips, int nsegs,
                                               int nwaves) {
-  using Vw = View<R, P2>;
+  using Vw = View<R, P2, false, kFastOf<R>>;
   constexpr int NS = 3 * N + 3 * NC + 1;
   constexpr int NSA = STATS ? NS : 1;
   const Model<R> &m = *mp;
@@ -857,7 +857,7 @@ __device__ __forceinline__ void dphi_cell(const Model<R> &m, const R *__restrict
       for (int dj = 0; dj < 3; dj++) rs.p[a][di][dj] = phi[a * fs + c + (di - 1) * ps + (dj - 1)];
 #pragma unroll
   for (int k = 0; k < NC; k++) rs.u[k][0] = mu[k * fs + c];
-  const RegView<N, NC, R, P2> v{&rs, 0, 0};
+  const RegView<N, NC, R, P2, kFastOf<R>> v{&rs, 0, 0};
   R d[N];
   phi_delta<N, NC>(m, v, false, d);
 #pragma unroll

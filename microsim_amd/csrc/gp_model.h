@@ -26,6 +26,8 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <type_traits>
+
 #if defined(__HIPCC__) || defined(__HIP__)
 #include <hip/hip_runtime.h>
 #define GP_HD __host__ __device__ __forceinline__
@@ -88,9 +90,12 @@ struct RowThermo {
 /* Stencil accessor over SoA planes: field a at (x+i, y+j) relative to the centre index c.
  * POW2: dx and dy are powers of two, so v/dx == v*(1/dx) exactly (the quotient and the product are the
  * same real number, rounded once) and the divisions by the spacings become multiplications. */
-template <typename R, bool POW2 = false, bool TG = false>
+template <typename R, bool POW2 = false, bool TG = false, bool FAST = false>
 struct View {
   static constexpr bool kPow2 = POW2;
+  /* FAST: the anisotropy terms take one reciprocal of |q|^2 per q instead of the reference's five quotients
+   * (aniso_ac_dadq_fast); for the modes whose results are not the reference's bit for bit */
+  static constexpr bool kFast = FAST;
   /* TG: non-isothermal, th[y] holds the coefficients of allocated row y and y is the centre's row */
   static constexpr bool kTG = TG;
   const R *phi, *mu, *dphi;
@@ -122,10 +127,11 @@ struct RegStore {
   R u[NC > 0 ? NC : 1][5];
   R d[N][5];
 };
-template <int N, int NC, typename R, bool POW2>
+template <int N, int NC, typename R, bool POW2, bool FAST = false>
 struct RegView {
   static constexpr bool kPow2 = POW2;
   static constexpr bool kTG = false;
+  static constexpr bool kFast = FAST;
   const RegStore<N, NC, R> *s;
   int si, sj;
   GP_HD R ph(int a, int i, int j) const { return s->p[a][si + i + 1][sj + j + 1]; }
@@ -133,6 +139,10 @@ struct RegView {
   GP_HD R dp(int a, int i, int j) const { return s->d[a][five_slot(si + i, sj + j)]; }
   GP_HD RegView shift(int i, int j) const { return RegView{s, si + i, sj + j}; }
 };
+
+/* views of the FP32 mode take the fast anisotropy forms (View::kFast) */
+template <typename R>
+constexpr bool kFastOf = std::is_same<R, float>::value;
 
 /* Quotient and square root of the step arithmetic (the one place that chooses their form). FP64: IEEE,
  * correctly rounded, as the reference computes them, so the FP64 results are bit-identical to it. FP32 (the
@@ -345,6 +355,26 @@ GP_HD void aniso_dadq(R dab, const R *q, R *d) {
   }
 }
 
+/* a_c and d a_c/dq together from one reciprocal of |q|^2: a_c = 1 - dab (3 - 4 Q4 r^2), d a_c/dq_i =
+ * 16 dab r^2 q_i (q_i^2 - Q4 r), r = 1/|q|^2 (the same real values as aniso_ac and aniso_dadq, within a few ulp) */
+template <typename R>
+GP_HD void aniso_ac_dadq_fast(R dab, const R *q, R &ac, R *d) {
+  const R qx2 = q[0] * q[0], qy2 = q[1] * q[1];
+  const R q2 = qx2 + qy2;
+  if (fabs(q2) > R(1.0e-15)) {
+    const R r = R(1.0) / q2, r2 = r * r;
+    const R q4 = qx2 * qx2 + qy2 * qy2;
+    const R t = q4 * r, f = R(16.0) * dab * r2;
+    ac = R(1.0) - dab * (R(3.0) - R(4.0) * q4 * r2);
+    d[0] = f * q[0] * (qx2 - t);
+    d[1] = f * q[1] * (qy2 - t);
+  } else {
+    ac = R(1.0);
+    d[0] = R(0.0);
+    d[1] = R(0.0);
+  }
+}
+
 template <typename R>
 GP_HD R qnorm2(const R *q) { return q[0] * q[0] + q[1] * q[1] + R(0.0) * R(0.0); }
 
@@ -377,9 +407,15 @@ GP_HD R dAdphi_A01(const Model<R> &m, const V &v, const R *phi, const R *cxc, co
     r[0] = (phi[a] * cxc[b] - phi[b] * cxc[a]);
     r[1] = (phi[a] * cyc[b] - phi[b] * cyc[a]);
     rot2(m.rot[a][b], r, q);
-    aniso_dadq(m.dab[a][b], q, dadq);
+    R ac1;
+    if constexpr (V::kFast) {
+      aniso_ac_dadq_fast(m.dab[a][b], q, ac1, dadq);
+    } else {
+      aniso_dadq(m.dab[a][b], q, dadq);
+      ac1 = aniso_ac(m.dab[a][b], q);
+    }
     R qab2 = qnorm2(q);
-    R ac1 = aniso_ac(m.dab[a][b], q), ac2 = ac1 * ac1;
+    R ac2 = ac1 * ac1;
     dq[0] = cxc[b];
     dq[1] = cyc[b];
     rot2(m.rot[a][b], dq, dqr);
@@ -405,7 +441,22 @@ GP_HD R divdAdgradphi_A01(const Model<R> &m, const V &v, int a) {
     q_face_y(m, v, a, b, 0, 0, qr);
     q_face_y(m, v, a, b, 0, -1, ql);
     const R dab = m.dab[a][b];
-    R acf = aniso_ac(dab, qf), acb = aniso_ac(dab, qb), acr = aniso_ac(dab, qr), acl = aniso_ac(dab, ql);
+    R acf, acb, acr, acl, df[2], db[2], dr[2], dl[2];
+    if constexpr (V::kFast) {
+      aniso_ac_dadq_fast(dab, qf, acf, df);
+      aniso_ac_dadq_fast(dab, qb, acb, db);
+      aniso_ac_dadq_fast(dab, qr, acr, dr);
+      aniso_ac_dadq_fast(dab, ql, acl, dl);
+    } else {
+      acf = aniso_ac(dab, qf);
+      acb = aniso_ac(dab, qb);
+      acr = aniso_ac(dab, qr);
+      acl = aniso_ac(dab, ql);
+      aniso_dadq(dab, qf, df);
+      aniso_dadq(dab, qb, db);
+      aniso_dadq(dab, qr, dr);
+      aniso_dadq(dab, ql, dl);
+    }
     R acf2 = acf * acf, acb2 = acb * acb, acr2 = acr * acr, acl2 = acl * acl;
     R sxa = sx(m, v, a, 0, 0), sxb = sx(m, v, b, 0, 0), gxa = gx(m, v, a, 0, 0), gxb = gx(m, v, b, 0, 0);
     R Bsxa = sx(m, v, a, -1, 0), Bsxb = sx(m, v, b, -1, 0), Bgxa = gx(m, v, a, -1, 0), Bgxb = gx(m, v, b, -1, 0);
@@ -415,14 +466,10 @@ GP_HD R divdAdgradphi_A01(const Model<R> &m, const V &v, int a) {
     d1 += divy<V>(m, ((acr2 * (sya * gyb - syb * gya) * (-syb)) - (acl2 * (Lsya * Lgyb - Lsyb * Lgya) * (-Lsyb))));
     d1 *= m.gamma[a][b];
     sum1 += d1;
-    R df[2], db[2], dr[2], dl[2], rf[2], rb[2], rr[2], rl[2];
-    aniso_dadq(dab, qf, df);
+    R rf[2], rb[2], rr[2], rl[2];
     rot2(m.rot_inv[a][b], df, rf);
-    aniso_dadq(dab, qb, db);
     rot2(m.rot_inv[a][b], db, rb);
-    aniso_dadq(dab, qr, dr);
     rot2(m.rot_inv[a][b], dr, rr);
-    aniso_dadq(dab, ql, dl);
     rot2(m.rot_inv[a][b], dl, rl);
     R qf2 = qnorm2(qf), qb2 = qnorm2(qb), qr2 = qnorm2(qr), ql2 = qnorm2(ql);
     R d2 = divx<V>(m, (acf * qf2 * rf[0] * (-sxb) - acb * qb2 * rb[0] * (-Bsxb)));

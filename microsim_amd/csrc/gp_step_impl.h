@@ -25,7 +25,7 @@ __global__ __launch_bounds__(kBX *kBY) void k_phi(const Model<R> *__restrict__ m
   if (x > mx + 3 || y > my + 3) return;
   const Model<R> &m = *mp;
   const int64_t c = x * ps + y + kYOff;
-  View<R, P2, TG> v{phi, mu, nullptr, fs, ps, c, th, y};
+  View<R, P2, TG, kFastOf<R>> v{phi, mu, nullptr, fs, ps, c, th, y};
   R d[N];
   phi_delta<N, NC>(m, v, SMOOTH, d);
   for (int a = 0; a < N; a++) dphi[a * fs + c] = d[a];
@@ -56,7 +56,7 @@ __global__ __launch_bounds__(kBX *kBY) void k_mu(const Model<R> *__restrict__ mp
   if (live) {
     const Model<R> &m = *mp;
     const int64_t c = x * ps + y + kYOff;
-    View<R, P2, TG> v{phi, mu, dphi, fs, ps, c, th, y};
+    View<R, P2, TG, kFastOf<R>> v{phi, mu, dphi, fs, ps, c, th, y};
     R jx0[NC], jxb[NC], jy0[NC], jyl[NC], d[N], pn[N], mn[NC], dm[NC];
     if (!SMOOTH) {
       jat_x<N, NC>(m, v, 0, 0, jx0);

@@ -278,7 +278,7 @@ static int run_c32(const gp_params *p, double *phi, double *mu, int64_t nsteps) 
       code[i] = (int8_t)(nz == 1 ? cd : -1);
     }
     std::fill(D.begin(), D.end(), 0.0);
-    View<double, P2> vw{P.data(), M.data(), D.data(), n, ry, 0};
+    View<double, P2, false, true> vw{P.data(), M.data(), D.data(), n, ry, 0};
     for (int64_t x = 2; x <= mx + 3; x++)
       for (int64_t y = 2; y <= my + 3; y++) {
         const int64_t c = x * ry + y;

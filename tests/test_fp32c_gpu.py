@@ -91,8 +91,10 @@ def test_fp32c_error_budget_seed_growth():
 
 
 def test_fp32c_delta_phi_download_and_invariants():
-    """delta-phi of the last step and the solute/energy: FP64 kernels on the decoded state, equal to the oracle's
-    on the downloaded (decoded) states (delta-phi bit for bit; the sums to 1e-12, their order differs)."""
+    """delta-phi of the last step and the solute/energy: FP64 arithmetic on the decoded state, equal to the oracle's
+    on the downloaded (decoded) states: delta-phi to 1e-12 of its largest value (the mode's anisotropy terms take
+    one reciprocal of |q|^2 where the oracle takes the reference's five quotients), the sums to 1e-12 (their
+    order differs)."""
     import ctypes as C
     from microsim_amd.solver import GPSolver
     case, _ = load_golden("tern_c1_mini")
@@ -106,7 +108,8 @@ def test_fp32c_delta_phi_download_and_invariants():
     op, om, od = p9.copy(), m9.copy(), np.zeros_like(p9)
     oracle_step(case.params, op, om, 1, dphi=od)
     sl = (slice(None), slice(2, -2), slice(2, -2))
-    assert np.array_equal(d10[sl], od[sl])
+    np.testing.assert_allclose(d10[sl], od[sl], rtol=0, atol=1e-12 * np.abs(od[sl]).max())
+    assert np.abs(od[sl]).max() > 1e-4
     sol, en = (C.c_double * 4)(), C.c_double()
     oracle().oracle_invariants(C.byref(case.params), _abi.dptr(np.ascontiguousarray(p10)),
                                _abi.dptr(np.ascontiguousarray(m10)), sol, C.byref(en))
