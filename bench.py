@@ -112,8 +112,9 @@ def run_mode(case, precision: int, steps: int, warmup: int, late_at: int = 0):
     from microsim_amd.solver import GPSolver
     N, NC = case.N, case.NC
     cells = int(case.params.mesh_x) * int(case.params.mesh_y)
-    es = 8 if precision == 0 else 4  # algorithmic bytes: FP32 state for both FP32 modes
-    b_alg = 2 * (N + NC) * es
+    # algorithmic bytes: the state read and written once per cell-update. FP64: 8 B per field; FP32: 4 B; the
+    # compensated mode: 8 B per phi field (two FP32 parts) and 5 B per mu field (FP32 + 8-bit extension)
+    b_alg = {0: 2 * (N + NC) * 8, 1: 2 * (N + NC) * 4, 2: 2 * (8 * N + 5 * NC)}[precision]
 
     def window(s, done):
         phi_a, _ = s.download()
@@ -404,7 +405,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--mesh", type=int, default=4096)
     ap.add_argument("--seeds", type=int, default=1024)
-    ap.add_argument("--modes", default="fp32,fp64")
+    ap.add_argument("--modes", default="fp32c,fp32,fp64",
+                    help="the first is the headline (fp32c: the compensated FP32 production mode)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--late-at", type=int, default=5000,
                     help="repeat the timings after this many steps of the same run (0: skip)")
