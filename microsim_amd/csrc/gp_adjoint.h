@@ -115,8 +115,9 @@ __global__ __launch_bounds__(256) void k_adj_cbar(const Model<double> *__restric
   list_append(dual, (int32_t)c, qlist, qcount);
 }
 
-/* the gather above for every cell n of [2, mesh+3]^2: plain stores into gphi / gmu (cleared before) and, for bulk
- * n, into gdphi; the dual kernels then add the other cells' contributions */
+/* the gather above for every cell n of [2, mesh+3]^2: plain stores into gphi / gmu and into gdphi (its value for a
+ * bulk n, else 0); every other allocated cell gets zeros in all three, so no clearing pass is needed. The
+ * reverse-mode kernels then add the other cells' contributions. */
 template <int N, int NC>
 __global__ __launch_bounds__(256) void k_adj_bulk(const Model<double> *__restrict__ mp, const double *__restrict__ phi,
                                                   const double *__restrict__ mu, const double *__restrict__ dphi,
@@ -125,12 +126,20 @@ __global__ __launch_bounds__(256) void k_adj_bulk(const Model<double> *__restric
                                                   double *__restrict__ gdphi, double *gtheta, int th0, int64_t fs,
                                                   int64_t ps, int64_t mx, int64_t my) {
   const Model<double> &m = *mp;
-  const int64_t y = 2 + (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
-  const int64_t x = 2 + (int64_t)blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int64_t y = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
+  const int64_t x = (int64_t)blockIdx.y * 4 + (threadIdx.x >> 6);
   constexpr int L = N - 1;
   double th[NC];
   for (int k = 0; k < NC; k++) th[k] = 0.0;
-  if (x <= mx + 3 && y <= my + 3) {
+  const bool alloc = x < mx + 6 && y < my + 6;
+  if (alloc && !(x >= 2 && x <= mx + 3 && y >= 2 && y <= my + 3)) {
+    const int64_t c = x * ps + y + kYOff;
+    for (int a = 0; a < N; a++) {
+      gphi[a * fs + c] = 0.0;
+      gdphi[a * fs + c] = 0.0;
+    }
+    for (int k = 0; k < NC; k++) gmu[k * fs + c] = 0.0;
+  } else if (alloc) {
     const int64_t c = x * ps + y + kYOff;
     const bool inner = x >= 3 && x <= mx + 2 && y >= 3 && y <= my + 2;
     const int b = inner ? bulk_q<N, NC>(phi, dphi, c, fs, ps) : -1;
@@ -151,6 +160,8 @@ __global__ __launch_bounds__(256) void k_adj_bulk(const Model<double> *__restric
       }
       for (int l = 0; l < NC; l++)
         for (int k = 0; k < NC; k++) gm[l] += cb[k] * m.cmu[b][k][l];
+    } else {
+      for (int a = 0; a < N; a++) gdphi[a * fs + c] = 0.0;
     }
     for (int k = 0; k < NC; k++)
       for (int l = 0; l < NC; l++) {
@@ -320,23 +331,34 @@ __device__ __forceinline__ bool p_candidate(const double *phi, int64_t c, int64_
   return false;
 }
 
-/* the cells of [2, mesh+3]^2 with a nonzero delta-phi adjoint that are P candidates */
-template <int N>
-__global__ __launch_bounds__(256) void k_adj_plist(const double *__restrict__ phi, const double *__restrict__ gdphi,
-                                                   int32_t *__restrict__ plist, int *pcount, int64_t fs, int64_t ps,
-                                                   int64_t mx, int64_t my) {
+/* delta-phi of the state at every cell of [2, mesh+3]^2 (phi_delta, FP64, as the step forms it) and the list of the
+ * P candidates among them: a cell whose five-point phi is not one pure phase throughout has delta-phi formed by
+ * phi_delta and is listed; every other cell's delta-phi is +0 (no divergence, no clamp), stored without evaluating */
+template <int N, int NC, bool P2>
+__global__ __launch_bounds__(256) void k_adj_dphi(const Model<double> *__restrict__ mp, const double *__restrict__ phi,
+                                                  const double *__restrict__ mu, double *__restrict__ dphi,
+                                                  int32_t *__restrict__ plist, int *pcount, int64_t fs, int64_t ps,
+                                                  int64_t mx, int64_t my) {
   const int64_t y = 2 + (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
   const int64_t x = 2 + (int64_t)blockIdx.y * 4 + (threadIdx.x >> 6);
   const int64_t c = x * ps + y + kYOff;
-  bool flag = false;
+  bool cand = false;
   if (x <= mx + 3 && y <= my + 3) {
-    for (int a = 0; a < N; a++) flag = flag || gdphi[a * fs + c] != 0.0;
-    flag = flag && p_candidate<N>(phi, c, fs, ps);
+    cand = p_candidate<N>(phi, c, fs, ps);
+    double d[N];
+    if (cand) {
+      const View<double, P2> v{phi, mu, nullptr, fs, ps, c};
+      phi_delta<N, NC>(*mp, v, false, d);
+    } else {
+      for (int a = 0; a < N; a++) d[a] = 0.0;
+    }
+    for (int a = 0; a < N; a++) dphi[a * fs + c] = d[a];
   }
-  list_append(flag, (int32_t)c, plist, pcount);
+  list_append(cand, (int32_t)c, plist, pcount);
 }
 
-/* Stage P transposed by reverse mode (gp_reverse.h: phi_delta_vjp), one pass per listed cell */
+/* Stage P transposed by reverse mode (gp_reverse.h: phi_delta_vjp), one pass per listed candidate (k_adj_dphi's
+ * list) with a nonzero delta-phi adjoint */
 template <int N, int NC, bool P2>
 __global__ __launch_bounds__(256) void k_adj_prev(const Model<double> *__restrict__ mp, const double *__restrict__ phi,
                                                   const double *__restrict__ mu, const double *__restrict__ gdphi,
@@ -351,7 +373,12 @@ __global__ __launch_bounds__(256) void k_adj_prev(const Model<double> *__restric
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int64_t c = list[i];
     double go[N];
-    for (int a = 0; a < N; a++) go[a] = gdphi[a * fs + c];
+    bool nz = false;
+    for (int a = 0; a < N; a++) {
+      go[a] = gdphi[a * fs + c];
+      nz = nz || go[a] != 0.0;
+    }
+    if (!nz) continue; /* a candidate with no incoming delta-phi adjoint */
     RegStore<N, NC, double> rs;
     gather_stencil<N, NC>(phi, mu, nullptr, c, fs, ps, rs);
     RevAcc<N, NC> acc;

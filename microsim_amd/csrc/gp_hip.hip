@@ -863,10 +863,12 @@ int gp_adjoint_backward(gp_handle *h, const double *gphi_T, const double *gmu_T,
       ProfScope ps(h, GP_KERNEL_ADJ);
       if ((rc = state_load(h, A->seg[j]))) return rc;
       if (wide) {
-        if ((rc = f(h, MODE_DPHI_CUR))) return rc;
-        rc = fa(h, (const double *)h->phi[h->cur], (const double *)h->mu[h->cur], (const double *)h->dphi, h->model);
+        const double *p64 = (const double *)h->phi[h->cur], *m64 = (const double *)h->mu[h->cur];
+        if ((rc = h->ops->adj_dphi(h, p64, m64, (double *)h->dphi, h->model))) return rc;
+        rc = fa(h, p64, m64, (const double *)h->dphi, h->model);
       } else {
         if ((rc = fw(h))) return rc;
+        if ((rc = h->ops->adj_dphi(h, A->st64, A->st64 + (size_t)h->N * h->fs, A->dphi64, A->model64))) return rc;
         rc = fa(h, A->st64, A->st64 + (size_t)h->N * h->fs, A->dphi64, A->model64);
       }
       if (rc) return rc;

@@ -138,7 +138,7 @@ int gp_fail(gp_handle *h, int code, const char *fmt, ...);
     if (e_ != hipSuccess) return gp_fail((h), GP_E_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
   } while (0)
 
-enum { MODE_SMOOTH = 0, MODE_TWOPASS = 1, MODE_MARCH = 2, MODE_DPHI_PREV = 3, MODE_MARCH_STATS = 4, MODE_DPHI_CUR = 5 };
+enum { MODE_SMOOTH = 0, MODE_TWOPASS = 1, MODE_MARCH = 2, MODE_DPHI_PREV = 3, MODE_MARCH_STATS = 4 };
 
 inline size_t esize(const gp_handle *h) { return h->precision == GP_FP64 ? sizeof(double) : sizeof(float); }
 inline size_t state_elems(const gp_handle *h) { return (size_t)(h->N + h->NC) * h->fs; }
@@ -183,6 +183,7 @@ struct Adjoint {
 typedef int (*step_fn)(gp_handle *, int);
 typedef int (*inv_fn)(gp_handle *);
 typedef int (*adj_fn)(gp_handle *, const double *, const double *, const double *, const void *);
+typedef int (*adj_dphi_fn)(gp_handle *, const double *, const double *, double *, const void *);
 typedef int (*widen_fn)(gp_handle *);
 typedef int (*inv_on_fn)(gp_handle *, const void *, const void *, int64_t, int64_t, int64_t, double *, double *,
                          hipStream_t);
@@ -196,6 +197,7 @@ struct GpOps {
   adj_fn adj_step;
   widen_fn widen;
   inv_on_fn invariants_on; /* the invariants of a given state, with given partials buffer and stream */
+  adj_dphi_fn adj_dphi;    /* the adjoint's delta-phi of an evaluation state and its P candidates */
 };
 
 #define GP_DECLARE_OPS(a, b) const GpOps *gp_ops_##a##_##b(int precision);

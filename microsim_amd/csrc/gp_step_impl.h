@@ -288,12 +288,6 @@ static int launch_step_p(gp_handle *h, int mode) {
   dim3 g1((unsigned)((my + 2 + kBX - 1) / kBX), (unsigned)((mx + 2 + kBY - 1) / kBY));
   dim3 g2((unsigned)((my + kBX - 1) / kBX), (unsigned)((mx + kBY - 1) / kBY));
   const bool smooth = mode == MODE_SMOOTH;
-  if (mode == MODE_DPHI_CUR) {
-    /* delta-phi of the current state (the adjoint's recompute) */
-    hipLaunchKernelGGL((k_phi<N, NC, R, false, P2>), g1, blk, 0, h->stream, m, phi, mu, dphi, h->fs, h->ps, mx, my, (const RowThermo<R> *)nullptr);
-    HIPCHK(h, hipGetLastError());
-    return GP_OK;
-  }
   const bool tg = h->p.tempgrady != 0;
   const RowThermo<R> *thp = (const RowThermo<R> *)h->thp, *thc = (const RowThermo<R> *)h->thc;
   if (mode == MODE_DPHI_PREV) {
@@ -506,23 +500,19 @@ static int adj_step_t(gp_handle *h, const double *phi, const double *mu, const d
   const dim3 gall((unsigned)((h->ry + 63) / 64), (unsigned)((h->rx + 3) / 4));
   hipLaunchKernelGGL(k_adj_bc, gall, blk, 0, h->stream, gn, N, h->maps, 0, fs, h->ps, h->rx, h->ry);
   hipLaunchKernelGGL(k_adj_bc, gall, blk, 0, h->stream, gn + (size_t)N * fs, NC, h->maps, 1, fs, h->ps, h->rx, h->ry);
-  HIPCHK(h, hipMemsetAsync(gc, 0, state_elems(h) * sizeof(double), h->stream));
-  HIPCHK(h, hipMemsetAsync(A->gdphi, 0, (size_t)N * fs * sizeof(double), h->stream));
-  /* bulk cells of stage Q in closed form (plain stores), then the reverse-mode kernels add the rest */
-  HIPCHK(h, hipMemsetAsync(A->counts, 0, 2 * sizeof(int), h->stream));
+  /* bulk cells of stage Q in closed form (plain stores over the whole allocated grid: no clearing pass), then the
+   * reverse-mode kernels add the rest; counts[1] is the P candidate list of adj_dphi_t */
+  HIPCHK(h, hipMemsetAsync(A->counts, 0, sizeof(int), h->stream));
   const Model<double> *md = (const Model<double> *)model;
   hipLaunchKernelGGL((k_adj_cbar<N, NC>), gall, blk, 0, h->stream, md, phi, dphi, gn, gn + (size_t)N * fs, A->cbar,
                      A->qlist, A->counts, fs, h->ps, mx, my);
-  const dim3 gb((unsigned)((my + 2 + 63) / 64), (unsigned)((mx + 2 + 3) / 4));
-  hipLaunchKernelGGL((k_adj_bulk<N, NC>), gb, blk, 0, h->stream, md, phi, mu, dphi, (const double *)A->cbar, gn, gc,
-                     gc + (size_t)N * fs, A->gdphi, A->gth, 3 * (N * (N - 1) / 2), fs, h->ps, mx, my);
+  hipLaunchKernelGGL((k_adj_bulk<N, NC>), gall, blk, 0, h->stream, md, phi, mu, dphi, (const double *)A->cbar, gn,
+                     gc, gc + (size_t)N * fs, A->gdphi, A->gth, 3 * (N * (N - 1) / 2), fs, h->ps, mx, my);
   /* the other cells by reverse mode, one pass per listed cell (grid-stride loops) */
   const dim3 gl(kAdjBlocks);
   hipLaunchKernelGGL((k_adj_qrev<N, NC, P2>), gl, blk, 0, h->stream, md, phi, mu, dphi, gn, gn + (size_t)N * fs, gc,
                      gc + (size_t)N * fs, A->gdphi, A->gth, (const int32_t *)A->qlist, (const int *)A->counts, fs,
                      h->ps);
-  hipLaunchKernelGGL((k_adj_plist<N>), gb, blk, 0, h->stream, phi, (const double *)A->gdphi, A->plist, A->counts + 1, fs,
-                     h->ps, mx, my);
   hipLaunchKernelGGL((k_adj_prev<N, NC, P2>), gl, blk, 0, h->stream, md, phi, mu, (const double *)A->gdphi, gc,
                      gc + (size_t)N * fs, A->gth, (const int32_t *)A->plist, (const int *)A->counts + 1, fs, h->ps);
   HIPCHK(h, hipGetLastError());
@@ -530,7 +520,25 @@ static int adj_step_t(gp_handle *h, const double *phi, const double *mu, const d
   return GP_OK;
 }
 
-/* FP32 handles: the current state widened into A->st64 and its delta-phi (FP64 model) into A->dphi64 */
+/* delta-phi (FP64 model) of the state an adjoint step is evaluated at, and the list of its P candidates (into
+ * A->plist / A->counts[1]) */
+template <int N, int NC, bool P2>
+static int adj_dphi_t(gp_handle *h, const double *phi, const double *mu, double *dphi, const void *model) {
+  Adjoint *A = h->adj;
+  const int64_t mx = h->p.mesh_x, my = h->p.mesh_y;
+  HIPCHK(h, hipMemsetAsync(A->counts + 1, 0, sizeof(int), h->stream));
+  const dim3 gb((unsigned)((my + 2 + 63) / 64), (unsigned)((mx + 2 + 3) / 4));
+  hipLaunchKernelGGL((k_adj_dphi<N, NC, P2>), gb, dim3(256), 0, h->stream, (const Model<double> *)model, phi, mu,
+                     dphi, A->plist, A->counts + 1, h->fs, h->ps, mx, my);
+  HIPCHK(h, hipGetLastError());
+  return GP_OK;
+}
+template <int N, int NC>
+static int adj_dphi_any(gp_handle *h, const double *phi, const double *mu, double *dphi, const void *model) {
+  return h->pow2 ? adj_dphi_t<N, NC, true>(h, phi, mu, dphi, model) : adj_dphi_t<N, NC, false>(h, phi, mu, dphi, model);
+}
+
+/* FP32 handles: the current state widened into A->st64 (its delta-phi then comes from adj_dphi_t) */
 static __global__ __launch_bounds__(256) void k_widen(const float *__restrict__ src, double *__restrict__ dst, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dst[i] = (double)src[i];
@@ -543,11 +551,6 @@ static int adj_widen_t(gp_handle *h) {
                      (const float *)h->phi[h->cur], A->st64, np);
   hipLaunchKernelGGL(k_widen, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, h->stream,
                      (const float *)h->mu[h->cur], A->st64 + np, nm);
-  const int64_t mx = h->p.mesh_x, my = h->p.mesh_y;
-  dim3 g1((unsigned)((my + 2 + kBX - 1) / kBX), (unsigned)((mx + 2 + kBY - 1) / kBY));
-  hipLaunchKernelGGL((k_phi<N, NC, double, false, P2>), g1, dim3(kBX, kBY), 0, h->stream,
-                     (const Model<double> *)A->model64, (const double *)A->st64, (const double *)(A->st64 + np),
-                     A->dphi64, h->fs, h->ps, mx, my, (const RowThermo<double> *)nullptr);
   HIPCHK(h, hipGetLastError());
   return GP_OK;
 }
@@ -606,8 +609,8 @@ static int launch_step_c(gp_handle *h, int mode) {
   dim3 g1((unsigned)((my + 2 + kBX - 1) / kBX), (unsigned)((mx + 2 + kBY - 1) / kBY));
   dim3 g2((unsigned)((my + kBX - 1) / kBX), (unsigned)((mx + kBY - 1) / kBY));
   int rc;
-  if (mode == MODE_DPHI_CUR || mode == MODE_DPHI_PREV) {
-    const int b = mode == MODE_DPHI_CUR ? h->cur : 1 - h->cur;
+  if (mode == MODE_DPHI_PREV) {
+    const int b = 1 - h->cur;
     if ((rc = c32_wide_buffer(h, 0)) || (rc = c32_widen_state(h, b, h->wide[0]))) return rc;
     hipLaunchKernelGGL((k_phi<N, NC, double, false, P2>), g1, blk, 0, h->stream, md, (const double *)h->wide[0],
                        (const double *)(h->wide[0] + N * fs), dphi, fs, h->ps, mx, my, (const RowThermo<double> *)nullptr);
@@ -720,19 +723,11 @@ static int launch_invariants_on_c(gp_handle *h, const void *phi, const void *mu,
   return launch_invariants_model<N, NC, double>(h, h->model64, phi, mu, fs, ps, yoff, partials, out, stream);
 }
 
-/* the adjoint's FP64 evaluation point of a compensated state: the decoded state and its FP64 delta-phi */
+/* the adjoint's FP64 evaluation point of a compensated state: the decoded state (its delta-phi then comes from
+ * adj_dphi_t) */
 template <int N, int NC, bool P2>
 static int adj_widen_c(gp_handle *h) {
-  Adjoint *A = h->adj;
-  int rc;
-  if ((rc = c32_widen_state(h, h->cur, A->st64))) return rc;
-  const int64_t mx = h->p.mesh_x, my = h->p.mesh_y, np = (int64_t)N * h->fs;
-  dim3 g1((unsigned)((my + 2 + kBX - 1) / kBX), (unsigned)((mx + 2 + kBY - 1) / kBY));
-  hipLaunchKernelGGL((k_phi<N, NC, double, false, P2>), g1, dim3(kBX, kBY), 0, h->stream,
-                     (const Model<double> *)A->model64, (const double *)A->st64, (const double *)(A->st64 + np),
-                     A->dphi64, h->fs, h->ps, mx, my, (const RowThermo<double> *)nullptr);
-  HIPCHK(h, hipGetLastError());
-  return GP_OK;
+  return c32_widen_state(h, h->cur, h->adj->st64);
 }
 template <int N, int NC>
 static int adj_widen_c_any(gp_handle *h) {
@@ -743,11 +738,11 @@ static int adj_widen_c_any(gp_handle *h) {
 template <int N, int NC>
 const GpOps *make_ops(int precision) {
   static const GpOps f64 = {launch_step<N, NC, double>, launch_invariants<N, NC, double>, adj_step_any<N, NC>,
-                            adj_widen_any<N, NC>, launch_invariants_on<N, NC, double>};
+                            adj_widen_any<N, NC>, launch_invariants_on<N, NC, double>, adj_dphi_any<N, NC>};
   static const GpOps f32 = {launch_step<N, NC, float>, launch_invariants<N, NC, float>, adj_step_any<N, NC>,
-                            adj_widen_any<N, NC>, launch_invariants_on<N, NC, float>};
+                            adj_widen_any<N, NC>, launch_invariants_on<N, NC, float>, adj_dphi_any<N, NC>};
   static const GpOps c32 = {launch_step_c_any<N, NC>, launch_invariants_c<N, NC>, adj_step_any<N, NC>,
-                            adj_widen_c_any<N, NC>, launch_invariants_on_c<N, NC>};
+                            adj_widen_c_any<N, NC>, launch_invariants_on_c<N, NC>, adj_dphi_any<N, NC>};
   return precision == GP_FP64 ? &f64 : precision == GP_FP32C ? &c32 : &f32;
 }
 
