@@ -90,8 +90,8 @@ template <int N, int NC>
 __global__ __launch_bounds__(256) void k_adj_cbar(const Model<double> *__restrict__ mp, const double *__restrict__ phi,
                                                   const double *__restrict__ dphi, const double *__restrict__ gphi_n,
                                                   const double *__restrict__ gmu_n, double *__restrict__ cbar,
-                                                  int32_t *__restrict__ qlist, int *qcount, int64_t fs, int64_t ps,
-                                                  int64_t mx, int64_t my) {
+                                                  int8_t *__restrict__ bq, int32_t *__restrict__ qlist, int *qcount,
+                                                  int64_t fs, int64_t ps, int64_t mx, int64_t my) {
   const int64_t y = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
   const int64_t x = (int64_t)blockIdx.y * 4 + (threadIdx.x >> 6);
   const bool valid = x < mx + 6 && y < my + 6;
@@ -100,6 +100,7 @@ __global__ __launch_bounds__(256) void k_adj_cbar(const Model<double> *__restric
   const int b = inner ? bulk_q<N, NC>(phi, dphi, c, fs, ps) : -1;
   bool dual = false;
   if (valid) {
+    bq[c] = (int8_t)b;
     const Model<double> &m = *mp;
     for (int k = 0; k < NC; k++) {
       double s = 0.0;
@@ -120,11 +121,11 @@ __global__ __launch_bounds__(256) void k_adj_cbar(const Model<double> *__restric
  * reverse-mode kernels then add the other cells' contributions. */
 template <int N, int NC>
 __global__ __launch_bounds__(256) void k_adj_bulk(const Model<double> *__restrict__ mp, const double *__restrict__ phi,
-                                                  const double *__restrict__ mu, const double *__restrict__ dphi,
-                                                  const double *__restrict__ cbar, const double *__restrict__ gphi_n,
-                                                  double *__restrict__ gphi, double *__restrict__ gmu,
-                                                  double *__restrict__ gdphi, double *gtheta, int th0, int64_t fs,
-                                                  int64_t ps, int64_t mx, int64_t my) {
+                                                  const double *__restrict__ mu, const int8_t *__restrict__ bq,
+                                                  const int8_t *__restrict__ pcode, const double *__restrict__ cbar,
+                                                  const double *__restrict__ gphi_n, double *__restrict__ gphi,
+                                                  double *__restrict__ gmu, double *__restrict__ gdphi, double *gtheta,
+                                                  int th0, int64_t fs, int64_t ps, int64_t mx, int64_t my) {
   const Model<double> &m = *mp;
   const int64_t y = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
   const int64_t x = (int64_t)blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -141,8 +142,8 @@ __global__ __launch_bounds__(256) void k_adj_bulk(const Model<double> *__restric
     for (int k = 0; k < NC; k++) gmu[k * fs + c] = 0.0;
   } else if (alloc) {
     const int64_t c = x * ps + y + kYOff;
-    const bool inner = x >= 3 && x <= mx + 2 && y >= 3 && y <= my + 2;
-    const int b = inner ? bulk_q<N, NC>(phi, dphi, c, fs, ps) : -1;
+    const int b = bq[c]; /* -1 outside the inner cells */
+    const int pc = pcode[c];
     double gph[N], gm[NC], cb[NC], mun[NC], phn[N], Mn[NC][NC];
     for (int k = 0; k < NC; k++) {
       cb[k] = cbar[k * fs + c];
@@ -150,7 +151,8 @@ __global__ __launch_bounds__(256) void k_adj_bulk(const Model<double> *__restric
       gm[k] = 0.0;
     }
     for (int a = 0; a < N; a++) {
-      phn[a] = phi[a * fs + c];
+      /* a pure cell's phi is e_pc exactly */
+      phn[a] = pc >= 0 ? (a == pc ? 1.0 : 0.0) : phi[a * fs + c];
       gph[a] = 0.0;
     }
     if (b >= 0) {
@@ -163,11 +165,16 @@ __global__ __launch_bounds__(256) void k_adj_bulk(const Model<double> *__restric
     } else {
       for (int a = 0; a < N; a++) gdphi[a * fs + c] = 0.0;
     }
+    /* M(n); a pure cell's is Mpure, formed with the same operations (bit-identical) */
     for (int k = 0; k < NC; k++)
       for (int l = 0; l < NC; l++) {
         double s = 0.0;
-        for (int a = 0; a < N; a++)
-          for (int q = 0; q < NC; q++) s += m.DC[a][k][l][q] * phn[a];
+        if (pc >= 0) {
+          s = m.Mpure[pc][k][l];
+        } else {
+          for (int a = 0; a < N; a++)
+            for (int q = 0; q < NC; q++) s += m.DC[a][k][l][q] * phn[a];
+        }
         Mn[k][l] = s;
       }
     const double wx = m.inv_dx * m.inv_dx, wy = m.inv_dy * m.inv_dy;
@@ -182,15 +189,20 @@ __global__ __launch_bounds__(256) void k_adj_bulk(const Model<double> *__restric
         nz = nz || dF[k] != 0.0;
       }
       if (!nz) continue;
+      const int pe = pcode[e];
       double phe[N];
-      for (int a = 0; a < N; a++) phe[a] = phi[a * fs + e];
+      for (int a = 0; a < N; a++) phe[a] = pe >= 0 ? (a == pe ? 1.0 : 0.0) : phi[a * fs + e];
       for (int l = 0; l < NC; l++) due[l] = mu[l * fs + e] - mun[l];
       for (int l = 0; l < NC; l++) {
         double s = 0.0;
         for (int k = 0; k < NC; k++) {
           double Me = 0.0;
-          for (int a = 0; a < N; a++)
-            for (int q = 0; q < NC; q++) Me += m.DC[a][k][l][q] * phe[a];
+          if (pe >= 0) {
+            Me = m.Mpure[pe][k][l];
+          } else {
+            for (int a = 0; a < N; a++)
+              for (int q = 0; q < NC; q++) Me += m.DC[a][k][l][q] * phe[a];
+          }
           s += dF[k] * 0.5 * (Mn[k][l] + Me);
         }
         gm[l] += w * s;
@@ -331,19 +343,28 @@ __device__ __forceinline__ bool p_candidate(const double *phi, int64_t c, int64_
   return false;
 }
 
-/* delta-phi of the state at every cell of [2, mesh+3]^2 (phi_delta, FP64, as the step forms it) and the list of the
- * P candidates among them: a cell whose five-point phi is not one pure phase throughout has delta-phi formed by
+/* the pure-phase code of every allocated cell (pcode), delta-phi of the state at every cell of [2, mesh+3]^2
+ * (phi_delta, FP64, as the step forms it) and the list of the P candidates among them: a cell whose five-point phi is not one pure phase throughout has delta-phi formed by
  * phi_delta and is listed; every other cell's delta-phi is +0 (no divergence, no clamp), stored without evaluating */
 template <int N, int NC, bool P2>
 __global__ __launch_bounds__(256) void k_adj_dphi(const Model<double> *__restrict__ mp, const double *__restrict__ phi,
                                                   const double *__restrict__ mu, double *__restrict__ dphi,
-                                                  int32_t *__restrict__ plist, int *pcount, int64_t fs, int64_t ps,
-                                                  int64_t mx, int64_t my) {
-  const int64_t y = 2 + (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
-  const int64_t x = 2 + (int64_t)blockIdx.y * 4 + (threadIdx.x >> 6);
+                                                  int8_t *__restrict__ pcode, int32_t *__restrict__ plist, int *pcount,
+                                                  int64_t fs, int64_t ps, int64_t mx, int64_t my) {
+  const int64_t y = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
+  const int64_t x = (int64_t)blockIdx.y * 4 + (threadIdx.x >> 6);
   const int64_t c = x * ps + y + kYOff;
   bool cand = false;
-  if (x <= mx + 3 && y <= my + 3) {
+  if (x < mx + 6 && y < my + 6) {
+    int code = -1, nz = 0;
+    for (int a = 0; a < N; a++) {
+      const double v = phi[a * fs + c];
+      if (v == 1.0) code = a;
+      nz += (v != 0.0);
+    }
+    pcode[c] = (int8_t)(nz == 1 ? code : -1);
+  }
+  if (x >= 2 && x <= mx + 3 && y >= 2 && y <= my + 3) {
     cand = p_candidate<N>(phi, c, fs, ps);
     double d[N];
     if (cand) {

@@ -171,10 +171,13 @@ struct Adjoint {
   void *model64 = nullptr;
   int *counts = nullptr;                      /* their lengths */
   unsigned long long *scan = nullptr;         /* GP_ADJOINT_CHECK: first non-finite index, max |g| bits, theta flag */
+  /* per allocated cell of the evaluation state: the pure phase of phi (b when phi = e_b exactly, else -1) and the
+   * bulk phase of stage Q (bulk_q, -1 for the other cells) */
+  int8_t *pcode = nullptr, *bq = nullptr;
   ~Adjoint() {
     for (double *b : ckpt) (void)hipFree(b);
     for (double *b : seg) (void)hipFree(b);
-    void *bufs[] = {g[0], g[1], gdphi, gth, cbar, qlist, plist, counts, st64, dphi64, model64, scan};
+    void *bufs[] = {g[0], g[1], gdphi, gth, cbar, qlist, plist, counts, st64, dphi64, model64, scan, pcode, bq};
     for (void *b : bufs)
       if (b) (void)hipFree(b);
   }

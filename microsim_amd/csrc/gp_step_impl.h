@@ -505,9 +505,10 @@ static int adj_step_t(gp_handle *h, const double *phi, const double *mu, const d
   HIPCHK(h, hipMemsetAsync(A->counts, 0, sizeof(int), h->stream));
   const Model<double> *md = (const Model<double> *)model;
   hipLaunchKernelGGL((k_adj_cbar<N, NC>), gall, blk, 0, h->stream, md, phi, dphi, gn, gn + (size_t)N * fs, A->cbar,
-                     A->qlist, A->counts, fs, h->ps, mx, my);
-  hipLaunchKernelGGL((k_adj_bulk<N, NC>), gall, blk, 0, h->stream, md, phi, mu, dphi, (const double *)A->cbar, gn,
-                     gc, gc + (size_t)N * fs, A->gdphi, A->gth, 3 * (N * (N - 1) / 2), fs, h->ps, mx, my);
+                     A->bq, A->qlist, A->counts, fs, h->ps, mx, my);
+  hipLaunchKernelGGL((k_adj_bulk<N, NC>), gall, blk, 0, h->stream, md, phi, mu, (const int8_t *)A->bq,
+                     (const int8_t *)A->pcode, (const double *)A->cbar, gn, gc, gc + (size_t)N * fs, A->gdphi, A->gth,
+                     3 * (N * (N - 1) / 2), fs, h->ps, mx, my);
   /* the other cells by reverse mode, one pass per listed cell (grid-stride loops) */
   const dim3 gl(kAdjBlocks);
   hipLaunchKernelGGL((k_adj_qrev<N, NC, P2>), gl, blk, 0, h->stream, md, phi, mu, dphi, gn, gn + (size_t)N * fs, gc,
@@ -527,9 +528,9 @@ static int adj_dphi_t(gp_handle *h, const double *phi, const double *mu, double 
   Adjoint *A = h->adj;
   const int64_t mx = h->p.mesh_x, my = h->p.mesh_y;
   HIPCHK(h, hipMemsetAsync(A->counts + 1, 0, sizeof(int), h->stream));
-  const dim3 gb((unsigned)((my + 2 + 63) / 64), (unsigned)((mx + 2 + 3) / 4));
-  hipLaunchKernelGGL((k_adj_dphi<N, NC, P2>), gb, dim3(256), 0, h->stream, (const Model<double> *)model, phi, mu,
-                     dphi, A->plist, A->counts + 1, h->fs, h->ps, mx, my);
+  const dim3 gall((unsigned)((h->ry + 63) / 64), (unsigned)((h->rx + 3) / 4));
+  hipLaunchKernelGGL((k_adj_dphi<N, NC, P2>), gall, dim3(256), 0, h->stream, (const Model<double> *)model, phi, mu,
+                     dphi, A->pcode, A->plist, A->counts + 1, h->fs, h->ps, mx, my);
   HIPCHK(h, hipGetLastError());
   return GP_OK;
 }
