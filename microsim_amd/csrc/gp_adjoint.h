@@ -343,14 +343,14 @@ __device__ __forceinline__ bool p_candidate(const double *phi, int64_t c, int64_
   return false;
 }
 
-/* the pure-phase code of every allocated cell (pcode), delta-phi of the state at every cell of [2, mesh+3]^2
- * (phi_delta, FP64, as the step forms it) and the list of the P candidates among them: a cell whose five-point phi is not one pure phase throughout has delta-phi formed by
- * phi_delta and is listed; every other cell's delta-phi is +0 (no divergence, no clamp), stored without evaluating */
-template <int N, int NC, bool P2>
-__global__ __launch_bounds__(256) void k_adj_dphi(const Model<double> *__restrict__ mp, const double *__restrict__ phi,
-                                                  const double *__restrict__ mu, double *__restrict__ dphi,
-                                                  int8_t *__restrict__ pcode, int32_t *__restrict__ plist, int *pcount,
-                                                  int64_t fs, int64_t ps, int64_t mx, int64_t my) {
+/* the pure-phase code of every allocated cell (pcode) and the list of the P candidates of [2, mesh+3]^2: a cell whose
+ * five-point phi is not one pure phase throughout. Every other cell's delta-phi is +0 (no divergence, no clamp):
+ * this pass stores +0 at every cell of [2, mesh+3]^2, and k_dlist then forms the candidates' delta-phi
+ * (phi_delta, FP64, as the step does) over the list in full wavefronts. */
+template <int N>
+__global__ __launch_bounds__(256) void k_adj_codes(const double *__restrict__ phi, double *__restrict__ dphi,
+                                                   int8_t *__restrict__ pcode, int32_t *__restrict__ plist, int *pcount,
+                                                   int64_t fs, int64_t ps, int64_t mx, int64_t my) {
   const int64_t y = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
   const int64_t x = (int64_t)blockIdx.y * 4 + (threadIdx.x >> 6);
   const int64_t c = x * ps + y + kYOff;
@@ -366,19 +366,12 @@ __global__ __launch_bounds__(256) void k_adj_dphi(const Model<double> *__restric
   }
   if (x >= 2 && x <= mx + 3 && y >= 2 && y <= my + 3) {
     cand = p_candidate<N>(phi, c, fs, ps);
-    double d[N];
-    if (cand) {
-      const View<double, P2> v{phi, mu, nullptr, fs, ps, c};
-      phi_delta<N, NC>(*mp, v, false, d);
-    } else {
-      for (int a = 0; a < N; a++) d[a] = 0.0;
-    }
-    for (int a = 0; a < N; a++) dphi[a * fs + c] = d[a];
+    for (int a = 0; a < N; a++) dphi[a * fs + c] = 0.0;
   }
   list_append(cand, (int32_t)c, plist, pcount);
 }
 
-/* Stage P transposed by reverse mode (gp_reverse.h: phi_delta_vjp), one pass per listed candidate (k_adj_dphi's
+/* Stage P transposed by reverse mode (gp_reverse.h: phi_delta_vjp), one pass per listed candidate (k_adj_codes'
  * list) with a nonzero delta-phi adjoint */
 template <int N, int NC, bool P2>
 __global__ __launch_bounds__(256) void k_adj_prev(const Model<double> *__restrict__ mp, const double *__restrict__ phi,

@@ -529,8 +529,11 @@ static int adj_dphi_t(gp_handle *h, const double *phi, const double *mu, double 
   const int64_t mx = h->p.mesh_x, my = h->p.mesh_y;
   HIPCHK(h, hipMemsetAsync(A->counts + 1, 0, sizeof(int), h->stream));
   const dim3 gall((unsigned)((h->ry + 63) / 64), (unsigned)((h->rx + 3) / 4));
-  hipLaunchKernelGGL((k_adj_dphi<N, NC, P2>), gall, dim3(256), 0, h->stream, (const Model<double> *)model, phi, mu,
-                     dphi, A->pcode, A->plist, A->counts + 1, h->fs, h->ps, mx, my);
+  hipLaunchKernelGGL((k_adj_codes<N>), gall, dim3(256), 0, h->stream, phi, dphi, A->pcode, A->plist, A->counts + 1,
+                     h->fs, h->ps, mx, my);
+  hipLaunchKernelGGL((k_dlist<N, NC, double, P2>), dim3(kListBlocks), dim3(256), 0, h->stream,
+                     (const Model<double> *)model, phi, mu, dphi, h->fs, h->ps, (const int32_t *)A->plist,
+                     (const int *)A->counts + 1);
   HIPCHK(h, hipGetLastError());
   return GP_OK;
 }
