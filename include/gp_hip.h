@@ -262,11 +262,14 @@ int gp_adjoint_forward(gp_handle *h, int64_t nsteps, int64_t every);
 /* Reverse sweep of the trajectory recorded by gp_adjoint_forward. Given the gradient of a scalar loss with
  * respect to the final state (gphi_T, gmu_T: halo-inclusive host arrays as in gp_upload), returns its gradient
  * with respect to the state passed to gp_upload (gphi_0, gmu_0; the halo entries, which gp_upload overwrites
- * from the interior, get 0) and with respect to theta. Segments between checkpoints are recomputed; each step's
- * local Jacobians are evaluated exactly (forward-mode dual numbers through the step's own per-cell arithmetic)
- * and applied transposed. On an FP32 handle the Jacobians are those of the FP64 model at the FP32 states (the
- * state is widened to FP64 for each transposed step); adjoint fields and parameter gradients are accumulated in
- * FP64 in both precisions. The handle holds the initial state afterwards. */
+ * from the interior, get 0) and with respect to theta. Segments between checkpoints are recomputed; each step is
+ * transposed exactly: a closed-form gather for bulk cells and, for the other cells, reverse-mode passes through
+ * the step's own per-cell arithmetic. On an FP32 or compensated FP32 handle the Jacobians are those of the FP64
+ * model at the mode's (decoded) states, widened to FP64 for each transposed step; adjoint fields and parameter
+ * gradients are accumulated in FP64 in every precision. The handle holds the initial state afterwards.
+ * Environment: GP_ADJOINT_CHECK=<n> scans the adjoint after every transposed step, returns GP_E_NAN at the first
+ * non-finite value (naming the step, the cell and the state there) and prints the largest |adjoint| every n
+ * steps to stderr. Non-isothermal and moving-frame runs are rejected by gp_adjoint_forward. */
 int gp_adjoint_backward(gp_handle *h, const double *gphi_T, const double *gmu_T, double *gphi_0, double *gmu_0,
                         double *gtheta);
 
