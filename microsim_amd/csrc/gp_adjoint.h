@@ -84,20 +84,40 @@ __device__ __forceinline__ int bulk_q(const double *phi, const double *dphi, int
   return b;
 }
 
+/* bulk_q from the byte planes of the evaluation state: the pure-phase code (a cell that is not pure falls back to
+ * phi) and the candidate flags (delta-phi is read only at candidates; it is +0 at every other cell). Same result. */
+template <int N, int NC>
+__device__ __forceinline__ int bulk_q_codes(const double *phi, const double *dphi, const int8_t *pcode,
+                                            const int8_t *cand, int64_t c, int64_t fs, int64_t ps) {
+  int b = pcode[c];
+  if (b < 0) {
+    for (int a = 0; a < N; a++)
+      if (b < 0 && phi[a * fs + c] == 1.0) b = a;
+    if (b < 0) return -1;
+  }
+  const int64_t o[5] = {0, ps, -ps, 1, -1};
+  for (int i = 0; i < 5; i++)
+    if (cand[c + o[i]])
+      for (int a = 0; a < N - 1; a++)
+        if (dphi[a * fs + c + o[i]] != 0.0) return -1;
+  return b;
+}
+
 /* cbar of every cell of the allocated grid (zero outside the interior and for cells that are not bulk), and the
- * list of the interior cells that are not bulk and have a nonzero incoming adjoint (stage Q by dual numbers) */
+ * list of the interior cells that are not bulk and have a nonzero incoming adjoint (stage Q by reverse mode) */
 template <int N, int NC>
 __global__ __launch_bounds__(256) void k_adj_cbar(const Model<double> *__restrict__ mp, const double *__restrict__ phi,
                                                   const double *__restrict__ dphi, const double *__restrict__ gphi_n,
                                                   const double *__restrict__ gmu_n, double *__restrict__ cbar,
-                                                  int8_t *__restrict__ bq, int32_t *__restrict__ qlist, int *qcount,
-                                                  int64_t fs, int64_t ps, int64_t mx, int64_t my) {
+                                                  int8_t *__restrict__ bq, const int8_t *__restrict__ pcode,
+                                                  const int8_t *__restrict__ cand, int32_t *__restrict__ qlist,
+                                                  int *qcount, int64_t fs, int64_t ps, int64_t mx, int64_t my) {
   const int64_t y = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
   const int64_t x = (int64_t)blockIdx.y * 4 + (threadIdx.x >> 6);
   const bool valid = x < mx + 6 && y < my + 6;
   const int64_t c = x * ps + y + kYOff;
   const bool inner = valid && x >= 3 && x <= mx + 2 && y >= 3 && y <= my + 2;
-  const int b = inner ? bulk_q<N, NC>(phi, dphi, c, fs, ps) : -1;
+  const int b = inner ? bulk_q_codes<N, NC>(phi, dphi, pcode, cand, c, fs, ps) : -1;
   bool dual = false;
   if (valid) {
     bq[c] = (int8_t)b;
@@ -349,8 +369,9 @@ __device__ __forceinline__ bool p_candidate(const double *phi, int64_t c, int64_
  * (phi_delta, FP64, as the step does) over the list in full wavefronts. */
 template <int N>
 __global__ __launch_bounds__(256) void k_adj_codes(const double *__restrict__ phi, double *__restrict__ dphi,
-                                                   int8_t *__restrict__ pcode, int32_t *__restrict__ plist, int *pcount,
-                                                   int64_t fs, int64_t ps, int64_t mx, int64_t my) {
+                                                   int8_t *__restrict__ pcode, int8_t *__restrict__ cflag,
+                                                   int32_t *__restrict__ plist, int *pcount, int64_t fs, int64_t ps,
+                                                   int64_t mx, int64_t my) {
   const int64_t y = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
   const int64_t x = (int64_t)blockIdx.y * 4 + (threadIdx.x >> 6);
   const int64_t c = x * ps + y + kYOff;
@@ -368,6 +389,7 @@ __global__ __launch_bounds__(256) void k_adj_codes(const double *__restrict__ ph
     cand = p_candidate<N>(phi, c, fs, ps);
     for (int a = 0; a < N; a++) dphi[a * fs + c] = 0.0;
   }
+  if (x < mx + 6 && y < my + 6) cflag[c] = cand ? 1 : 0;
   list_append(cand, (int32_t)c, plist, pcount);
 }
 

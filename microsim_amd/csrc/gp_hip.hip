@@ -823,7 +823,8 @@ int gp_adjoint_backward(gp_handle *h, const double *gphi_T, const double *gmu_T,
               hipMalloc((void **)&A->plist, (size_t)h->fs * sizeof(int32_t)) == hipSuccess &&
               hipMalloc((void **)&A->counts, 2 * sizeof(int)) == hipSuccess &&
               hipMalloc((void **)&A->pcode, (size_t)h->fs) == hipSuccess &&
-              hipMalloc((void **)&A->bq, (size_t)h->fs) == hipSuccess;
+              hipMalloc((void **)&A->bq, (size_t)h->fs) == hipSuccess &&
+              hipMalloc((void **)&A->cand, (size_t)h->fs) == hipSuccess;
     for (int64_t j = 0; ok && j < std::min(A->every, A->nsteps); j++) {
       double *b = nullptr;
       ok = hipMalloc((void **)&b, state_bytes(h)) == hipSuccess;

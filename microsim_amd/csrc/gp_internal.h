@@ -174,10 +174,11 @@ struct Adjoint {
   /* per allocated cell of the evaluation state: the pure phase of phi (b when phi = e_b exactly, else -1) and the
    * bulk phase of stage Q (bulk_q, -1 for the other cells) */
   int8_t *pcode = nullptr, *bq = nullptr;
+  int8_t *cand = nullptr; /* 1 at the delta-phi candidates (k_adj_codes); every other cell's delta-phi is +0 */
   ~Adjoint() {
     for (double *b : ckpt) (void)hipFree(b);
     for (double *b : seg) (void)hipFree(b);
-    void *bufs[] = {g[0], g[1], gdphi, gth, cbar, qlist, plist, counts, st64, dphi64, model64, scan, pcode, bq};
+    void *bufs[] = {g[0], g[1], gdphi, gth, cbar, qlist, plist, counts, st64, dphi64, model64, scan, pcode, bq, cand};
     for (void *b : bufs)
       if (b) (void)hipFree(b);
   }

@@ -505,7 +505,8 @@ static int adj_step_t(gp_handle *h, const double *phi, const double *mu, const d
   HIPCHK(h, hipMemsetAsync(A->counts, 0, sizeof(int), h->stream));
   const Model<double> *md = (const Model<double> *)model;
   hipLaunchKernelGGL((k_adj_cbar<N, NC>), gall, blk, 0, h->stream, md, phi, dphi, gn, gn + (size_t)N * fs, A->cbar,
-                     A->bq, A->qlist, A->counts, fs, h->ps, mx, my);
+                     A->bq, (const int8_t *)A->pcode, (const int8_t *)A->cand, A->qlist, A->counts, fs, h->ps, mx,
+                     my);
   hipLaunchKernelGGL((k_adj_bulk<N, NC>), gall, blk, 0, h->stream, md, phi, mu, (const int8_t *)A->bq,
                      (const int8_t *)A->pcode, (const double *)A->cbar, gn, gc, gc + (size_t)N * fs, A->gdphi, A->gth,
                      3 * (N * (N - 1) / 2), fs, h->ps, mx, my);
@@ -529,8 +530,8 @@ static int adj_dphi_t(gp_handle *h, const double *phi, const double *mu, double 
   const int64_t mx = h->p.mesh_x, my = h->p.mesh_y;
   HIPCHK(h, hipMemsetAsync(A->counts + 1, 0, sizeof(int), h->stream));
   const dim3 gall((unsigned)((h->ry + 63) / 64), (unsigned)((h->rx + 3) / 4));
-  hipLaunchKernelGGL((k_adj_codes<N>), gall, dim3(256), 0, h->stream, phi, dphi, A->pcode, A->plist, A->counts + 1,
-                     h->fs, h->ps, mx, my);
+  hipLaunchKernelGGL((k_adj_codes<N>), gall, dim3(256), 0, h->stream, phi, dphi, A->pcode, A->cand, A->plist,
+                     A->counts + 1, h->fs, h->ps, mx, my);
   hipLaunchKernelGGL((k_dlist<N, NC, double, P2>), dim3(kListBlocks), dim3(256), 0, h->stream,
                      (const Model<double> *)model, phi, mu, dphi, h->fs, h->ps, (const int32_t *)A->plist,
                      (const int *)A->counts + 1);
