@@ -14,6 +14,10 @@
  *   gp_read_vtk         <- readfromfile_serial2D[_binary] (GPs/solverloop/file_writer.h:47-70, 256-416)
  *   gp_write_h5 / gp_read_h5 <- writetofile_mpi2D_hdf5 / readfromfile_mpi2D_hdf5 (GPm/solverloop/file_writer.h)
  *   gp_write_xdmf       <- write_xdmf_xml (GPm/write_xdmf.c)
+ *   gp_rank_block       <- the MPI driver's decomposition (GPm/solverloop/mpiinfo_xy.h:37-56, 323-326)
+ *   gp_write_rank_vtk / gp_read_rank_vtk <- writetofile_mpi2D[_binary] / readfromfile_mpi2D[_binary]
+ *                          (GPm/solverloop/file_writer.h:52-75, 119-142, 339-533, 731-869)
+ *   gp_reconstruct      <- the reconstruct tool (GPm/reconstruct.c)
  *   gp_write_echo       <- the <infile>.out / <infile>.bd echo files
  *                          (GPs/functions/reading_input_parameters.h:310-516, read_boundary_conditions.h:195-207)
  */
@@ -127,6 +131,25 @@ int gp_read_h5(const gp_input *in, const char *path, double *phi, double *mu);
 /* The XDMF description of an HDF5 snapshot (write_xdmf_xml, GPm/write_xdmf.c:66-146, 2-D form): h5_name is the
  * file name the XML points at, relative to the XML file. */
 int gp_write_xdmf(const gp_input *in, const char *path, const char *h5_name);
+
+/* Block of rank `rank` in the MPI driver's workers_x x workers_y decomposition of the halo-inclusive grid
+ * (GPm/solverloop/mpiinfo_xy.h:37-56, 323-326): rows[X], rows[Y] and their offsets; rank r is at
+ * (r / workers_y, r % workers_y). GP_E_ARG when a block would be empty. rows or offset may be NULL. */
+int gp_rank_block(const gp_params *p, int workers_x, int workers_y, int rank, int64_t rows[2], int64_t offset[2]);
+
+/* The MPI driver's per-rank snapshot <dir>/Processor_<r>/<out>_<t>.vtk for every r < workers_x * workers_y
+ * (writetofile_mpi2D[_binary] / write_cells_vtk_2D_mpi[binary], GPm/solverloop/file_writer.h:52-75, 339-533),
+ * creating the Processor_<r> folders; ASCII unless in->ascii is 0. T_rows as for gp_write_vtk. The reader
+ * (readfromfile_mpi2D[_binary] / read_cells_vtk_2D_mpi[binary], :119-142, 731-869) takes the phi and mu of nranks such files, placed by each file's own
+ * counts and offsets; every halo-inclusive cell must come from exactly one file. */
+int gp_write_rank_vtk(const gp_input *in, const char *dir, const char *out, int64_t t, int workers_x, int workers_y,
+                      const double *phi, const double *mu, const double *T_rows);
+int gp_read_rank_vtk(const gp_input *in, const char *dir, const char *out, int64_t t, int nranks, double *phi,
+                     double *mu);
+
+/* The reconstruct tool for one time (GPm/reconstruct.c:88-275): joins the nranks per-rank files of time t into
+ * <dir>/<out>_<t>.vtk in its ASCII or binary layout (:277-533). */
+int gp_reconstruct(const gp_input *in, const char *dir, const char *out, int nranks, int64_t t);
 
 /* The same for every cell of a halo-inclusive SoA state (phi [N][n], mu [NC][n]) into c [NC][n], n = ncells. */
 void gp_composition_grid(const gp_params *p, const double *phi, const double *mu, double *c, int64_t ncells);

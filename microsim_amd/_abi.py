@@ -127,6 +127,12 @@ def _declare_host(lib):
     lib.gp_write_h5.argtypes = [C.POINTER(GPInput), C.c_char_p, _dp, _dp, _dp]
     lib.gp_read_h5.argtypes = [C.POINTER(GPInput), C.c_char_p, _dp, _dp]
     lib.gp_write_xdmf.argtypes = [C.POINTER(GPInput), C.c_char_p, C.c_char_p]
+    lib.gp_rank_block.argtypes = [C.POINTER(GPParams), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int64),
+                                  C.POINTER(C.c_int64)]
+    lib.gp_write_rank_vtk.argtypes = [C.POINTER(GPInput), C.c_char_p, C.c_char_p, C.c_int64, C.c_int, C.c_int, _dp,
+                                      _dp, _dp]
+    lib.gp_read_rank_vtk.argtypes = [C.POINTER(GPInput), C.c_char_p, C.c_char_p, C.c_int64, C.c_int, _dp, _dp]
+    lib.gp_reconstruct.argtypes = [C.POINTER(GPInput), C.c_char_p, C.c_char_p, C.c_int, C.c_int64]
     lib.gp_composition_at.argtypes = [C.POINTER(GPParams), _dp, _dp, C.c_int, C.c_double]
     lib.gp_composition_at.restype = C.c_double
     lib.gp_temperature_rows.argtypes = [C.POINTER(GPParams), C.c_int64, C.c_int64, _dp]

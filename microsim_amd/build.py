@@ -46,7 +46,7 @@ def build_host():
     os.makedirs(LIB, exist_ok=True)
     _run(["gcc"] + HOST_CFLAGS + ["-shared", "-I" + INC, "-o", os.path.join(LIB, "libgp_host.so"),
                                   os.path.join(HERE, "host", "gp_host.c"), os.path.join(HERE, "host", "gp_h5.c"),
-                                  "-lm"])
+                                  os.path.join(HERE, "host", "gp_ranks.c"), "-lm"])
 
 
 def combos():
@@ -90,8 +90,16 @@ def build_driver():
                                   "-L" + LIB, "-lgp_host", "-lgp_hip", "-Wl,-rpath,$ORIGIN", "-lm"])
 
 
+def build_reconstruct():
+    """The reconstruct tool (joins the MPI layout's per-rank snapshots); host library only."""
+    _run(["gcc"] + HOST_CFLAGS + ["-I" + INC, "-o", os.path.join(LIB, "reconstruct"),
+                                  os.path.join(HERE, "host", "reconstruct.c"), "-L" + LIB, "-lgp_host",
+                                  "-Wl,-rpath,$ORIGIN", "-lm"])
+
+
 def build_all(host_only: bool = False):
     build_host()
+    build_reconstruct()
     if not host_only:
         build_hip()
         build_driver()

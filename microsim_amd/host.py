@@ -124,3 +124,46 @@ def composition(params: _abi.GPParams, phi: np.ndarray, mu: np.ndarray) -> np.nd
     c = np.empty((mu.shape[0],) + phi.shape[1:])
     _abi.host_lib().gp_composition_grid(C.byref(params), _abi.dptr(phi), _abi.dptr(mu), _abi.dptr(c), n)
     return c
+
+
+def rank_block(params: _abi.GPParams, workers_x: int, workers_y: int, rank: int):
+    """((rows_X, rows_Y), (offset_X, offset_Y)) of one rank in the MPI driver's decomposition of the halo-inclusive
+    grid."""
+    rows, off = (C.c_int64 * 2)(), (C.c_int64 * 2)()
+    if _abi.host_lib().gp_rank_block(C.byref(params), workers_x, workers_y, rank, rows, off) != 0:
+        raise ValueError("a %d x %d decomposition leaves a worker without rows" % (workers_x, workers_y))
+    return (rows[0], rows[1]), (off[0], off[1])
+
+
+def write_rank_vtk(case_inp: _abi.GPInput, data_dir: str, out: str, t: int, workers_x: int, workers_y: int,
+                   phi: np.ndarray, mu: np.ndarray, T_rows: np.ndarray = None) -> None:
+    """The MPI driver's per-rank snapshot files <data_dir>/Processor_<r>/<out>_<t>.vtk."""
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    mu = np.ascontiguousarray(mu, dtype=np.float64)
+    if case_inp.p.tempgrady and T_rows is None:
+        raise ValueError("a non-isothermal snapshot needs the row temperatures (T_rows)")
+    T = None if T_rows is None else np.ascontiguousarray(T_rows, dtype=np.float64)
+    rank_block(case_inp.p, workers_x, workers_y, 0)
+    rc = _abi.host_lib().gp_write_rank_vtk(C.byref(case_inp), data_dir.encode(), out.encode(), t, workers_x,
+                                           workers_y, _abi.dptr(phi), _abi.dptr(mu),
+                                           None if T is None else _abi.dptr(T))
+    if rc != 0:
+        raise IOError("cannot write the per-rank files of %s_%d in %s (%d)" % (out, t, data_dir, rc))
+
+
+def read_rank_vtk(case_inp: _abi.GPInput, data_dir: str, out: str, t: int, nranks: int):
+    """(phi, mu), halo-inclusive, from nranks per-rank snapshot files."""
+    phi, mu = empty_state(case_inp)
+    rc = _abi.host_lib().gp_read_rank_vtk(C.byref(case_inp), data_dir.encode(), out.encode(), t, nranks,
+                                          _abi.dptr(phi), _abi.dptr(mu))
+    if rc != 0:
+        raise IOError("cannot read the per-rank files of %s_%d in %s (%d)" % (out, t, data_dir, rc))
+    return phi, mu
+
+
+def reconstruct(case_inp: _abi.GPInput, data_dir: str, out: str, nranks: int, t: int) -> str:
+    """Joins the per-rank files of time t into <data_dir>/<out>_<t>.vtk (the reconstruct tool); returns the path."""
+    rc = _abi.host_lib().gp_reconstruct(C.byref(case_inp), data_dir.encode(), out.encode(), nranks, t)
+    if rc != 0:
+        raise IOError("cannot reconstruct %s_%d from %d ranks in %s (%d)" % (out, t, nranks, data_dir, rc))
+    return os.path.join(data_dir, "%s_%d.vtk" % (out, t))

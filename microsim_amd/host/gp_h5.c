@@ -14,6 +14,7 @@
  * object headers with continuation messages, contiguous or compact layout, binary64 of either byte order.
  */
 #include "gp_host.h"
+#include "gp_snapshot.h"
 
 #include <stdint.h>
 #include <stdio.h>
@@ -45,15 +46,14 @@ static void put64(wbuf *w, size_t at, uint64_t v) {
 }
 static size_t pad8(size_t n) { return (n + 7) & ~(size_t)7; }
 
-/* the fields of a snapshot, in the reference's order */
-typedef struct {
-  char name[GP_NAMELEN + 16];
-  const double *data; /* rows_x * rows_y values; NULL: computed into tmp */
-  double *tmp;
-} h5field;
+typedef gp_field h5field;
 
-static int snapshot_fields(const gp_input *in, const double *phi, const double *mu, const double *T_rows,
-                           h5field *f, int *nf) {
+void gp_snapshot_free(gp_field *f, int nf) {
+  for (int i = 0; i < nf; i++) free(f[i].tmp);
+}
+
+int gp_snapshot_fields(const gp_input *in, const double *phi, const double *mu, const double *T_rows, gp_field *f,
+                       int *nf) {
   const gp_params *p = &in->p;
   const int N = p->nphases, NC = p->ncomp - 1;
   const int64_t rx = gp_rows_x(p), ry = gp_rows_y(p), n = rx * ry;
@@ -113,12 +113,12 @@ int gp_write_h5(const gp_input *in, const char *path, const double *phi, const d
   const gp_params *p = &in->p;
   if (p->tempgrady && !T_rows) return GP_E_ARG;
   const uint64_t rx = (uint64_t)gp_rows_x(p), ry = (uint64_t)gp_rows_y(p), dbytes = rx * ry * 8;
-  h5field f[GP_MAXP + 2 * GP_MAXNC + 1];
-  int nf = 0, rc = snapshot_fields(in, phi, mu, T_rows, f, &nf);
+  h5field f[GP_SNAPSHOT_MAXF];
+  int nf = 0, rc = gp_snapshot_fields(in, phi, mu, T_rows, f, &nf);
   if (rc) return rc;
   if (nf > 2 * H5_LEAF_K) rc = GP_E_ARG;
   /* links sorted by name, as a symbol-table node keeps them */
-  h5field s[GP_MAXP + 2 * GP_MAXNC + 1];
+  h5field s[GP_SNAPSHOT_MAXF];
   memcpy(s, f, sizeof(h5field) * (size_t)nf);
   qsort(s, (size_t)nf, sizeof(h5field), cmp_names);
 
@@ -127,7 +127,7 @@ int gp_write_h5(const gp_input *in, const char *path, const double *phi, const d
   const size_t root_oh = sb + sb_size, root_oh_size = 16 + 8 + 16;
   const size_t btree = root_oh + root_oh_size, btree_size = 24 + (2 * H5_NODE_K + 1) * 8 + 2 * H5_NODE_K * 8;
   const size_t heap = btree + btree_size, heap_size = 32;
-  size_t heap_data_size = 8, name_off[GP_MAXP + 2 * GP_MAXNC + 1];
+  size_t heap_data_size = 8, name_off[GP_SNAPSHOT_MAXF];
   for (int i = 0; i < nf; i++) {
     name_off[i] = heap_data_size;
     heap_data_size += pad8(strlen(s[i].name) + 1);
@@ -140,7 +140,7 @@ int gp_write_h5(const gp_input *in, const char *path, const double *phi, const d
   wbuf w = {(unsigned char *)calloc(raw0, 1), raw0};
   if (!w.b || rc) {
     free(w.b);
-    for (int i = 0; i < nf; i++) free(f[i].tmp);
+    gp_snapshot_free(f, nf);
     return rc ? rc : GP_E_OOM;
   }
 
@@ -265,7 +265,7 @@ int gp_write_h5(const gp_input *in, const char *path, const double *phi, const d
   free(row);
   if (fp && fclose(fp) != 0) rc = GP_E_IO;
   free(w.b);
-  for (int i = 0; i < nf; i++) free(f[i].tmp);
+  gp_snapshot_free(f, nf);
   return rc;
 }
 

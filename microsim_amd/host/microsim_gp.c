@@ -2,15 +2,20 @@
  * microsim_gp -- drop-in driver for the serial grand-potential solver with the solver loop on the GPU.
  *
  *   ./microsim_gp Input.in Filling.in Output            (serial form, GPs/microsim_gp.c)
- *   ./microsim_gp Input.in Filling.in Output wx wy      (MPI form: the worker counts are accepted; one GPU
- *                                                        holds the whole domain)
+ *   ./microsim_gp Input.in Filling.in Output wx wy      (MPI form, GPm/microsim_gp.c: one GPU holds the whole
+ *                                                        domain; wx x wy sets the layout of the per-rank files)
  *
  * Same sequence as the reference main (GPs = Grand_potential_Finite_difference_2D_serial, GPs/microsim_gp.c:41-151):
  * mkdir DATA, read Input.in (+ <stem>.out / <stem>.bd echoes), fill the domain or read the restart snapshot,
  * apply the boundary conditions, write t = 0, NSMOOTH-1 smoothing steps, NTIMESTEPS steps with a snapshot every
- * SAVET (plus the DATA/shift.dat line) and the progress block every TRACK_PROGRESS. WRITEHDF5 = 1 selects the MPI
- * driver's HDF5 snapshots (GPm/microsim_gp.c:202-210, 368-377) with their XDMF descriptions. The loop body
+ * SAVET (plus the DATA/shift.dat line) and the progress block every TRACK_PROGRESS. The loop body
  * (GPs/microsim_gp.c:103-106) is gp_step; steps between two output events are issued as one batch.
+ *
+ * The MPI driver's snapshots: WRITEHDF5 = 1 selects its HDF5 files (GPm/microsim_gp.c:202-210, 368-377) with their
+ * XDMF descriptions; in the MPI form without WRITEHDF5 each snapshot is its per-rank VTK files
+ * DATA/Processor_<r>/<out>_<t>.vtk for the wx x wy decomposition (GPm/microsim_gp.c:113-118, 368-376), which the
+ * reconstruct tool joins. With either, the file at STARTTIME is written before the smoothing steps and again after
+ * them (GPm/microsim_gp.c:248-285), and a restart reads that driver's files (:201-210).
  *
  * Non-isothermal runs (ISOTHERMAL = 0 with Tempgrady) and the moving frame (Shift = 1) follow the same main: on a
  * restart the frame position is read back from DATA/shift.dat and the gradient offset moved by the elapsed time and
@@ -37,11 +42,16 @@ static void die(const char *what, const char *msg) {
   exit(1);
 }
 
-/* DATA/<out>_<t>.vtk, or with WRITEHDF5 = 1 the MPI driver's DATA/<out>_<t>.h5 plus its XDMF description
- * DATA/<out>_<t>.xmf (the reference writes the latter with its separate write_xdmf tool) */
+/* DATA/<out>_<t>.vtk; with WRITEHDF5 = 1 the MPI driver's DATA/<out>_<t>.h5 plus its XDMF description
+ * DATA/<out>_<t>.xmf (the reference writes the latter with its separate write_xdmf tool); else, with a wx x wy
+ * layout (wx > 0), the MPI driver's per-rank files */
 static void write_snapshot(const gp_input *in, const char *out, int64_t t, const double *phi, const double *mu,
-                           const double *T_rows) {
+                           const double *T_rows, int wx, int wy) {
   char name[1200], h5[1100];
+  if (!in->writehdf5 && wx > 0) {
+    if (gp_write_rank_vtk(in, "DATA", out, t, wx, wy, phi, mu, T_rows)) die("write", "DATA/Processor_*");
+    return;
+  }
   if (!in->writehdf5) {
     snprintf(name, sizeof name, "DATA/%s_%ld.vtk", out, (long)t);
     if (gp_write_vtk(in, name, phi, mu, T_rows)) die("write", name);
@@ -65,9 +75,16 @@ int main(int argc, char *argv[]) {
   if (gp_read_input(argv[1], &in, 0, err, sizeof err)) die("Input", err);
   if (gp_validate_input(&in, err, sizeof err)) die("Input", err);
   gp_write_echo(&in, argv[1]);
-  if (argc >= 6) printf("workers %s x %s requested; the domain runs on one GPU\n", argv[4], argv[5]);
-
   const gp_params *p = &in.p;
+  int wx = 0, wy = 0;
+  if (argc >= 6) {
+    wx = atoi(argv[4]);
+    wy = atoi(argv[5]);
+    if (gp_rank_block(p, wx, wy, 0, NULL, NULL)) die("workers", "the decomposition leaves a worker without rows");
+    printf("workers %d x %d requested; the domain runs on one GPU\n", wx, wy);
+  }
+  /* the MPI driver's main: its snapshot files, its STARTTIME writes */
+  const int mpi_main = wx > 0 || in.writehdf5;
   const int N = p->nphases, NC = p->ncomp - 1;
   const int64_t n = gp_rows_x(p) * gp_rows_y(p);
   double *phi = calloc((size_t)N * n, sizeof(double));
@@ -81,7 +98,13 @@ int main(int argc, char *argv[]) {
     if (gp_fill_domain(&in, argv[2], phi, mu, NULL, 0, err, sizeof err)) die("Filling", err);
   } else {
     snprintf(name, sizeof name, "DATA/%s_%ld.%s", argv[3], (long)in.starttime, in.writehdf5 ? "h5" : "vtk");
-    if ((in.writehdf5 ? gp_read_h5(&in, name, phi, mu) : gp_read_vtk(&in, name, phi, mu))) die("restart", name);
+    if (in.writehdf5) {
+      if (gp_read_h5(&in, name, phi, mu)) die("restart", name);
+    } else if (wx > 0) {
+      if (gp_read_rank_vtk(&in, "DATA", argv[3], in.starttime, wx * wy, phi, mu)) die("restart", "DATA/Processor_*");
+    } else if (gp_read_vtk(&in, name, phi, mu)) {
+      die("restart", name);
+    }
     if (in.shift) {
       /* the position of line STARTTIME/SAVET (0-based) of DATA/shift.dat; the last line read when it is shorter */
       FILE *fp = fopen("DATA/shift.dat", "r");
@@ -102,7 +125,8 @@ int main(int argc, char *argv[]) {
   }
   gp_apply_bc_host(p, phi, mu);
   gp_temperature_rows(p, 0, 0, T_rows);
-  if (in.restart == 0 || in.starttime == 0) write_snapshot(&in, argv[3], 0, phi, mu, T_rows);
+  if (mpi_main) write_snapshot(&in, argv[3], in.starttime, phi, mu, T_rows, wx, wy);
+  else if (in.restart == 0 || in.starttime == 0) write_snapshot(&in, argv[3], 0, phi, mu, T_rows, wx, wy);
 
   int precision = GP_FP64;
   const char *pe = getenv("GP_PRECISION");
@@ -111,7 +135,13 @@ int main(int argc, char *argv[]) {
   gp_handle *h = NULL;
   if (gp_create(p, precision, &h)) die("gp_create", gp_last_error(NULL));
   if (gp_upload(h, phi, mu)) die("gp_upload", gp_last_error(h));
-  if (in.nsmooth > 1 && gp_smooth(h, in.nsmooth - 1)) die("gp_smooth", gp_last_error(h));
+  if (in.nsmooth > 1) {
+    if (gp_smooth(h, in.nsmooth - 1)) die("gp_smooth", gp_last_error(h));
+    if (mpi_main) {
+      if (gp_download(h, phi, mu, NULL)) die("gp_download", gp_last_error(h));
+      write_snapshot(&in, argv[3], in.starttime, phi, mu, T_rows, wx, wy);
+    }
+  }
 
   const int64_t saveT = in.savet > 0 ? in.savet : in.ntimesteps + 1;
   const int64_t track = in.track_progress > 0 ? in.track_progress : in.ntimesteps + 1;
@@ -130,7 +160,7 @@ int main(int argc, char *argv[]) {
       int64_t shift_offset = 0;
       if (gp_download(h, phi, mu, NULL)) die("gp_download", gp_last_error(h));
       if (gp_get_frame(h, &shift_offset, T_rows)) die("gp_get_frame", gp_last_error(h));
-      write_snapshot(&in, argv[3], t + in.starttime, phi, mu, T_rows);
+      write_snapshot(&in, argv[3], t + in.starttime, phi, mu, T_rows, wx, wy);
       FILE *fp = fopen("DATA/shift.dat", "a");
       if (fp) {
         fprintf(fp, "%ld %ld\n", (long)(t + in.starttime), (long)(shift_offset + shift_position));

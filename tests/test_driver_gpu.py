@@ -23,6 +23,7 @@ from microsim_amd import _abi, configs
 pytestmark = pytest.mark.gpu
 
 DRIVER = os.path.join(_abi.LIBDIR, "microsim_gp")
+GATE = 1e-12
 
 
 def _progress(text):
@@ -173,6 +174,84 @@ def test_driver_hdf5_snapshots_and_restart():
         a = open(os.path.join(d1, "DATA", "out_%d.h5" % n), "rb").read()
         b = open(os.path.join(d2, "DATA", "out_%d.h5" % n), "rb").read()
         assert a == b
+    finally:
+        shutil.rmtree(d1, ignore_errors=True)
+        shutil.rmtree(d2, ignore_errors=True)
+
+
+def test_driver_mpi_form_per_rank_snapshots_restart_and_reconstruct():
+    """The MPI form `microsim_gp Input.in Filling.in out 2 3` writes the MPI driver's per-rank files
+    DATA/Processor_<r>/out_<t>.vtk (Grand_potential_Finite_difference_2D_MPI/microsim_gp.c:113-118, 248-285,
+    368-376): they hold the reference run's state at every saved step, the files at t = 0 hold the state
+    after the NSMOOTH - 1 smoothing steps (the MPI main writes t = 0 again after smoothing), a restart from the
+    per-rank files at m continues to the same files at n, and the reconstruct tool joins them into DATA/out_<t>.vtk
+    with the same values. State comparisons use the FP64 gate of the parity tests (relative 1e-12)."""
+    from gp_testlib import max_rel_err, oracle_smooth
+    from test_ranks_cpu import RECONSTRUCT, _parse_joined
+    from microsim_amd import host
+    print("[gpu] driver mpi form", flush=True)
+    sys.stdout.flush()
+    if not os.path.exists(DRIVER) or not os.path.exists(RECONSTRUCT):
+        pytest.fail("microsim_gp / reconstruct are not built: run __graft_entry__.build()")
+    name = "bin_smooth"
+    case, ref = load_golden(name)
+    inp = case.inp
+    saves = [s for s in ref["steps"] if s > 0]
+    n, m = max(saves), saves[0]
+    d1, d2 = tempfile.mkdtemp(prefix="gp_mpi1_"), tempfile.mkdtemp(prefix="gp_mpi2_")
+    try:
+        for d in (d1, d2):
+            for f in ("Input.in", "Filling.in"):
+                shutil.copy(os.path.join(GOLDEN, name, f), d)
+        _set_keys(os.path.join(d1, "Input.in"), NTIMESTEPS=n, SAVET=1, TRACK_PROGRESS=n)
+        r = subprocess.run([DRIVER, "Input.in", "Filling.in", "out", "2", "3"], cwd=d1, capture_output=True,
+                           text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        data = os.path.join(d1, "DATA")
+        assert not os.path.exists(os.path.join(data, "out_0.vtk"))
+        for i, s in enumerate(ref["steps"]):
+            if s == 0:
+                continue
+            phi, mu = host.read_rank_vtk(inp, data, "out", s, 6)
+            assert max_rel_err(phi[:, 3:-3, 3:-3], ref["phi"][i]) <= GATE, "step %d" % s
+            assert max_rel_err(mu[:, 3:-3, 3:-3], ref["mu"][i]) <= GATE, "step %d" % s
+        phi0, mu0 = case.phi.copy(), case.mu.copy()
+        oracle_smooth(case.params, phi0, mu0, int(inp.nsmooth) - 1)
+        phi, mu = host.read_rank_vtk(inp, data, "out", 0, 6)
+        assert max_rel_err(phi[:, 3:-3, 3:-3], phi0[:, 3:-3, 3:-3]) <= GATE
+        assert max_rel_err(mu[:, 3:-3, 3:-3], mu0[:, 3:-3, 3:-3]) <= GATE
+        assert max_rel_err(phi[:, 3:-3, 3:-3], case.phi[:, 3:-3, 3:-3]) > 1e-6, "t = 0 holds the smoothed state"
+        # restart from the per-rank files at m
+        for rk in range(6):
+            os.makedirs(os.path.join(d2, "DATA", "Processor_%d" % rk))
+            shutil.copy(os.path.join(data, "Processor_%d" % rk, "out_%d.vtk" % m),
+                        os.path.join(d2, "DATA", "Processor_%d" % rk))
+        _set_keys(os.path.join(d2, "Input.in"), NTIMESTEPS=n - m, SAVET=1, TRACK_PROGRESS=n, RESTART=1,
+                  STARTTIME=m, NSMOOTH=1)
+        r = subprocess.run([DRIVER, "Input.in", "Filling.in", "out", "2", "3"], cwd=d2, capture_output=True,
+                           text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        for rk in range(6):
+            a = open(os.path.join(data, "Processor_%d" % rk, "out_%d.vtk" % n), "rb").read()
+            b = open(os.path.join(d2, "DATA", "Processor_%d" % rk, "out_%d.vtk" % n), "rb").read()
+            assert a == b, "rank %d" % rk
+        # reconstruct every saved step
+        r = subprocess.run([RECONSTRUCT, "Input.in", "out", "6", "1", str(n)], cwd=d1, capture_output=True,
+                           text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        names = [inp.phases[a].value.decode() for a in range(case.N)]
+        comps = [inp.components[k].value.decode() for k in range(case.NC)]
+        for i, s in enumerate(ref["steps"]):
+            if s == 0:
+                continue
+            got = _parse_joined(os.path.join(data, "out_%d.vtk" % s), inp, bool(inp.ascii))
+            phi, mu = host.read_rank_vtk(inp, data, "out", s, 6)
+            for a, nm in enumerate(names):
+                assert np.array_equal(got[nm], phi[a, 3:-3, 3:-3]), "step %d %s" % (s, nm)
+                assert max_rel_err(got[nm], ref["phi"][i][a]) <= GATE, "step %d %s" % (s, nm)
+            for k, c in enumerate(comps):
+                assert np.array_equal(got["Mu_" + c], mu[k, 3:-3, 3:-3]), "step %d Mu_%s" % (s, c)
+                assert max_rel_err(got["Mu_" + c], ref["mu"][i][k]) <= GATE, "step %d Mu_%s" % (s, c)
     finally:
         shutil.rmtree(d1, ignore_errors=True)
         shutil.rmtree(d2, ignore_errors=True)
