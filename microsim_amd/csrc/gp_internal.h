@@ -36,7 +36,7 @@ constexpr int kNStatMax = 3 * GP_MAXP + 3 * GP_MAXNC + 1;
 #ifndef GP_LIST_BLOCKS
 #define GP_LIST_BLOCKS 2048
 #endif
-constexpr int kListBlocks = GP_LIST_BLOCKS; /* grid of the list-driven kernels (grid-stride loops), 4 wavefronts each */
+constexpr int kListBlocks = GP_LIST_BLOCKS; /* largest grid of the list-driven kernels (grid-stride loops), 4 wavefronts each */
 #ifndef GP_ADJ_BLOCKS
 #define GP_ADJ_BLOCKS 1024
 #endif
@@ -84,6 +84,7 @@ struct gp_handle {
   int32_t *clist, *ilist;    /* delta-phi candidates and interface cells of the current step (cell offsets) */
   int *lcount;               /* their lengths: two pairs, used by alternate steps (lcount + 2 * lsel) */
   int lsel;
+  int lgrid_d, lgrid_i;      /* grids of this handle's delta-phi and interface list kernels (0: not yet set) */
   int *flags[2];            /* per-tile purity flags of the last two steps */
   int8_t *codes[2];         /* phase codes (pure phase b, or -1) of every cell: codes[ccur] of the current state */
   int ccur;

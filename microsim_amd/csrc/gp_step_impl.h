@@ -16,6 +16,19 @@ namespace gpk {
 /* ------------------------------------------------------------------------------------------------ kernels */
 
 /* TG: non-isothermal, th holds the coefficients of every allocated row (gp_frame.h) */
+/* Grid of a list-driven kernel (a grid-stride loop over a list, 256-thread blocks): the blocks the device holds at
+ * once, its occupancy times the compute units, at most kListBlocks (the bound the statistics partials are sized
+ * for). One round of resident wavefronts: none waits for a slot while others finish their share of the list. */
+template <class K>
+static int list_grid(const gp_handle *h, K kernel) {
+  int nb = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, 256, 0) != hipSuccess || nb < 1) return kListBlocks;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device) != hipSuccess || cus < 1)
+    return kListBlocks;
+  const int64_t g = (int64_t)nb * cus;
+  return (int)(g < kListBlocks ? g : kListBlocks);
+}
+
 template <int N, int NC, typename R, bool SMOOTH, bool P2, bool TG = false>
 __global__ __launch_bounds__(kBX *kBY) void k_phi(const Model<R> *__restrict__ mp, const R *__restrict__ phi,
                                                   const R *__restrict__ mu, R *__restrict__ dphi, int64_t fs,
@@ -330,7 +343,11 @@ static int launch_step_p(gp_handle *h, int mode) {
       /* statistics partials: k_cbulk's wavefronts, then k_ilist's */
       const int ct = (int)h->cell_nwaves;
       const unsigned cblk = (unsigned)((ct + 3) / 4);
-      const int64_t cstride = (int64_t)ct + 4 * kListBlocks;
+      if (!h->lgrid_d) {
+        h->lgrid_d = list_grid(h, k_dlist<N, NC, R, P2>);
+        h->lgrid_i = list_grid(h, k_ilist<N, NC, R, P2, true>);
+      }
+      const int64_t cstride = (int64_t)ct + 4 * h->lgrid_i;
       /* this step's list lengths are zero (the previous step's k_cbulk, or the set-up, cleared them); this
        * step's k_cbulk clears the other pair for the next step, which the previous step's kernels read */
       int *lc = h->lcount + 2 * h->lsel, *lnext = h->lcount + 2 * (1 - h->lsel);
@@ -354,17 +371,17 @@ static int launch_step_p(gp_handle *h, int mode) {
       {
         /* delta-phi of the listed candidates, full wavefronts */
         ProfScope ps(h, GP_KERNEL_DPHI);
-        hipLaunchKernelGGL((k_dlist<N, NC, R, P2>), dim3(kListBlocks), dim3(256), 0, h->stream, m, phi, mu, dphi,
+        hipLaunchKernelGGL((k_dlist<N, NC, R, P2>), dim3(h->lgrid_d), dim3(256), 0, h->stream, m, phi, mu, dphi,
                            h->fs, h->ps, (const int32_t *)h->clist, (const int *)lc + 1);
       }
       {
         ProfScope ps(h, GP_KERNEL_CIFACE);
         if (stats)
-          hipLaunchKernelGGL((k_ilist<N, NC, R, P2, true>), dim3(kListBlocks), dim3(256), 0, h->stream, m, phi, mu,
+          hipLaunchKernelGGL((k_ilist<N, NC, R, P2, true>), dim3(h->lgrid_i), dim3(256), 0, h->stream, m, phi, mu,
                              dphi, codes, phi_o, mu_o, codes_o, h->maps, h->partials + ct, cstride, h->fs, h->ps, mx,
                              my, (const int32_t *)h->ilist, (const int *)lc);
         else
-          hipLaunchKernelGGL((k_ilist<N, NC, R, P2, false>), dim3(kListBlocks), dim3(256), 0, h->stream, m, phi, mu,
+          hipLaunchKernelGGL((k_ilist<N, NC, R, P2, false>), dim3(h->lgrid_i), dim3(256), 0, h->stream, m, phi, mu,
                              dphi, codes, phi_o, mu_o, codes_o, h->maps, h->partials + ct, cstride, h->fs, h->ps, mx,
                              my, (const int32_t *)h->ilist, (const int *)lc);
       }
@@ -532,7 +549,8 @@ static int adj_dphi_t(gp_handle *h, const double *phi, const double *mu, double 
   const dim3 gall((unsigned)((h->ry + 63) / 64), (unsigned)((h->rx + 3) / 4));
   hipLaunchKernelGGL((k_adj_codes<N>), gall, dim3(256), 0, h->stream, phi, dphi, A->pcode, A->cand, A->plist,
                      A->counts + 1, h->fs, h->ps, mx, my);
-  hipLaunchKernelGGL((k_dlist<N, NC, double, P2>), dim3(kListBlocks), dim3(256), 0, h->stream,
+  static const int grid = list_grid(h, k_dlist<N, NC, double, P2>);
+  hipLaunchKernelGGL((k_dlist<N, NC, double, P2>), dim3(grid), dim3(256), 0, h->stream,
                      (const Model<double> *)model, phi, mu, dphi, h->fs, h->ps, (const int32_t *)A->plist,
                      (const int *)A->counts + 1);
   HIPCHK(h, hipGetLastError());
@@ -661,7 +679,11 @@ static int launch_step_c(gp_handle *h, int mode) {
                        0, h->stream, phi, plo, h->codes[h->ccur], fs, h->ps, h->rx, h->ry);
   const int ct = (int)h->cell_nwaves;
   const unsigned cblk = (unsigned)((ct + 3) / 4);
-  const int64_t cstride = (int64_t)ct + 4 * kListBlocks;
+  if (!h->lgrid_d) {
+    h->lgrid_d = list_grid(h, k_dlist_c<N, NC, P2>);
+    h->lgrid_i = list_grid(h, k_ilist_c<N, NC, P2, true>);
+  }
+  const int64_t cstride = (int64_t)ct + 4 * h->lgrid_i;
   int *lc = h->lcount + 2 * h->lsel, *lnext = h->lcount + 2 * (1 - h->lsel);
   h->lsel = 1 - h->lsel;
   const int64_t nring = 2 * (my + 2) + 2 * mx;
@@ -681,17 +703,17 @@ static int launch_step_c(gp_handle *h, int mode) {
   }
   {
     ProfScope ps(h, GP_KERNEL_DPHI);
-    hipLaunchKernelGGL((k_dlist_c<N, NC, P2>), dim3(kListBlocks), dim3(256), 0, h->stream, md, phi, plo, mu, mlo, dphi,
+    hipLaunchKernelGGL((k_dlist_c<N, NC, P2>), dim3(h->lgrid_d), dim3(256), 0, h->stream, md, phi, plo, mu, mlo, dphi,
                        fs, h->ps, (const int32_t *)h->clist, (const int *)lc + 1);
   }
   {
     ProfScope ps(h, GP_KERNEL_CIFACE);
     if (stats)
-      hipLaunchKernelGGL((k_ilist_c<N, NC, P2, true>), dim3(kListBlocks), dim3(256), 0, h->stream, md, phi, plo, mu,
+      hipLaunchKernelGGL((k_ilist_c<N, NC, P2, true>), dim3(h->lgrid_i), dim3(256), 0, h->stream, md, phi, plo, mu,
                          mlo, (const double *)dphi, codes, phi_o, plo_o, mu_o, mlo_o, codes_o, h->maps,
                          h->partials + ct, cstride, fs, h->ps, mx, my, (const int32_t *)h->ilist, (const int *)lc);
     else
-      hipLaunchKernelGGL((k_ilist_c<N, NC, P2, false>), dim3(kListBlocks), dim3(256), 0, h->stream, md, phi, plo, mu,
+      hipLaunchKernelGGL((k_ilist_c<N, NC, P2, false>), dim3(h->lgrid_i), dim3(256), 0, h->stream, md, phi, plo, mu,
                          mlo, (const double *)dphi, codes, phi_o, plo_o, mu_o, mlo_o, codes_o, h->maps,
                          h->partials + ct, cstride, fs, h->ps, mx, my, (const int32_t *)h->ilist, (const int *)lc);
   }
