@@ -12,6 +12,11 @@ and a summary with:
 - the adjoint's device memory for the state copies it keeps.
 
     python -m microsim_amd.calibrate --mesh 2048 --steps 1000 --iters 15 --precision fp32
+
+A long run can be split over several processes: with ``--state FILE`` the optimiser state (log-parameters, Adam
+moments, step size, history) is written after every iteration and read back at the start, and ``--budget-s``
+ends a process before an iteration that would not finish within that many seconds; the next process with the same
+arguments continues where it stopped (the target is recomputed, bit for bit the same).
 """
 from __future__ import annotations
 
@@ -49,7 +54,11 @@ def main(argv=None, summary_out: dict = None) -> int:
     ap.add_argument("--lr-decay", type=float, default=0.8, help="step size factor per iteration")
     ap.add_argument("--precision", choices=("fp32c", "fp32", "fp64"), default="fp32")
     ap.add_argument("--offset", type=float, default=0.2, help="relative offset of the starting parameters")
+    ap.add_argument("--state", default=None, help="optimiser state file: read at the start, written every iteration")
+    ap.add_argument("--budget-s", type=float, default=0.0,
+                    help="stop before an iteration that would end later than this many seconds after the start")
     a = ap.parse_args(argv)
+    t_start = time.perf_counter()
     prec = {"fp32c": _abi.GP_FP32C, "fp32": _abi.GP_FP32, "fp64": _abi.GP_FP64}[a.precision]
     case = _case(a.mesh, a.seeds)
     names = adjoint.theta_names(case.N, case.NC + 1)
@@ -79,7 +88,37 @@ def main(argv=None, summary_out: dict = None) -> int:
     ncheck = (a.steps + every - 1) // every
     adj_bytes = (ncheck + every) * state_bytes + 3 * (case.N + case.NC) * case.phi.shape[1] * case.phi.shape[2] * 8
     hist = []
-    for it in range(a.iters + 1):
+    it0 = 0
+    if a.state and os.path.exists(a.state):
+        st = json.load(open(a.state))
+        with torch.no_grad():
+            z.copy_(torch.tensor(st["z"], dtype=z.dtype))
+        if st["adam_step"] > 0:
+            opt.state[z] = {"step": torch.tensor(float(st["adam_step"])),
+                            "exp_avg": torch.tensor(st["exp_avg"], dtype=z.dtype),
+                            "exp_avg_sq": torch.tensor(st["exp_avg_sq"], dtype=z.dtype)}
+        for g in opt.param_groups:
+            g["lr"] = st["lr"]
+        hist, it0 = st["hist"], st["iter"]
+
+    def save_state(next_it):
+        if not a.state:
+            return
+        ad = opt.state.get(z, {})
+        st = {"iter": next_it, "z": z.detach().tolist(), "lr": opt.param_groups[0]["lr"],
+              "adam_step": int(ad["step"]) if ad else 0,
+              "exp_avg": ad["exp_avg"].tolist() if ad else None, "exp_avg_sq": ad["exp_avg_sq"].tolist() if ad else None,
+              "hist": hist}
+        tmp = a.state + ".tmp"
+        json.dump(st, open(tmp, "w"))
+        os.replace(tmp, a.state)
+
+    for it in range(it0, a.iters + 1):
+        if a.budget_s > 0 and hist:
+            per_it = hist[-1]["forward_s"] + hist[-1]["backward_s"]
+            if time.perf_counter() - t_start + 1.2 * per_it > a.budget_s:
+                print(json.dumps({"stopped_before_iter": it, "state": a.state}), flush=True)
+                return 0
         th = base.clone()
         th[idx] = torch.exp(z)
         t1 = time.perf_counter()
@@ -101,6 +140,7 @@ def main(argv=None, summary_out: dict = None) -> int:
             sched.step()
         # the trajectory's device buffers go with its graph (the last iteration takes no backward)
         del loss, phiT
+        save_state(it + 1)
     fw = np.mean([h["forward_s"] for h in hist[:-1]]) if len(hist) > 1 else float("nan")
     bw = np.mean([h["backward_s"] for h in hist[:-1]]) if len(hist) > 1 else float("nan")
     summary = {
