@@ -239,7 +239,8 @@ const char *gp_version(void);
 #define GP_KERNEL_CBULK 8    /* update of the bulk cells (cell-classified step) */
 #define GP_KERNEL_CIFACE 9   /* update of the interface cells (cell-classified step, k_ilist) */
 #define GP_KERNEL_CALL 10    /* one whole gp_step / gp_smooth call (GP_PROFILE_CALLS mode) */
-#define GP_NKERNELS 11
+#define GP_KERNEL_HALO 11    /* halo copies of the new state (cell-classified step, k_halo) */
+#define GP_NKERNELS 12
 /* mode: GP_PROFILE_OFF, GP_PROFILE_LAUNCHES (any other nonzero value) or GP_PROFILE_CALLS */
 int gp_profile_enable(gp_handle *h, int mode);
 int gp_profile_read(gp_handle *h, double *avg_ms, int64_t *count);

@@ -30,7 +30,7 @@ GP_SOLVE_BINARY = 1
 GP_SOLVE_TERNARY = 2
 GP_SOLVE_DILUTE = 3
 
-GP_NKERNELS = 11
+GP_NKERNELS = 12
 GP_PROFILE_OFF = 0
 GP_PROFILE_LAUNCHES = 1
 GP_PROFILE_CALLS = 2

@@ -152,10 +152,10 @@ __global__ __launch_bounds__(256) void k_ilist_c(const Model<double> *__restrict
                                                  const int8_t *__restrict__ mlo, const double *__restrict__ dphi,
                                                  const int8_t *__restrict__ codes_in, float *__restrict__ phi_o,
                                                  float *__restrict__ plo_o, float *__restrict__ mu_o,
-                                                 int8_t *__restrict__ mlo_o, int8_t *__restrict__ codes_out, Maps maps,
+                                                 int8_t *__restrict__ mlo_o, int8_t *__restrict__ codes_out,
                                                  double *__restrict__ partials, int64_t pstride, int64_t fs,
-                                                 int64_t ps, int64_t mx, int64_t my,
-                                                 const int32_t *__restrict__ ilist, const int *__restrict__ icount) {
+                                                 int64_t ps, const int32_t *__restrict__ ilist,
+                                                 const int *__restrict__ icount) {
   constexpr int NS = 3 * N + 3 * NC + 1;
   constexpr int NSA = STATS ? NS : 1;
   const Model<double> &m = *mp;
@@ -167,7 +167,7 @@ __global__ __launch_bounds__(256) void k_ilist_c(const Model<double> *__restrict
   const int n = *icount;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int64_t c = ilist[i];
-    const int64_t x = (c - kYOff) / ps, y = (c - kYOff) % ps;
+    const int64_t y = (c - kYOff) % ps;
     int8_t cd[5][5];
 #pragma unroll
     for (int di = -2; di <= 2; di++)
@@ -215,15 +215,7 @@ __global__ __launch_bounds__(256) void k_ilist_c(const Model<double> *__restrict
       mu_o[k * fs + c] = mh[k];
       mlo_o[k * fs + c] = mq[k];
     }
-    const int8_t code = (int8_t)pure_phase<N>(pd);
-    codes_out[c] = code;
-    if (y < 7 || y > my - 2 || x < 7 || x > mx - 2) {
-      write_images(codes_out, 1, &code, 0, maps, x, y, 0, ps);
-      write_images(phi_o, N, ph, 0, maps, x, y, fs, ps);
-      write_images(plo_o, N, pl, 0, maps, x, y, fs, ps);
-      write_images(mu_o, NC, mh, 1, maps, x, y, fs, ps);
-      write_images(mlo_o, NC, mq, 1, maps, x, y, fs, ps);
-    }
+    codes_out[c] = (int8_t)pure_phase<N>(pd);
     if (STATS) stats_add<N, NC>(st, pd, d, md, dm);
   }
   if (!STATS) return;

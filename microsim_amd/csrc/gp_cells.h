@@ -17,7 +17,7 @@
  * reference by construction) on the listed cells in a grid-stride loop, so that every lane of every wavefront
  * works on an interface cell (a march row has about one in five) and no wavefront holds more than its share.
  * Every interior cell is written by exactly one of the two. They read the phase codes of the current state and
- * write those of the new one (a second buffer).
+ * write those of the new one (a second buffer). k_halo then fills the new state's halo from its interior.
  */
 #pragma once
 
@@ -45,7 +45,7 @@ __device__ __forceinline__ int bulk_of(const int8_t *cs, int r, int l) {
 /* per-wavefront tile geometry, shared by the two kernels */
 struct CellTile {
   int64_t y, yl, xa, xb;
-  bool out_lane, img_y;
+  bool out_lane;
   __device__ CellTile(int w, int lane, int64_t mx, int64_t my, int seg_len, int nstrips) {
     const int strip = w % nstrips, seg = w / nstrips;
     const int64_t ry = my + 6;
@@ -54,7 +54,6 @@ struct CellTile {
     xa = 3 + (int64_t)seg * seg_len;
     xb = (xa + seg_len - 1 < mx + 2) ? xa + seg_len - 1 : mx + 2;
     out_lane = lane >= 2 && lane <= 61 && y <= my + 2;
-    img_y = out_lane && (y < 7 || y > my - 2);
   }
 };
 
@@ -78,12 +77,11 @@ constexpr bool kLeanBulk = sizeof(R) < sizeof(double);
  * once per tile) and the phi extrema (e_b on every cell) are left to the caller, once per tile. Otherwise the
  * caller calls it on the bulk lanes only, and it folds the full statistics. */
 template <int N, int NC, typename R, bool P2, bool STATS, bool UNIFORM, int NSA, bool C = false>
-__device__ __forceinline__ void cbulk_cell(const Model<R> &m, int b, int64_t x, int64_t c, bool img,
-                                           const CellTile &t, const R *m0,
+__device__ __forceinline__ void cbulk_cell(const Model<R> &m, int b, int64_t c, const R *m0,
                                            const R *m1, const R *m2, const R *mud, const R *muu, R divjat0,
                                            R *__restrict__ phi_o, R *__restrict__ mu_o,
-                                           int8_t *__restrict__ codes_out, const Maps &maps, int64_t fs, int64_t ps,
-                                           int64_t mx, bool kept, bool store, bool live, R (&st)[NSA],
+                                           int8_t *__restrict__ codes_out, int64_t fs, bool kept, bool store,
+                                           bool live, R (&st)[NSA], unsigned &seen,
                                            const int8_t *q1 = nullptr, float *__restrict__ plo_o = nullptr,
                                            int8_t *__restrict__ mlo_o = nullptr) {
   using Vw = View<R, P2, false, kFastOf<R>>;
@@ -228,16 +226,6 @@ __device__ __forceinline__ void cbulk_cell(const Model<R> &m, int b, int64_t x, 
 #pragma unroll
       for (int k = 0; k < NC; k++) mlo_o[k * fs + c] = qn[k];
     }
-    if (img) {
-      write_images(codes_out, 1, &code, 0, maps, x, t.y, 0, ps);
-      write_images(phi_o, N, pe, 0, maps, x, t.y, fs, ps);
-      write_images(mu_o, NC, munew, 1, maps, x, t.y, fs, ps);
-      if constexpr (C) {
-        const float z[N] = {};
-        write_images(plo_o, N, z, 0, maps, x, t.y, fs, ps);
-        write_images(mlo_o, NC, qn, 1, maps, x, t.y, fs, ps);
-      }
-    }
   }
   if (STATS && UNIFORM && live) {
 #pragma unroll
@@ -249,12 +237,9 @@ __device__ __forceinline__ void cbulk_cell(const Model<R> &m, int b, int64_t x, 
     }
   } else if (STATS) {
     /* stats_add with delta-phi = 0 and phi = e_b: the squared increments of phi add +0 to sums that start at +0
-     * and e_b is finite, so only the extrema of phi and the mu terms remain */
-#pragma unroll
-    for (int a = 0; a < N; a++) {
-      st[a] = fmax(st[a], pe[a]);
-      st[N + a] = fmin(st[N + a], pe[a]);
-    }
+     * and e_b is finite, so only the extrema of phi and the mu terms remain; the extrema of phi over cells that are
+     * each some e_b follow from the set of phases seen (folded once, bulk_seen_fold) */
+    seen |= 1u << b;
 #pragma unroll
     for (int k = 0; k < NC; k++) {
       st[2 * N + k] = fmax(st[2 * N + k], munew[k]);
@@ -262,6 +247,20 @@ __device__ __forceinline__ void cbulk_cell(const Model<R> &m, int b, int64_t x, 
       st[3 * N + 2 * NC + k] += deltamu[k] * deltamu[k];
       if (!isfinite(munew[k])) st[NSA - 1] += R(1.0);
     }
+  }
+}
+
+/* the phi extrema of a lane's bulk cells of the phases in `seen` (bit b: a cell equal to e_b) into st: over such
+ * cells phi_a is 1 where b = a and 0 elsewhere */
+template <int N, typename R, int NSA>
+__device__ __forceinline__ void bulk_seen_fold(R (&st)[NSA], unsigned seen) {
+#pragma unroll
+  for (int a = 0; a < N; a++) {
+    const bool own = (seen >> a) & 1u, other = (seen & ~(1u << a)) != 0u;
+    if (own) st[a] = fmax(st[a], R(1.0));
+    else if (other) st[a] = fmax(st[a], R(0.0));
+    if (other) st[N + a] = fmin(st[N + a], R(0.0));
+    else if (own) st[N + a] = fmin(st[N + a], R(1.0));
   }
 }
 
@@ -324,7 +323,7 @@ __device__ __forceinline__ void dring_cell(int64_t i, const int8_t *__restrict__
 template <int N, int NC, typename R, bool P2, bool STATS, bool C = false>
 __global__ __launch_bounds__(256) void k_cbulk(const Model<R> *__restrict__ mp, const R *__restrict__ mu,
                                                const int8_t *__restrict__ codes_in, R *__restrict__ phi_o,
-                                               R *__restrict__ mu_o, int8_t *__restrict__ codes_out, Maps maps,
+                                               R *__restrict__ mu_o, int8_t *__restrict__ codes_out,
                                                double *__restrict__ partials, int64_t pstride, int64_t fs, int64_t ps,
                                                int64_t mx, int64_t my, int seg_len, int nstrips, int nwaves,
                                                int keep, int32_t *__restrict__ ilist, int32_t *__restrict__ clist,
@@ -352,6 +351,7 @@ __global__ __launch_bounds__(256) void k_cbulk(const Model<R> *__restrict__ mp, 
   int8_t *cs = cs_all[wv];
   R st[NSA];
   if (STATS) stats_init<N, NC>(st);
+  unsigned seen = 0u; /* phases of the lane's bulk cells in mixed tiles (their phi extrema, folded at the end) */
   const CellTile t(w, lane, mx, my, seg_len, nstrips);
   const int64_t c0 = t.yl + kYOff;
   /* the next step's list lengths start at zero (the kernels that read them last ran before this one) */
@@ -414,7 +414,6 @@ __global__ __launch_bounds__(256) void k_cbulk(const Model<R> *__restrict__ mp, 
     const int pso = (int)ps, c0i = (int)c0, cyi = (int)(t.y + kYOff);
     const int xa = (int)t.xa, xb = (int)t.xb, mxi = (int)mx;
     const int xlast = (xb + 1) * pso;
-    const bool img_x = xa < 7 || xb > mxi - 2; /* the tile holds rows whose halo images are written */
     auto rows = [&](auto uniform) {
       constexpr bool UNI = decltype(uniform)::value;
       int xoff = xa * pso;
@@ -426,7 +425,6 @@ __global__ __launch_bounds__(256) void k_cbulk(const Model<R> *__restrict__ mp, 
           const int x = x0 + j;
           const bool live = x <= xb;
           const int64_t c = (int64_t)xoff + cyi;
-          const bool img = t.img_y || (img_x && (x < 7 || x > mxi - 2));
           const R *m0 = ring[(j + RG - 1) % RG], *m1 = ring[j], *m2 = ring[(j + 1) % RG];
           R mud[NC], muu[NC];
 #pragma unroll
@@ -435,10 +433,9 @@ __global__ __launch_bounds__(256) void k_cbulk(const Model<R> *__restrict__ mp, 
             muu[k] = lane_up(m1[k]);
           }
           if (UNI) {
-            cbulk_cell<N, NC, R, P2, STATS, true, NSA, C>(m, tb, x, c, img, t, m0, m1, m2, mud, muu, divjat0, phi_o,
-                                                          mu_o, codes_out, maps, fs, ps, mx,
-                                                          keep && kc[j] == (uint8_t)tb, live && t.out_lane, live, st,
-                                                          qring[j], plo_o, mlo_o);
+            cbulk_cell<N, NC, R, P2, STATS, true, NSA, C>(m, tb, c, m0, m1, m2, mud, muu, divjat0, phi_o, mu_o,
+                                                          codes_out, fs, keep && kc[j] == (uint8_t)tb,
+                                                          live && t.out_lane, live, st, seen, qring[j], plo_o, mlo_o);
           } else if (live) {
             const int bl = t.out_lane ? bulk_of(cs, x - xa + 2, lane) : -1;
             {
@@ -458,10 +455,9 @@ __global__ __launch_bounds__(256) void k_cbulk(const Model<R> *__restrict__ mp, 
               ccnt += __popcll(mc);
             }
             if (bl >= 0)
-              cbulk_cell<N, NC, R, P2, STATS, false, NSA, C>(m, bl, x, c, img, t, m0, m1, m2, mud, muu, divjat0,
-                                                             phi_o, mu_o, codes_out, maps, fs, ps, mx,
-                                                             keep && kc[j] == (uint8_t)bl, true, true, st, qring[j],
-                                                             plo_o, mlo_o);
+              cbulk_cell<N, NC, R, P2, STATS, false, NSA, C>(m, bl, c, m0, m1, m2, mud, muu, divjat0, phi_o, mu_o,
+                                                             codes_out, fs, keep && kc[j] == (uint8_t)bl, true, true,
+                                                             st, seen, qring[j], plo_o, mlo_o);
           }
           /* refill: row x + D + 1 into the slot of row x - 1, the output codes of row x + D */
           const int xn = min(xoff + (D + 1) * pso, xlast);
@@ -520,9 +516,78 @@ __global__ __launch_bounds__(256) void k_cbulk(const Model<R> *__restrict__ mp, 
     clist[wbase[1][wv] + i] = (int32_t)((t.xa + (e >> 6)) * ps + (t.y - lane + (e & 63)) + kYOff);
   }
   if (!STATS) return;
+  bulk_seen_fold<N>(st, seen);
   stats_partials<N, NC, true>(st, partials, pstride, w, nwaves, lane);
 }
 
+
+/* The halo of the new state, after k_cbulk and k_ilist wrote every interior cell: each halo cell takes the value
+ * of its proxy, as write_images scatters it (apply_boundary_conditions as a gather). Halo row X = hx[k] has the
+ * proxy row sx[f][k] when that is an interior row and at most one k' < k shares it (write_images takes the first
+ * two images of a row); an interior row is its own (likewise for columns). Field group f = 0 holds phi, its low
+ * parts and the phase codes, f = 1 mu and its extensions. A halo cell without a proxy is left as it is. One thread
+ * per halo cell: the six halo rows over the full width, then the six halo columns over the interior rows. plo and
+ * mlo are the compensated mode's planes (null otherwise). */
+__device__ __forceinline__ int64_t halo_source(const int32_t *src, const int32_t *h, int64_t X, int64_t lo,
+                                               int64_t hi) {
+  if (X >= lo && X <= hi) return X;
+  int64_t s = -1;
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    if (h[k] != X) continue;
+    int before = 0;
+    for (int j = 0; j < k; j++) before += src[j] == src[k];
+    if (src[k] >= lo && src[k] <= hi && before < 2) s = src[k];
+  }
+  return s;
+}
+
+template <int N, int NC, typename R>
+__global__ __launch_bounds__(256) void k_halo(R *__restrict__ phi, R *__restrict__ mu, int8_t *__restrict__ codes,
+                                              float *__restrict__ plo, int8_t *__restrict__ mlo, Maps maps,
+                                              int64_t fs, int64_t ps, int64_t mx, int64_t my) {
+  const int64_t ry = my + 6, nrow = 6 * ry, ncol = 6 * mx;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrow + ncol) return;
+  int64_t X, Y;
+  if (i < nrow) {
+    const int64_t r = i / ry;
+    X = r < 3 ? r : mx + r;
+    Y = i % ry;
+  } else {
+    const int64_t k = i - nrow, q = k / mx;
+    Y = q < 3 ? q : my + q;
+    X = 3 + k % mx;
+  }
+  const int64_t c = X * ps + Y + kYOff;
+  {
+    const int64_t sx = halo_source(maps.sx[0], maps.hx, X, 3, mx + 2);
+    const int64_t sy = halo_source(maps.sy[0], maps.hy, Y, 3, my + 2);
+    if (sx >= 0 && sy >= 0) {
+      const int64_t s = sx * ps + sy + kYOff;
+      codes[c] = codes[s];
+#pragma unroll
+      for (int a = 0; a < N; a++) phi[a * fs + c] = phi[a * fs + s];
+      if (plo) {
+#pragma unroll
+        for (int a = 0; a < N; a++) plo[a * fs + c] = plo[a * fs + s];
+      }
+    }
+  }
+  {
+    const int64_t sx = halo_source(maps.sx[1], maps.hx, X, 3, mx + 2);
+    const int64_t sy = halo_source(maps.sy[1], maps.hy, Y, 3, my + 2);
+    if (sx >= 0 && sy >= 0) {
+      const int64_t s = sx * ps + sy + kYOff;
+#pragma unroll
+      for (int k = 0; k < NC; k++) mu[k * fs + c] = mu[k * fs + s];
+      if (mlo) {
+#pragma unroll
+        for (int k = 0; k < NC; k++) mlo[k * fs + c] = mlo[k * fs + s];
+      }
+    }
+  }
+}
 
 /* The general update of the interface cells listed by k_cbulk (any order): a grid-stride loop of full
  * wavefronts. Each cell gathers its stencil from memory in one batch: phi on the 3x3 neighbourhood, mu and
@@ -533,10 +598,10 @@ template <int N, int NC, typename R, bool P2, bool STATS>
 __global__ __launch_bounds__(256) void k_ilist(const Model<R> *__restrict__ mp, const R *__restrict__ phi,
                                                const R *__restrict__ mu, const R *__restrict__ dphi,
                                                const int8_t *__restrict__ codes_in, R *__restrict__ phi_o,
-                                               R *__restrict__ mu_o, int8_t *__restrict__ codes_out, Maps maps,
+                                               R *__restrict__ mu_o, int8_t *__restrict__ codes_out,
                                                double *__restrict__ partials, int64_t pstride, int64_t fs,
-                                               int64_t ps, int64_t mx, int64_t my,
-                                               const int32_t *__restrict__ ilist, const int *__restrict__ icount) {
+                                               int64_t ps, const int32_t *__restrict__ ilist,
+                                               const int *__restrict__ icount) {
   constexpr int NS = 3 * N + 3 * NC + 1;
   constexpr int NSA = STATS ? NS : 1;
   const Model<R> &m = *mp;
@@ -548,7 +613,7 @@ __global__ __launch_bounds__(256) void k_ilist(const Model<R> *__restrict__ mp, 
   const int n = *icount;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int64_t c = ilist[i];
-    const int64_t x = (c - kYOff) / ps, y = (c - kYOff) % ps;
+    const int64_t y = (c - kYOff) % ps;
     /* codes of the diamond: slot (di, dj) for |di| + |dj| <= 2 */
     int8_t cd[5][5];
 #pragma unroll
@@ -591,13 +656,7 @@ __global__ __launch_bounds__(256) void k_ilist(const Model<R> *__restrict__ mp, 
     for (int a = 0; a < N; a++) phi_o[a * fs + c] = pn[a];
 #pragma unroll
     for (int k = 0; k < NC; k++) mu_o[k * fs + c] = mn[k];
-    const int8_t code = (int8_t)pure_phase<N>(pn);
-    codes_out[c] = code;
-    if (y < 7 || y > my - 2 || x < 7 || x > mx - 2) {
-      write_images(codes_out, 1, &code, 0, maps, x, y, 0, ps);
-      write_images(phi_o, N, pn, 0, maps, x, y, fs, ps);
-      write_images(mu_o, NC, mn, 1, maps, x, y, fs, ps);
-    }
+    codes_out[c] = (int8_t)pure_phase<N>(pn);
     if (STATS) stats_add<N, NC>(st, pn, d, mn, dm);
   }
   if (!STATS) return;

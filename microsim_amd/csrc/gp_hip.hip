@@ -899,7 +899,7 @@ int gp_profile_enable(gp_handle *h, int enable) {
 
 const char *gp_kernel_name(int kind) {
   static const char *names[GP_NKERNELS] = {"k_phi", "k_mu", "k_stats", "k_march", "k_march_stats", "k_bulk",
-                                           "k_dphi", "adj_step", "k_cbulk", "k_ilist", "call"};
+                                           "k_dphi", "adj_step", "k_cbulk", "k_ilist", "call", "k_halo"};
   return (kind >= 0 && kind < GP_NKERNELS) ? names[kind] : "?";
 }
 

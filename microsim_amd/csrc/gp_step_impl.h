@@ -359,12 +359,12 @@ static int launch_step_p(gp_handle *h, int mode) {
         ProfScope ps(h, GP_KERNEL_CBULK);
         if (stats)
           hipLaunchKernelGGL((k_cbulk<N, NC, R, P2, true>), dim3(cblk + rblk), dim3(256), 0, h->stream, m, mu, codes,
-                             phi_o, mu_o, codes_o, h->maps, h->partials, cstride, h->fs, h->ps, mx, my, h->cell_seg,
+                             phi_o, mu_o, codes_o, h->partials, cstride, h->fs, h->ps, mx, my, h->cell_seg,
                              h->march_nstrips, ct, (int)h->prev_valid, h->ilist, h->clist, lc, lnext, (int)cblk,
                              (const int8_t *)nullptr, (float *)nullptr, (int8_t *)nullptr);
         else
           hipLaunchKernelGGL((k_cbulk<N, NC, R, P2, false>), dim3(cblk + rblk), dim3(256), 0, h->stream, m, mu, codes,
-                             phi_o, mu_o, codes_o, h->maps, h->partials, cstride, h->fs, h->ps, mx, my, h->cell_seg,
+                             phi_o, mu_o, codes_o, h->partials, cstride, h->fs, h->ps, mx, my, h->cell_seg,
                              h->march_nstrips, ct, (int)h->prev_valid, h->ilist, h->clist, lc, lnext, (int)cblk,
                              (const int8_t *)nullptr, (float *)nullptr, (int8_t *)nullptr);
       }
@@ -378,12 +378,19 @@ static int launch_step_p(gp_handle *h, int mode) {
         ProfScope ps(h, GP_KERNEL_CIFACE);
         if (stats)
           hipLaunchKernelGGL((k_ilist<N, NC, R, P2, true>), dim3(h->lgrid_i), dim3(256), 0, h->stream, m, phi, mu,
-                             dphi, codes, phi_o, mu_o, codes_o, h->maps, h->partials + ct, cstride, h->fs, h->ps, mx,
-                             my, (const int32_t *)h->ilist, (const int *)lc);
+                             dphi, codes, phi_o, mu_o, codes_o, h->partials + ct, cstride, h->fs, h->ps,
+                             (const int32_t *)h->ilist, (const int *)lc);
         else
           hipLaunchKernelGGL((k_ilist<N, NC, R, P2, false>), dim3(h->lgrid_i), dim3(256), 0, h->stream, m, phi, mu,
-                             dphi, codes, phi_o, mu_o, codes_o, h->maps, h->partials + ct, cstride, h->fs, h->ps, mx,
-                             my, (const int32_t *)h->ilist, (const int *)lc);
+                             dphi, codes, phi_o, mu_o, codes_o, h->partials + ct, cstride, h->fs, h->ps,
+                             (const int32_t *)h->ilist, (const int *)lc);
+      }
+      {
+        /* the new state's halo, from the interior cells both kernels wrote */
+        ProfScope ps(h, GP_KERNEL_HALO);
+        const int64_t nh = 6 * (my + 6) + 6 * mx;
+        hipLaunchKernelGGL((k_halo<N, NC, R>), dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, h->stream, phi_o, mu_o,
+                           codes_o, (float *)nullptr, (int8_t *)nullptr, h->maps, h->fs, h->ps, mx, my);
       }
       if (stats) {
         ProfScope ps(h, GP_KERNEL_STATS);
@@ -692,12 +699,12 @@ static int launch_step_c(gp_handle *h, int mode) {
     ProfScope ps(h, GP_KERNEL_CBULK);
     if (stats)
       hipLaunchKernelGGL((k_cbulk<N, NC, float, P2, true, true>), dim3(cblk + rblk), dim3(256), 0, h->stream, m, mu,
-                         codes, phi_o, mu_o, codes_o, h->maps, h->partials, cstride, fs, h->ps, mx, my, h->cell_seg,
+                         codes, phi_o, mu_o, codes_o, h->partials, cstride, fs, h->ps, mx, my, h->cell_seg,
                          h->march_nstrips, ct, (int)h->prev_valid, h->ilist, h->clist, lc, lnext, (int)cblk, mlo, plo_o,
                          mlo_o);
     else
       hipLaunchKernelGGL((k_cbulk<N, NC, float, P2, false, true>), dim3(cblk + rblk), dim3(256), 0, h->stream, m, mu,
-                         codes, phi_o, mu_o, codes_o, h->maps, h->partials, cstride, fs, h->ps, mx, my, h->cell_seg,
+                         codes, phi_o, mu_o, codes_o, h->partials, cstride, fs, h->ps, mx, my, h->cell_seg,
                          h->march_nstrips, ct, (int)h->prev_valid, h->ilist, h->clist, lc, lnext, (int)cblk, mlo, plo_o,
                          mlo_o);
   }
@@ -710,12 +717,18 @@ static int launch_step_c(gp_handle *h, int mode) {
     ProfScope ps(h, GP_KERNEL_CIFACE);
     if (stats)
       hipLaunchKernelGGL((k_ilist_c<N, NC, P2, true>), dim3(h->lgrid_i), dim3(256), 0, h->stream, md, phi, plo, mu,
-                         mlo, (const double *)dphi, codes, phi_o, plo_o, mu_o, mlo_o, codes_o, h->maps,
-                         h->partials + ct, cstride, fs, h->ps, mx, my, (const int32_t *)h->ilist, (const int *)lc);
+                         mlo, (const double *)dphi, codes, phi_o, plo_o, mu_o, mlo_o, codes_o, h->partials + ct,
+                         cstride, fs, h->ps, (const int32_t *)h->ilist, (const int *)lc);
     else
       hipLaunchKernelGGL((k_ilist_c<N, NC, P2, false>), dim3(h->lgrid_i), dim3(256), 0, h->stream, md, phi, plo, mu,
-                         mlo, (const double *)dphi, codes, phi_o, plo_o, mu_o, mlo_o, codes_o, h->maps,
-                         h->partials + ct, cstride, fs, h->ps, mx, my, (const int32_t *)h->ilist, (const int *)lc);
+                         mlo, (const double *)dphi, codes, phi_o, plo_o, mu_o, mlo_o, codes_o, h->partials + ct,
+                         cstride, fs, h->ps, (const int32_t *)h->ilist, (const int *)lc);
+  }
+  {
+    ProfScope ps(h, GP_KERNEL_HALO);
+    const int64_t nh = 6 * (my + 6) + 6 * mx;
+    hipLaunchKernelGGL((k_halo<N, NC, float>), dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, h->stream, phi_o, mu_o,
+                       codes_o, plo_o, mlo_o, h->maps, fs, h->ps, mx, my);
   }
   if (stats) {
     ProfScope ps(h, GP_KERNEL_STATS);
