@@ -363,15 +363,41 @@ __device__ __forceinline__ bool p_candidate(const double *phi, int64_t c, int64_
   return false;
 }
 
+/* Stage B transposed at cell (x, y) for field group f (the ncomp planes at g): a halo cell's adjoint goes to the
+ * interior cell it copies (x maps of the field for halo rows, y maps for halo columns, both for corners), and the
+ * halo adjoint is cleared. */
+__device__ __forceinline__ void adj_bc_cell(double *g, int ncomp, const Maps &maps, int f, int64_t x, int64_t y,
+                                            int64_t fs, int64_t ps) {
+  int64_t sxv = x, syv = y;
+  for (int k = 0; k < 6; k++) {
+    if (maps.hx[k] == x) sxv = maps.sx[f][k];
+    if (maps.hy[k] == y) syv = maps.sy[f][k];
+  }
+  if (sxv == x && syv == y) return; /* interior cell */
+  const int64_t c = x * ps + y + kYOff, s = sxv * ps + syv + kYOff;
+  for (int a = 0; a < ncomp; a++) {
+    const double v = g[a * fs + c];
+    if (v != 0.0) atomicAdd(&g[a * fs + s], v);
+    g[a * fs + c] = 0.0;
+  }
+}
+
 /* the pure-phase code of every allocated cell (pcode) and the list of the P candidates of [2, mesh+3]^2: a cell whose
  * five-point phi is not one pure phase throughout. Every other cell's delta-phi is +0 (no divergence, no clamp):
  * this pass stores +0 at every cell of [2, mesh+3]^2, and k_dlist then forms the candidates' delta-phi
  * (phi_delta, FP64, as the step does) over the list in full wavefronts. */
-template <int N>
+template <int N, int NC>
 __global__ __launch_bounds__(256) void k_adj_codes(const double *__restrict__ phi, double *__restrict__ dphi,
                                                    int8_t *__restrict__ pcode, int8_t *__restrict__ cflag,
-                                                   int32_t *__restrict__ plist, int *pcount, int64_t fs, int64_t ps,
+                                                   int32_t *__restrict__ plist, int *pcount, int *pclear,
+                                                   double *gfold, Maps maps, int *qclear, int64_t fs, int64_t ps,
                                                    int64_t mx, int64_t my) {
+  /* the other P slot (read by the previous transposed step, which ran before this pass) starts the next one at 0,
+   * and so does the stage-Q list that the coming transposed step fills */
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+    *pclear = 0;
+    *qclear = 0;
+  }
   const int64_t y = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
   const int64_t x = (int64_t)blockIdx.y * 4 + (threadIdx.x >> 6);
   const int64_t c = x * ps + y + kYOff;
@@ -389,7 +415,12 @@ __global__ __launch_bounds__(256) void k_adj_codes(const double *__restrict__ ph
     cand = p_candidate<N>(phi, c, fs, ps);
     for (int a = 0; a < N; a++) dphi[a * fs + c] = 0.0;
   }
-  if (x < mx + 6 && y < my + 6) cflag[c] = cand ? 1 : 0;
+  if (x < mx + 6 && y < my + 6) {
+    cflag[c] = cand ? 1 : 0;
+    /* stage B of the coming transposed step (its adjoint input gfold), which nothing here reads */
+    adj_bc_cell(gfold, N, maps, 0, x, y, fs, ps);
+    adj_bc_cell(gfold + (int64_t)N * fs, NC, maps, 1, x, y, fs, ps);
+  }
   list_append(cand, (int32_t)c, plist, pcount);
 }
 
@@ -426,25 +457,15 @@ __global__ __launch_bounds__(256) void k_adj_prev(const Model<double> *__restric
   theta_flush<NTH>(th, NTH, gtheta);
 }
 
-/* Stage B transposed for one field: every halo cell's adjoint goes to the interior cell it copies (x maps of
- * the field for halo rows, y maps for halo columns, both for corners), and the halo adjoint is cleared. */
-static __global__ __launch_bounds__(256) void k_adj_bc(double *g, int ncomp, Maps maps, int f, int64_t fs, int64_t ps,
+/* Stage B transposed over the allocated grid; blockIdx.z is the field group (0: the N phi planes at g, 1: the NC mu
+ * planes after them) */
+static __global__ __launch_bounds__(256) void k_adj_bc(double *g, int N, int NC, Maps maps, int64_t fs, int64_t ps,
                                                 int64_t rx, int64_t ry) {
+  const int f = (int)blockIdx.z;
   const int64_t y = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
   const int64_t x = (int64_t)blockIdx.y * 4 + (threadIdx.x >> 6);
   if (x >= rx || y >= ry) return;
-  int64_t sxv = x, syv = y;
-  for (int k = 0; k < 6; k++) {
-    if (maps.hx[k] == x) sxv = maps.sx[f][k];
-    if (maps.hy[k] == y) syv = maps.sy[f][k];
-  }
-  if (sxv == x && syv == y) return; /* interior cell */
-  const int64_t c = x * ps + y + kYOff, s = sxv * ps + syv + kYOff;
-  for (int a = 0; a < ncomp; a++) {
-    const double v = g[a * fs + c];
-    if (v != 0.0) atomicAdd(&g[a * fs + s], v);
-    g[a * fs + c] = 0.0;
-  }
+  adj_bc_cell(f ? g + (int64_t)N * fs : g, f ? NC : N, maps, f, x, y, fs, ps);
 }
 
 }  // namespace gpk

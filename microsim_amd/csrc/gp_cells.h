@@ -526,8 +526,8 @@ __global__ __launch_bounds__(256) void k_cbulk(const Model<R> *__restrict__ mp, 
  * proxy row sx[f][k] when that is an interior row and at most one k' < k shares it (write_images takes the first
  * two images of a row); an interior row is its own (likewise for columns). Field group f = 0 holds phi, its low
  * parts and the phase codes, f = 1 mu and its extensions. A halo cell without a proxy is left as it is. One thread
- * per halo cell: the six halo rows over the full width, then the six halo columns over the interior rows. plo and
- * mlo are the compensated mode's planes (null otherwise). */
+ * per halo cell: the six halo rows over the full width, then the six halo cells of each interior row. C: the
+ * compensated mode, whose planes plo and mlo are copied too. */
 __device__ __forceinline__ int64_t halo_source(const int32_t *src, const int32_t *h, int64_t X, int64_t lo,
                                                int64_t hi) {
   if (X >= lo && X <= hi) return X;
@@ -542,7 +542,7 @@ __device__ __forceinline__ int64_t halo_source(const int32_t *src, const int32_t
   return s;
 }
 
-template <int N, int NC, typename R>
+template <int N, int NC, typename R, bool C>
 __global__ __launch_bounds__(256) void k_halo(R *__restrict__ phi, R *__restrict__ mu, int8_t *__restrict__ codes,
                                               float *__restrict__ plo, int8_t *__restrict__ mlo, Maps maps,
                                               int64_t fs, int64_t ps, int64_t mx, int64_t my) {
@@ -555,9 +555,10 @@ __global__ __launch_bounds__(256) void k_halo(R *__restrict__ phi, R *__restrict
     X = r < 3 ? r : mx + r;
     Y = i % ry;
   } else {
-    const int64_t k = i - nrow, q = k / mx;
+    /* six consecutive threads take the six halo cells of one interior row (two runs of three adjacent elements) */
+    const int64_t k = i - nrow, q = k % 6;
     Y = q < 3 ? q : my + q;
-    X = 3 + k % mx;
+    X = 3 + k / 6;
   }
   const int64_t c = X * ps + Y + kYOff;
   {
@@ -568,7 +569,7 @@ __global__ __launch_bounds__(256) void k_halo(R *__restrict__ phi, R *__restrict
       codes[c] = codes[s];
 #pragma unroll
       for (int a = 0; a < N; a++) phi[a * fs + c] = phi[a * fs + s];
-      if (plo) {
+      if constexpr (C) {
 #pragma unroll
         for (int a = 0; a < N; a++) plo[a * fs + c] = plo[a * fs + s];
       }
@@ -581,7 +582,7 @@ __global__ __launch_bounds__(256) void k_halo(R *__restrict__ phi, R *__restrict
       const int64_t s = sx * ps + sy + kYOff;
 #pragma unroll
       for (int k = 0; k < NC; k++) mu[k * fs + c] = mu[k * fs + s];
-      if (mlo) {
+      if constexpr (C) {
 #pragma unroll
         for (int k = 0; k < NC; k++) mlo[k * fs + c] = mlo[k * fs + s];
       }

@@ -299,8 +299,7 @@ static int adj_check(gp_handle *h, int64_t t, int64_t every, const double *phi64
 static int adj_fold(gp_handle *h) {
   Adjoint *A = h->adj;
   const dim3 gall((unsigned)((h->ry + 63) / 64), (unsigned)((h->rx + 3) / 4));
-  hipLaunchKernelGGL(k_adj_bc, gall, dim3(256), 0, h->stream, A->g[1], h->N, h->maps, 0, h->fs, h->ps, h->rx, h->ry);
-  hipLaunchKernelGGL(k_adj_bc, gall, dim3(256), 0, h->stream, A->g[1] + (size_t)h->N * h->fs, h->NC, h->maps, 1, h->fs,
+  hipLaunchKernelGGL(k_adj_bc, dim3(gall.x, gall.y, 2), dim3(256), 0, h->stream, A->g[1], h->N, h->NC, h->maps, h->fs,
                      h->ps, h->rx, h->ry);
   HIPCHK(h, hipGetLastError());
   return GP_OK;
@@ -821,7 +820,8 @@ int gp_adjoint_backward(gp_handle *h, const double *gphi_T, const double *gmu_T,
               hipMalloc((void **)&A->cbar, (size_t)h->NC * h->fs * sizeof(double)) == hipSuccess &&
               hipMalloc((void **)&A->qlist, (size_t)h->fs * sizeof(int32_t)) == hipSuccess &&
               hipMalloc((void **)&A->plist, (size_t)h->fs * sizeof(int32_t)) == hipSuccess &&
-              hipMalloc((void **)&A->counts, 2 * sizeof(int)) == hipSuccess &&
+              hipMalloc((void **)&A->counts, 4 * sizeof(int)) == hipSuccess &&
+              hipMemsetAsync(A->counts, 0, 4 * sizeof(int), h->stream) == hipSuccess &&
               hipMalloc((void **)&A->pcode, (size_t)h->fs) == hipSuccess &&
               hipMalloc((void **)&A->bq, (size_t)h->fs) == hipSuccess &&
               hipMalloc((void **)&A->cand, (size_t)h->fs) == hipSuccess;

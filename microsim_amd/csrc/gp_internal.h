@@ -170,7 +170,9 @@ struct Adjoint {
    * transposed step is evaluated (the FP64 model's derivative at the FP32 trajectory) */
   double *st64 = nullptr, *dphi64 = nullptr;
   void *model64 = nullptr;
-  int *counts = nullptr;                      /* their lengths */
+  int *counts = nullptr;                      /* their lengths: [0] stage Q, [1] and [2] stage P (alternating) */
+  int psel = 0;                               /* the P slot the next adj_dphi fills is counts[1 + psel] */
+  int pslot = 1;                              /* the P slot the last adj_dphi filled */
   unsigned long long *scan = nullptr;         /* GP_ADJOINT_CHECK: first non-finite index, max |g| bits, theta flag */
   /* per allocated cell of the evaluation state: the pure phase of phi (b when phi = e_b exactly, else -1) and the
    * bulk phase of stage Q (bulk_q, -1 for the other cells) */

@@ -389,7 +389,7 @@ static int launch_step_p(gp_handle *h, int mode) {
         /* the new state's halo, from the interior cells both kernels wrote */
         ProfScope ps(h, GP_KERNEL_HALO);
         const int64_t nh = 6 * (my + 6) + 6 * mx;
-        hipLaunchKernelGGL((k_halo<N, NC, R>), dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, h->stream, phi_o, mu_o,
+        hipLaunchKernelGGL((k_halo<N, NC, R, false>), dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, h->stream, phi_o, mu_o,
                            codes_o, (float *)nullptr, (int8_t *)nullptr, h->maps, h->fs, h->ps, mx, my);
       }
       if (stats) {
@@ -522,11 +522,9 @@ static int adj_step_t(gp_handle *h, const double *phi, const double *mu, const d
   double *gn = A->g[1], *gc = A->g[0];
   const dim3 blk(256);
   const dim3 gall((unsigned)((h->ry + 63) / 64), (unsigned)((h->rx + 3) / 4));
-  hipLaunchKernelGGL(k_adj_bc, gall, blk, 0, h->stream, gn, N, h->maps, 0, fs, h->ps, h->rx, h->ry);
-  hipLaunchKernelGGL(k_adj_bc, gall, blk, 0, h->stream, gn + (size_t)N * fs, NC, h->maps, 1, fs, h->ps, h->rx, h->ry);
-  /* bulk cells of stage Q in closed form (plain stores over the whole allocated grid: no clearing pass), then the
-   * reverse-mode kernels add the rest; counts[1] is the P candidate list of adj_dphi_t */
-  HIPCHK(h, hipMemsetAsync(A->counts, 0, sizeof(int), h->stream));
+  /* stage B ran in adj_dphi_t's k_adj_codes pass, which also reset the stage-Q list length. The bulk cells of
+   * stage Q in closed form (plain stores over the whole allocated grid: no clearing pass), then the reverse-mode
+   * kernels add the rest; counts[pslot] is the P candidate list of the last adj_dphi_t */
   const Model<double> *md = (const Model<double> *)model;
   hipLaunchKernelGGL((k_adj_cbar<N, NC>), gall, blk, 0, h->stream, md, phi, dphi, gn, gn + (size_t)N * fs, A->cbar,
                      A->bq, (const int8_t *)A->pcode, (const int8_t *)A->cand, A->qlist, A->counts, fs, h->ps, mx,
@@ -540,26 +538,30 @@ static int adj_step_t(gp_handle *h, const double *phi, const double *mu, const d
                      gc + (size_t)N * fs, A->gdphi, A->gth, (const int32_t *)A->qlist, (const int *)A->counts, fs,
                      h->ps);
   hipLaunchKernelGGL((k_adj_prev<N, NC, P2>), gl, blk, 0, h->stream, md, phi, mu, (const double *)A->gdphi, gc,
-                     gc + (size_t)N * fs, A->gth, (const int32_t *)A->plist, (const int *)A->counts + 1, fs, h->ps);
+                     gc + (size_t)N * fs, A->gth, (const int32_t *)A->plist, (const int *)A->counts + A->pslot, fs,
+                     h->ps);
   HIPCHK(h, hipGetLastError());
   std::swap(A->g[0], A->g[1]);
   return GP_OK;
 }
 
 /* delta-phi (FP64 model) of the state an adjoint step is evaluated at, and the list of its P candidates (into
- * A->plist / A->counts[1]) */
+ * A->plist, its length into counts[1 + psel]; the slots alternate, so that no clearing call is needed) */
 template <int N, int NC, bool P2>
 static int adj_dphi_t(gp_handle *h, const double *phi, const double *mu, double *dphi, const void *model) {
   Adjoint *A = h->adj;
   const int64_t mx = h->p.mesh_x, my = h->p.mesh_y;
-  HIPCHK(h, hipMemsetAsync(A->counts + 1, 0, sizeof(int), h->stream));
+  const int slot = 1 + A->psel, other = 1 + (1 - A->psel);
+  A->psel = 1 - A->psel;
+  A->pslot = slot;
   const dim3 gall((unsigned)((h->ry + 63) / 64), (unsigned)((h->rx + 3) / 4));
-  hipLaunchKernelGGL((k_adj_codes<N>), gall, dim3(256), 0, h->stream, phi, dphi, A->pcode, A->cand, A->plist,
-                     A->counts + 1, h->fs, h->ps, mx, my);
+  /* with the codes, stage B of the transposed step that follows (its adjoint input A->g[1]) */
+  hipLaunchKernelGGL((k_adj_codes<N, NC>), gall, dim3(256), 0, h->stream, phi, dphi, A->pcode, A->cand, A->plist,
+                     A->counts + slot, A->counts + other, A->g[1], h->maps, A->counts, h->fs, h->ps, mx, my);
   static const int grid = list_grid(h, k_dlist<N, NC, double, P2>);
   hipLaunchKernelGGL((k_dlist<N, NC, double, P2>), dim3(grid), dim3(256), 0, h->stream,
                      (const Model<double> *)model, phi, mu, dphi, h->fs, h->ps, (const int32_t *)A->plist,
-                     (const int *)A->counts + 1);
+                     (const int *)A->counts + slot);
   HIPCHK(h, hipGetLastError());
   return GP_OK;
 }
@@ -727,7 +729,7 @@ static int launch_step_c(gp_handle *h, int mode) {
   {
     ProfScope ps(h, GP_KERNEL_HALO);
     const int64_t nh = 6 * (my + 6) + 6 * mx;
-    hipLaunchKernelGGL((k_halo<N, NC, float>), dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, h->stream, phi_o, mu_o,
+    hipLaunchKernelGGL((k_halo<N, NC, float, true>), dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, h->stream, phi_o, mu_o,
                        codes_o, plo_o, mlo_o, h->maps, fs, h->ps, mx, my);
   }
   if (stats) {
