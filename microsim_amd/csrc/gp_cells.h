@@ -543,11 +543,11 @@ __device__ __forceinline__ int64_t halo_source(const int32_t *src, const int32_t
 }
 
 template <int N, int NC, typename R, bool C>
-__global__ __launch_bounds__(256) void k_halo(R *__restrict__ phi, R *__restrict__ mu, int8_t *__restrict__ codes,
-                                              float *__restrict__ plo, int8_t *__restrict__ mlo, Maps maps,
-                                              int64_t fs, int64_t ps, int64_t mx, int64_t my) {
+__device__ __forceinline__ void halo_cell(int64_t i, R *__restrict__ phi, R *__restrict__ mu,
+                                          int8_t *__restrict__ codes, float *__restrict__ plo,
+                                          int8_t *__restrict__ mlo, const Maps &maps, int64_t fs, int64_t ps,
+                                          int64_t mx, int64_t my) {
   const int64_t ry = my + 6, nrow = 6 * ry, ncol = 6 * mx;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nrow + ncol) return;
   int64_t X, Y;
   if (i < nrow) {
@@ -588,6 +588,17 @@ __global__ __launch_bounds__(256) void k_halo(R *__restrict__ phi, R *__restrict
       }
     }
   }
+}
+
+/* number of halo cells halo_cell covers */
+__host__ __device__ constexpr int64_t halo_cells(int64_t mx, int64_t my) { return 6 * (my + 6) + 6 * mx; }
+
+template <int N, int NC, typename R, bool C>
+__global__ __launch_bounds__(256) void k_halo(R *__restrict__ phi, R *__restrict__ mu, int8_t *__restrict__ codes,
+                                              float *__restrict__ plo, int8_t *__restrict__ mlo, Maps maps,
+                                              int64_t fs, int64_t ps, int64_t mx, int64_t my) {
+  halo_cell<N, NC, R, C>((int64_t)blockIdx.x * blockDim.x + threadIdx.x, phi, mu, codes, plo, mlo, maps, fs, ps, mx,
+                         my);
 }
 
 /* The general update of the interface cells listed by k_cbulk (any order): a grid-stride loop of full

@@ -131,11 +131,10 @@ __global__ __launch_bounds__(kBX *kBY) void k_mu(const Model<R> *__restrict__ mp
 constexpr int kStatThreads = 1024;
 
 template <int N, int NC>
-__global__ __launch_bounds__(kStatThreads) void k_stats(const double *__restrict__ partials, int64_t stride,
-                                                        int64_t nparts, double *__restrict__ cum) {
+__device__ __forceinline__ void stats_fold_block(const double *__restrict__ partials, int64_t stride, int64_t nparts,
+                                                 double *__restrict__ cum, int i) {
   constexpr int NS = 3 * N + 3 * NC + 1;
   __shared__ double red[kStatThreads];
-  const int i = blockIdx.x;
   if (i >= NS) return;
   const bool mx = (i < N || (i >= 2 * N && i < 2 * N + NC));
   const bool mn = !mx && i < 2 * N + 2 * NC;
@@ -170,6 +169,31 @@ __global__ __launch_bounds__(kStatThreads) void k_stats(const double *__restrict
     else if (i < NS - 1) cum[i] = v;
     else cum[i] += v;
   }
+}
+
+template <int N, int NC>
+__global__ __launch_bounds__(kStatThreads) void k_stats(const double *__restrict__ partials, int64_t stride,
+                                                        int64_t nparts, double *__restrict__ cum) {
+  stats_fold_block<N, NC>(partials, stride, nparts, cum, (int)blockIdx.x);
+}
+
+/* The statistics fold and the halo pass of the cell-classified step in one launch (both small and latency-bound,
+ * and independent: the fold reads the partials, the halo pass the new state): blocks below the number of
+ * statistics fold one statistic each, the others copy kStatThreads halo cells each (k_halo). */
+template <int N, int NC, typename R, bool C>
+__global__ __launch_bounds__(kStatThreads) void k_stats_halo(const double *__restrict__ partials, int64_t stride,
+                                                             int64_t nparts, double *__restrict__ cum,
+                                                             R *__restrict__ phi, R *__restrict__ mu,
+                                                             int8_t *__restrict__ codes, float *__restrict__ plo,
+                                                             int8_t *__restrict__ mlo, Maps maps, int64_t fs,
+                                                             int64_t ps, int64_t mx, int64_t my) {
+  constexpr int NS = 3 * N + 3 * NC + 1;
+  if ((int)blockIdx.x < NS) {
+    stats_fold_block<N, NC>(partials, stride, nparts, cum, (int)blockIdx.x);
+    return;
+  }
+  halo_cell<N, NC, R, C>((int64_t)(blockIdx.x - NS) * kStatThreads + threadIdx.x, phi, mu, codes, plo, mlo, maps, fs,
+                         ps, mx, my);
 }
 
 /* Global solute and grand-potential energy of the current state (no reference counterpart; the FP64 oracle's
@@ -385,17 +409,18 @@ static int launch_step_p(gp_handle *h, int mode) {
                              dphi, codes, phi_o, mu_o, codes_o, h->partials + ct, cstride, h->fs, h->ps,
                              (const int32_t *)h->ilist, (const int *)lc);
       }
-      {
-        /* the new state's halo, from the interior cells both kernels wrote */
-        ProfScope ps(h, GP_KERNEL_HALO);
-        const int64_t nh = 6 * (my + 6) + 6 * mx;
-        hipLaunchKernelGGL((k_halo<N, NC, R, false>), dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, h->stream, phi_o, mu_o,
-                           codes_o, (float *)nullptr, (int8_t *)nullptr, h->maps, h->fs, h->ps, mx, my);
-      }
+      /* the new state's halo, from the interior cells both kernels wrote; with the statistics fold when asked */
+      const int64_t nh = halo_cells(mx, my);
       if (stats) {
         ProfScope ps(h, GP_KERNEL_STATS);
-        hipLaunchKernelGGL((k_stats<N, NC>), dim3(3 * N + 3 * NC + 1), dim3(kStatThreads), 0, h->stream,
-                           h->partials, cstride, cstride, h->stats_dev);
+        hipLaunchKernelGGL((k_stats_halo<N, NC, R, false>),
+                           dim3((unsigned)(3 * N + 3 * NC + 1 + (nh + kStatThreads - 1) / kStatThreads)),
+                           dim3(kStatThreads), 0, h->stream, h->partials, cstride, cstride, h->stats_dev, phi_o, mu_o,
+                           codes_o, (float *)nullptr, (int8_t *)nullptr, h->maps, h->fs, h->ps, mx, my);
+      } else {
+        ProfScope ps(h, GP_KERNEL_HALO);
+        hipLaunchKernelGGL((k_halo<N, NC, R, false>), dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, h->stream,
+                           phi_o, mu_o, codes_o, (float *)nullptr, (int8_t *)nullptr, h->maps, h->fs, h->ps, mx, my);
       }
       HIPCHK(h, hipGetLastError());
       h->cur = 1 - h->cur;
@@ -726,16 +751,17 @@ static int launch_step_c(gp_handle *h, int mode) {
                          mlo, (const double *)dphi, codes, phi_o, plo_o, mu_o, mlo_o, codes_o, h->partials + ct,
                          cstride, fs, h->ps, (const int32_t *)h->ilist, (const int *)lc);
   }
-  {
-    ProfScope ps(h, GP_KERNEL_HALO);
-    const int64_t nh = 6 * (my + 6) + 6 * mx;
-    hipLaunchKernelGGL((k_halo<N, NC, float, true>), dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, h->stream, phi_o, mu_o,
-                       codes_o, plo_o, mlo_o, h->maps, fs, h->ps, mx, my);
-  }
+  const int64_t nh = halo_cells(mx, my);
   if (stats) {
     ProfScope ps(h, GP_KERNEL_STATS);
-    hipLaunchKernelGGL((k_stats<N, NC>), dim3(3 * N + 3 * NC + 1), dim3(kStatThreads), 0, h->stream, h->partials,
-                       cstride, cstride, h->stats_dev);
+    hipLaunchKernelGGL((k_stats_halo<N, NC, float, true>),
+                       dim3((unsigned)(3 * N + 3 * NC + 1 + (nh + kStatThreads - 1) / kStatThreads)),
+                       dim3(kStatThreads), 0, h->stream, h->partials, cstride, cstride, h->stats_dev, phi_o, mu_o,
+                       codes_o, plo_o, mlo_o, h->maps, fs, h->ps, mx, my);
+  } else {
+    ProfScope ps(h, GP_KERNEL_HALO);
+    hipLaunchKernelGGL((k_halo<N, NC, float, true>), dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, h->stream,
+                       phi_o, mu_o, codes_o, plo_o, mlo_o, h->maps, fs, h->ps, mx, my);
   }
   HIPCHK(h, hipGetLastError());
   h->cur = 1 - h->cur;
