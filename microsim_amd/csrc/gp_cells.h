@@ -549,16 +549,17 @@ __device__ __forceinline__ void halo_cell(int64_t i, R *__restrict__ phi, R *__r
                                           int64_t mx, int64_t my) {
   const int64_t ry = my + 6, nrow = 6 * ry, ncol = 6 * mx;
   if (i >= nrow + ncol) return;
+  /* the index arithmetic in 32 bits (the halo has fewer than 2^31 cells; 64-bit division is a long sequence) */
   int64_t X, Y;
   if (i < nrow) {
-    const int64_t r = i / ry;
-    X = r < 3 ? r : mx + r;
-    Y = i % ry;
+    const uint32_t iu = (uint32_t)i, ryu = (uint32_t)ry, r = iu / ryu;
+    X = r < 3u ? (int64_t)r : mx + r;
+    Y = iu - r * ryu;
   } else {
     /* six consecutive threads take the six halo cells of one interior row (two runs of three adjacent elements) */
-    const int64_t k = i - nrow, q = k % 6;
-    Y = q < 3 ? q : my + q;
-    X = 3 + k / 6;
+    const uint32_t k = (uint32_t)(i - nrow), row = k / 6u, q = k - 6u * row;
+    Y = q < 3u ? (int64_t)q : my + q;
+    X = 3 + (int64_t)row;
   }
   const int64_t c = X * ps + Y + kYOff;
   {
