@@ -320,8 +320,14 @@ __device__ __forceinline__ void dring_cell(int64_t i, const int8_t *__restrict__
   if (cand) clist[base + __popcll(mask & ((1ull << lane) - 1ull))] = (int32_t)c;
 }
 
+/* a floor of waves per SIMD for k_cbulk's register allocation: the FP32 kernels of up to two mu fields fit seven
+ * (the LDS bound) with at most one spilled register; the others keep their natural allocation */
+template <typename R, int NC>
+constexpr int kCbulkWaves = (sizeof(R) == 4 && NC <= 2) ? 7 : 1;
+
 template <int N, int NC, typename R, bool P2, bool STATS, bool C = false>
-__global__ __launch_bounds__(256) void k_cbulk(const Model<R> *__restrict__ mp, const R *__restrict__ mu,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kCbulkWaves<R, NC>)))
+void k_cbulk(const Model<R> *__restrict__ mp, const R *__restrict__ mu,
                                                const int8_t *__restrict__ codes_in, R *__restrict__ phi_o,
                                                R *__restrict__ mu_o, int8_t *__restrict__ codes_out,
                                                double *__restrict__ partials, int64_t pstride, int64_t fs, int64_t ps,
