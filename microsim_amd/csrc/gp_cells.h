@@ -531,9 +531,7 @@ void k_cbulk(const Model<R> *__restrict__ mp, const R *__restrict__ mu,
  * of its proxy, as write_images scatters it (apply_boundary_conditions as a gather). Halo row X = hx[k] has the
  * proxy row sx[f][k] when that is an interior row and at most one k' < k shares it (write_images takes the first
  * two images of a row); an interior row is its own (likewise for columns). Field group f = 0 holds phi, its low
- * parts and the phase codes, f = 1 mu and its extensions. A halo cell without a proxy is left as it is. One thread
- * per halo cell: the six halo rows over the full width, then the six halo cells of each interior row. C: the
- * compensated mode, whose planes plo and mlo are copied too. */
+ * parts and the phase codes, f = 1 mu and its extensions. A halo cell without a proxy is left as it is. */
 __device__ __forceinline__ int64_t halo_source(const int32_t *src, const int32_t *h, int64_t X, int64_t lo,
                                                int64_t hi) {
   if (X >= lo && X <= hi) return X;
@@ -548,64 +546,59 @@ __device__ __forceinline__ int64_t halo_source(const int32_t *src, const int32_t
   return s;
 }
 
+/* planes of the state the halo pass copies: the phase codes, phi, (C: phi's low parts), mu, (C: mu's extensions) */
+template <int N, int NC, bool C>
+constexpr int kHaloPlanes = 1 + N + NC + (C ? N + NC : 0);
+
+/* halo cell j (the six halo rows over the full width, then the six halo cells of each interior row) of plane p */
 template <int N, int NC, typename R, bool C>
-__device__ __forceinline__ void halo_cell(int64_t i, R *__restrict__ phi, R *__restrict__ mu,
+__device__ __forceinline__ void halo_cell(int64_t j, int p, R *__restrict__ phi, R *__restrict__ mu,
                                           int8_t *__restrict__ codes, float *__restrict__ plo,
                                           int8_t *__restrict__ mlo, const Maps &maps, int64_t fs, int64_t ps,
                                           int64_t mx, int64_t my) {
   const int64_t ry = my + 6, nrow = 6 * ry, ncol = 6 * mx;
-  if (i >= nrow + ncol) return;
+  if (j >= nrow + ncol) return;
   /* the index arithmetic in 32 bits (the halo has fewer than 2^31 cells; 64-bit division is a long sequence) */
   int64_t X, Y;
-  if (i < nrow) {
-    const uint32_t iu = (uint32_t)i, ryu = (uint32_t)ry, r = iu / ryu;
+  if (j < nrow) {
+    const uint32_t iu = (uint32_t)j, ryu = (uint32_t)ry, r = iu / ryu;
     X = r < 3u ? (int64_t)r : mx + r;
     Y = iu - r * ryu;
   } else {
     /* six consecutive threads take the six halo cells of one interior row (two runs of three adjacent elements) */
-    const uint32_t k = (uint32_t)(i - nrow), row = k / 6u, q = k - 6u * row;
+    const uint32_t k = (uint32_t)(j - nrow), row = k / 6u, q = k - 6u * row;
     Y = q < 3u ? (int64_t)q : my + q;
     X = 3 + (int64_t)row;
   }
-  const int64_t c = X * ps + Y + kYOff;
-  {
-    const int64_t sx = halo_source(maps.sx[0], maps.hx, X, 3, mx + 2);
-    const int64_t sy = halo_source(maps.sy[0], maps.hy, Y, 3, my + 2);
-    if (sx >= 0 && sy >= 0) {
-      const int64_t s = sx * ps + sy + kYOff;
-      codes[c] = codes[s];
-#pragma unroll
-      for (int a = 0; a < N; a++) phi[a * fs + c] = phi[a * fs + s];
-      if constexpr (C) {
-#pragma unroll
-        for (int a = 0; a < N; a++) plo[a * fs + c] = plo[a * fs + s];
-      }
-    }
-  }
-  {
-    const int64_t sx = halo_source(maps.sx[1], maps.hx, X, 3, mx + 2);
-    const int64_t sy = halo_source(maps.sy[1], maps.hy, Y, 3, my + 2);
-    if (sx >= 0 && sy >= 0) {
-      const int64_t s = sx * ps + sy + kYOff;
-#pragma unroll
-      for (int k = 0; k < NC; k++) mu[k * fs + c] = mu[k * fs + s];
-      if constexpr (C) {
-#pragma unroll
-        for (int k = 0; k < NC; k++) mlo[k * fs + c] = mlo[k * fs + s];
-      }
-    }
+  constexpr int P0 = 1 + N + (C ? N : 0); /* planes of field group 0 (codes, phi, plo); the rest are group 1 */
+  const int f = p < P0 ? 0 : 1;
+  const int64_t sx = halo_source(maps.sx[f], maps.hx, X, 3, mx + 2);
+  const int64_t sy = halo_source(maps.sy[f], maps.hy, Y, 3, my + 2);
+  if (sx < 0 || sy < 0) return;
+  const int64_t c = X * ps + Y + kYOff, s = sx * ps + sy + kYOff;
+  if (p == 0) {
+    codes[c] = codes[s];
+  } else if (p <= N) {
+    phi[(p - 1) * fs + c] = phi[(p - 1) * fs + s];
+  } else if (C && p < P0) {
+    plo[(p - 1 - N) * fs + c] = plo[(p - 1 - N) * fs + s];
+  } else if (p < P0 + NC) {
+    mu[(p - P0) * fs + c] = mu[(p - P0) * fs + s];
+  } else if (C) {
+    mlo[(p - P0 - NC) * fs + c] = mlo[(p - P0 - NC) * fs + s];
   }
 }
 
 /* number of halo cells halo_cell covers */
 __host__ __device__ constexpr int64_t halo_cells(int64_t mx, int64_t my) { return 6 * (my + 6) + 6 * mx; }
 
+/* one thread per halo cell and plane (blockIdx.y is the plane) */
 template <int N, int NC, typename R, bool C>
 __global__ __launch_bounds__(256) void k_halo(R *__restrict__ phi, R *__restrict__ mu, int8_t *__restrict__ codes,
                                               float *__restrict__ plo, int8_t *__restrict__ mlo, Maps maps,
                                               int64_t fs, int64_t ps, int64_t mx, int64_t my) {
-  halo_cell<N, NC, R, C>((int64_t)blockIdx.x * blockDim.x + threadIdx.x, phi, mu, codes, plo, mlo, maps, fs, ps, mx,
-                         my);
+  halo_cell<N, NC, R, C>((int64_t)blockIdx.x * blockDim.x + threadIdx.x, (int)blockIdx.y, phi, mu, codes, plo, mlo,
+                         maps, fs, ps, mx, my);
 }
 
 /* The general update of the interface cells listed by k_cbulk (any order): a grid-stride loop of full

@@ -179,7 +179,7 @@ __global__ __launch_bounds__(kStatThreads) void k_stats(const double *__restrict
 
 /* The statistics fold and the halo pass of the cell-classified step in one launch (both small and latency-bound,
  * and independent: the fold reads the partials, the halo pass the new state): blocks below the number of
- * statistics fold one statistic each, the others copy kStatThreads halo cells each (k_halo). */
+ * statistics fold one statistic each, the others copy kStatThreads halo cells of one plane each (k_halo). */
 template <int N, int NC, typename R, bool C>
 __global__ __launch_bounds__(kStatThreads) void k_stats_halo(const double *__restrict__ partials, int64_t stride,
                                                              int64_t nparts, double *__restrict__ cum,
@@ -192,8 +192,9 @@ __global__ __launch_bounds__(kStatThreads) void k_stats_halo(const double *__res
     stats_fold_block<N, NC>(partials, stride, nparts, cum, (int)blockIdx.x);
     return;
   }
-  halo_cell<N, NC, R, C>((int64_t)(blockIdx.x - NS) * kStatThreads + threadIdx.x, phi, mu, codes, plo, mlo, maps, fs,
-                         ps, mx, my);
+  const int64_t nchunk = (halo_cells(mx, my) + kStatThreads - 1) / kStatThreads, q = blockIdx.x - NS;
+  halo_cell<N, NC, R, C>((q % nchunk) * kStatThreads + threadIdx.x, (int)(q / nchunk), phi, mu, codes, plo, mlo, maps,
+                         fs, ps, mx, my);
 }
 
 /* Global solute and grand-potential energy of the current state (no reference counterpart; the FP64 oracle's
@@ -414,12 +415,14 @@ static int launch_step_p(gp_handle *h, int mode) {
       if (stats) {
         ProfScope ps(h, GP_KERNEL_STATS);
         hipLaunchKernelGGL((k_stats_halo<N, NC, R, false>),
-                           dim3((unsigned)(3 * N + 3 * NC + 1 + (nh + kStatThreads - 1) / kStatThreads)),
+                           dim3((unsigned)(3 * N + 3 * NC + 1 +
+                                           kHaloPlanes<N, NC, false> * ((nh + kStatThreads - 1) / kStatThreads))),
                            dim3(kStatThreads), 0, h->stream, h->partials, cstride, cstride, h->stats_dev, phi_o, mu_o,
                            codes_o, (float *)nullptr, (int8_t *)nullptr, h->maps, h->fs, h->ps, mx, my);
       } else {
         ProfScope ps(h, GP_KERNEL_HALO);
-        hipLaunchKernelGGL((k_halo<N, NC, R, false>), dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, h->stream,
+        hipLaunchKernelGGL((k_halo<N, NC, R, false>), dim3((unsigned)((nh + 255) / 256), kHaloPlanes<N, NC, false>),
+                           dim3(256), 0, h->stream,
                            phi_o, mu_o, codes_o, (float *)nullptr, (int8_t *)nullptr, h->maps, h->fs, h->ps, mx, my);
       }
       HIPCHK(h, hipGetLastError());
@@ -755,12 +758,14 @@ static int launch_step_c(gp_handle *h, int mode) {
   if (stats) {
     ProfScope ps(h, GP_KERNEL_STATS);
     hipLaunchKernelGGL((k_stats_halo<N, NC, float, true>),
-                       dim3((unsigned)(3 * N + 3 * NC + 1 + (nh + kStatThreads - 1) / kStatThreads)),
+                       dim3((unsigned)(3 * N + 3 * NC + 1 +
+                                       kHaloPlanes<N, NC, true> * ((nh + kStatThreads - 1) / kStatThreads))),
                        dim3(kStatThreads), 0, h->stream, h->partials, cstride, cstride, h->stats_dev, phi_o, mu_o,
                        codes_o, plo_o, mlo_o, h->maps, fs, h->ps, mx, my);
   } else {
     ProfScope ps(h, GP_KERNEL_HALO);
-    hipLaunchKernelGGL((k_halo<N, NC, float, true>), dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, h->stream,
+    hipLaunchKernelGGL((k_halo<N, NC, float, true>), dim3((unsigned)((nh + 255) / 256), kHaloPlanes<N, NC, true>),
+                       dim3(256), 0, h->stream,
                        phi_o, mu_o, codes_o, plo_o, mlo_o, h->maps, fs, h->ps, mx, my);
   }
   HIPCHK(h, hipGetLastError());
