@@ -11,7 +11,9 @@ the whole grid; one cell-update is one interior cell advanced by one step.
 
 Prints one JSON line. ``value`` is cell-updates/s of the first mode in --modes with the state resident in HBM,
 every step including the reference loop's per-step statistics (``no_stats`` gives the same steps without
-them), wall clock of one gp_step call; the other modes appear under ``modes``. ``roofline`` is for the whole step:
+them), wall clock of one gp_step call of --steps steps (as a driver issues between two progress reports; the
+call ends with one evaluation of the solute and energy sums of the final state); the other modes appear under
+``modes``. ``roofline`` is for the whole step:
 the algorithmic bytes 2 (N + NC) sizeof(real) per cell-update (phi and mu read once and written once) times every
 interior cell, over the step's duration from HIP events recorded on the solver's stream around the whole call
 (nothing is recorded between its launches). ``roofline_kernels`` gives each update kernel on its own: the cells one
@@ -401,7 +403,9 @@ def oracle_threads(case_small, nthreads: int, steps: int = 3):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=100,
+                    help="steps of each timed gp_step call (a driver's call between two progress reports; the call "
+                         "ends with one evaluation of the solute/energy invariants)")
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--mesh", type=int, default=4096)
     ap.add_argument("--seeds", type=int, default=1024)
