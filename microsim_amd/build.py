@@ -91,10 +91,12 @@ def build_driver():
 
 
 def build_reconstruct():
-    """The reconstruct tool (joins the MPI layout's per-rank snapshots); host library only."""
-    _run(["gcc"] + HOST_CFLAGS + ["-I" + INC, "-o", os.path.join(LIB, "reconstruct"),
-                                  os.path.join(HERE, "host", "reconstruct.c"), "-L" + LIB, "-lgp_host",
-                                  "-Wl,-rpath,$ORIGIN", "-lm"])
+    """The MPI layout's post-processing tools on the host library: reconstruct (joins the per-rank snapshots) and
+    write_xdmf (the XDMF descriptions of the HDF5 snapshots)."""
+    for tool in ("reconstruct", "write_xdmf"):
+        _run(["gcc"] + HOST_CFLAGS + ["-I" + INC, "-o", os.path.join(LIB, tool),
+                                      os.path.join(HERE, "host", tool + ".c"), "-L" + LIB, "-lgp_host",
+                                      "-Wl,-rpath,$ORIGIN", "-lm"])
 
 
 def build_all(host_only: bool = False):

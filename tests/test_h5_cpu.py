@@ -183,3 +183,28 @@ def test_xdmf_text(name, tmp_path):
     refs = [l.strip() for l in text if l.startswith("        out_100.h5:/")]
     assert refs == ["out_100.h5:/" + a for a in want]
     assert text[-4:] == ["   </Grid>", " </Domain>", "</Xdmf>", ""]
+
+
+def test_write_xdmf_command_line(tmp_path):
+    """write_xdmf Input.in Output t_start t_end writes DATA/Output_<t>.xmf for every SAVET-th time
+    (Grand_potential_Finite_difference_2D_MPI/write_xdmf.c:20-40), each the text gp_write_xdmf writes."""
+    import shutil
+    import subprocess
+    from gp_testlib import GOLDEN
+    tool = os.path.join(_abi.LIBDIR, "write_xdmf")
+    if not os.path.exists(tool):
+        from microsim_amd import build
+        build.build_reconstruct()
+    name = "tern_aniso_per"
+    case, _ = load_golden(name)
+    shutil.copy(os.path.join(GOLDEN, name, "Input.in"), tmp_path)
+    os.makedirs(tmp_path / "DATA")
+    savet = int(case.inp.savet)
+    r = subprocess.run([tool, "Input.in", "Out", "0", str(2 * savet)], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 0, r.stderr
+    for t in (0, savet, 2 * savet):
+        ref = str(tmp_path / ("ref_%d.xmf" % t))
+        assert _abi.host_lib().gp_write_xdmf(C.byref(case.inp), ref.encode(), ("Out_%d.h5" % t).encode()) == 0
+        assert open(tmp_path / "DATA" / ("Out_%d.xmf" % t)).read() == open(ref).read()
+    assert not os.path.exists(tmp_path / "DATA" / ("Out_%d.xmf" % (3 * savet)))
