@@ -18,7 +18,7 @@ namespace gpk {
 /* TG: non-isothermal, th holds the coefficients of every allocated row (gp_frame.h) */
 /* Grid of a list-driven kernel (a grid-stride loop over a list, 256-thread blocks): the blocks the device holds at
  * once, its occupancy times the compute units, at most kListBlocks (the bound the statistics partials are sized
- * for). One round of resident wavefronts: none waits for a slot while others finish their share of the list. */
+ * for). One batch of resident wavefronts: none waits for a slot while others finish their share of the list. */
 template <class K>
 static int list_grid(const gp_handle *h, K kernel) {
   int nb = 0, cus = 0;
