@@ -59,6 +59,13 @@ def main(argv=None, summary_out: dict = None) -> int:
                     help="stop before an iteration that would end later than this many seconds after the start")
     a = ap.parse_args(argv)
     t_start = time.perf_counter()
+    # a state file resumes only the run that wrote it (checked before any device work)
+    run_key = {k: getattr(a, k) for k in ("mesh", "seeds", "steps", "iters", "lr", "lr_decay", "precision", "offset")}
+    saved = None
+    if a.state and os.path.exists(a.state):
+        saved = json.load(open(a.state))
+        if saved.get("run") != run_key:
+            raise SystemExit("calibrate: %s belongs to another run (%s)" % (a.state, saved.get("run")))
     prec = {"fp32c": _abi.GP_FP32C, "fp32": _abi.GP_FP32, "fp64": _abi.GP_FP64}[a.precision]
     case = _case(a.mesh, a.seeds)
     names = adjoint.theta_names(case.N, case.NC + 1)
@@ -89,8 +96,8 @@ def main(argv=None, summary_out: dict = None) -> int:
     adj_bytes = (ncheck + every) * state_bytes + 3 * (case.N + case.NC) * case.phi.shape[1] * case.phi.shape[2] * 8
     hist = []
     it0 = 0
-    if a.state and os.path.exists(a.state):
-        st = json.load(open(a.state))
+    if saved is not None:
+        st = saved
         with torch.no_grad():
             z.copy_(torch.tensor(st["z"], dtype=z.dtype))
         if st["adam_step"] > 0:
@@ -105,7 +112,7 @@ def main(argv=None, summary_out: dict = None) -> int:
         if not a.state:
             return
         ad = opt.state.get(z, {})
-        st = {"iter": next_it, "z": z.detach().tolist(), "lr": opt.param_groups[0]["lr"],
+        st = {"run": run_key, "iter": next_it, "z": z.detach().tolist(), "lr": opt.param_groups[0]["lr"],
               "adam_step": int(ad["step"]) if ad else 0,
               "exp_avg": ad["exp_avg"].tolist() if ad else None, "exp_avg_sq": ad["exp_avg_sq"].tolist() if ad else None,
               "hist": hist}
